@@ -1,0 +1,209 @@
+/*
+ * spx_abi.h -- C ABI of the MI355X SPH hydro + TreeGravity engine (libspheral_amd.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No C++
+ * exceptions and no torch types cross this boundary.  Every call returns 0 on
+ * success or a negative SPX_ERR_* code; spx_last_error() returns the message of
+ * the last failing call made on the calling thread.
+ *
+ * Each entry point replaces a piece of the reference's Physics-package surface
+ * for the SPH hydro (SPH + MonaghanGingoldViscosity + SPHSmoothingScale) and
+ * TreeGravity packages, and of the CheapSynchronousRK2 integrator that drives
+ * them.  The reference interface each one stands in for is cited beside it
+ * (paths relative to the reference's src/).
+ *
+ * Threading: a run is not re-entrant; calls on one run must be serialised by the
+ * caller.  Different runs may be driven from different host threads.
+ *
+ * Array layouts at the boundary follow the reference's Field storage: one
+ * record per node, vectors as nDim contiguous doubles, H as the full nDim x nDim
+ * row-major symmetric matrix, tensors as nDim x nDim row-major.
+ */
+#ifndef SPX_ABI_H
+#define SPX_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define SPX_OK               0
+#define SPX_ERR_ARG         -1   /* bad argument / inconsistent configuration   */
+#define SPX_ERR_HIP         -2   /* a HIP runtime call failed                   */
+#define SPX_ERR_STATE       -3   /* call out of order (e.g. step before state)  */
+#define SPX_ERR_UNSUPPORTED -4   /* option the GPU path does not implement      */
+#define SPX_ERR_CAPACITY    -5   /* a size limit of the engine was exceeded     */
+#define SPX_ERR_NUMERIC     -6   /* non-finite state detected on the device     */
+
+/* ---- enums ---------------------------------------------------------------- */
+/* Base kernels tabulated by TableKernel (Kernel/BSplineKernelInline.hh,
+   Kernel/WendlandC4KernelInline.hh). */
+#define SPX_KERNEL_BSPLINE     0
+#define SPX_KERNEL_WENDLANDC4  1
+
+/* Per-side boundary condition of one axis (Boundary/PeriodicBoundary.cc,
+   Boundary/ReflectingBoundary.cc).  Periodic must be set on both sides. */
+#define SPX_BC_NONE        0
+#define SPX_BC_PERIODIC    1
+#define SPX_BC_REFLECTING  2
+
+/* Arithmetic of the pair passes. FP64: everything in double.  MIXED: pair
+   arithmetic in float on cell-local offsets, double accumulation and state. */
+#define SPX_PREC_FP64   0
+#define SPX_PREC_MIXED  1
+
+/* ---- configuration -------------------------------------------------------- */
+/* One NodeList of gamma-law fluid evolved by SPH (+ optional TreeGravity) under
+   CheapSynchronousRK2.  Field meaning and defaults follow the reference:
+     kernel/kernel_points/min_nperh/max_nperh : TableKernel ctor (Kernel/TableKernel.cc:169-209)
+     nperh, neighbor_extent, hmin, hmax, rho_min, rho_max : makeFluidNodeList (NodeList/FluidNodeLists.py:15-31)
+     gamma : GammaLawGas; Cl, Cq : MonaghanGingoldViscosity
+     cfl .. correct_velocity_gradient : SPH ctor (SPH/SPH.hh:40-56, SPH/SPHHydros.py:9-33)
+     dt_min .. last_dt : Integrator members (Integrator/Integrator.cc:41-47)
+     G .. ftimestep : TreeGravity ctor (Gravity/TreeGravity.hh:46-50) */
+typedef struct spx_config {
+  int32_t dim;                       /* 1, 2 or 3                                    */
+  int32_t kernel;                    /* SPX_KERNEL_*                                 */
+  int32_t kernel_points;             /* TableKernel numPoints                        */
+  int32_t precision;                 /* SPX_PREC_*                                   */
+  double  min_nperh, max_nperh;      /* TableKernel nperh lookup range               */
+  double  nperh;                     /* NodeList nodesPerSmoothingScale              */
+  double  neighbor_extent;           /* NodeList neighbor kernelExtent               */
+  double  hmin, hmax;                /* NodeList h bounds                            */
+  double  rho_min, rho_max;          /* NodeList mass-density bounds                 */
+  double  gamma;                     /* GammaLawGas gamma                            */
+  double  Cl, Cq;                    /* MonaghanGingoldViscosity coefficients        */
+  double  cfl;
+  int32_t use_velocity_magnitude_for_dt;
+  int32_t compatible_energy;         /* must be 1 (compatibleEnergyEvolution)        */
+  int32_t gradh_correction;          /* gradhCorrection                              */
+  int32_t correct_velocity_gradient; /* correctVelocityGradient                      */
+  double  dt_min, dt_max, dt_growth; /* Integrator dtMin, dtMax, dtGrowth            */
+  double  last_dt;                   /* Integrator lastDt at start                   */
+  int32_t bc_lo[3], bc_hi[3];        /* SPX_BC_* per axis side                       */
+  double  xmin[3], xmax[3];          /* boundary plane positions per axis            */
+  int32_t gravity;                   /* 1: TreeGravity package appended              */
+  int32_t reorder_interval;          /* steps between Morton reorders (GPU layout)   */
+  double  G, softening, opening, ftimestep;
+} spx_config;
+
+typedef struct spx_run spx_run;
+
+/* ---- library / errors ---------------------------------------------------- */
+const char* spx_last_error(void);
+/* Default configuration (reference defaults for every field). */
+int spx_config_default(int32_t dim, spx_config* out);
+/* Device count visible to the library (0 => no GPU). */
+int spx_device_count(int32_t* out);
+
+/* ---- kernel table (host, FP64) --------------------------------------------
+   TableKernel<Dim>(BaseKernel, numPoints, minNperh, maxNperh)
+   (Kernel/TableKernel.cc:169-209, Utilities/QuadraticInterpolator.cc:59-100).
+   Writes W, dW/deta for eta[0..n) with Hdet=1 via kernelAndGradValue
+   (Kernel/TableKernelViewInline.hh:84-99) and the Wsum->nPerh lookup
+   equivalentNodesPerSmoothingScale (Kernel/TableKernelViewInline.hh:151-156). */
+int spx_table_eval(const spx_config* cfg, const double* eta, int64_t n,
+                   double* W, double* gW, double* nperh_of_wsum);
+/* Raw interpolator coefficients: 3*(N1+1) each for W and gradW.  *ncoef is
+   set to 3*(N1+1); pass coef arrays of at least that size (or NULL to query). */
+int spx_table_coefficients(const spx_config* cfg, int64_t* ncoef,
+                           double* wcoef, double* gwcoef, double* xstep);
+
+/* ---- run lifetime ---------------------------------------------------------
+   A run owns one NodeList's state in HBM plus the packages that evolve it:
+   SPH (SPH/SPH.cc:43-80) with MonaghanGingoldViscosity and SPHSmoothingScale
+   (SPH/SPHHydros.py:88-140), optional TreeGravity, and CheapSynchronousRK2. */
+int  spx_run_create(const spx_config* cfg, int64_t n, int32_t device, spx_run** out);
+void spx_run_destroy(spx_run* run);
+
+/* Upload a full state (host pointers; x,v: n*dim; m,rho,eps: n; H: n*dim*dim).
+   Replaces NodeList positions/velocity/mass/massDensity/specificThermalEnergy/Hfield. */
+int spx_run_set_state(spx_run* run, const double* x, const double* v,
+                      const double* m, const double* rho, const double* eps,
+                      const double* H);
+
+/* Copy one field to host in the caller's original node order.  Names follow
+   HydroFieldNames (Hydro/HydroFieldNames.cc:11-40): "position", "velocity",
+   "mass", "mass density", "specific thermal energy", "H", "pressure",
+   "sound speed", "grad h corrections", "delta velocity hydro" (hydro DvDt),
+   "delta velocity" (gravity DvDt), "delta position", "delta specific thermal energy",
+   "delta mass density", "velocity gradient", "delta H", "new H",
+   "max viscous pressure", "effective viscous pressure", "mass zeroth moment",
+   "mass first moment", "M SPH correction", "mass density gradient",
+   "normalization", "new mass density", "gravitational potential".
+   *count receives the number of doubles written (n * components). */
+int spx_run_get_field(spx_run* run, const char* name, double* out, int64_t capacity,
+                      int64_t* count);
+
+/* SpheralController.reinitializeProblem subset (SimulationControl/SpheralController.py:154-300):
+   ghost images + connectivity, then initializeProblemStartupDependencies of every
+   package: P and cs (SPH/SPHBase.cc:147-161), grad-h Omega, TreeGravity dry run
+   (Gravity/TreeGravity.cc:343-356).  Derivatives start at zero. */
+int spx_run_initialize(spx_run* run);
+
+/* Integrator::step(maxTime) with CheapSynchronousRK2::step
+   (Integrator/Integrator.cc:67-111, Integrator/CheapSynchronousRK2.cc:40-132).
+   *dt_out receives the step taken.  Device work is enqueued on the run's stream;
+   the call returns after the step's dt has been read back. */
+int spx_run_step(spx_run* run, double max_time, double* dt_out);
+
+/* Advance nsteps steps (or until max_time) without per-step host reads of dt
+   where possible; *steps_done, *t_out report progress. */
+int spx_run_advance(spx_run* run, double max_time, int64_t nsteps,
+                    int64_t* steps_done, double* t_out);
+
+int spx_run_time(spx_run* run, double* t, int64_t* cycle, double* last_dt);
+
+/* ---- package-level entry points (Physics interface, Physics/Physics.hh:49-73) --- */
+/* ConnectivityMap rebuild from the current positions and H
+   (Neighbor/ConnectivityMap.cc:750-1152 via Integrator::setGhostNodes). */
+int spx_run_update_connectivity(spx_run* run);
+/* Number of neighbours (gather+scatter) per node, original order
+   (ConnectivityMap::numNeighborsForNode). */
+int spx_run_neighbor_counts(spx_run* run, int32_t* counts);
+/* Neighbour sets: for node i (original order) entries offsets[i]..offsets[i+1]
+   hold (neighbour original index, image code); image code packs per axis 2 bits
+   (0 none, 1 image beyond the low plane, 2 beyond the high plane), axis a at
+   bits 2a..2a+1.  Each node's list is sorted by (index, image).  offsets has n+1
+   entries; nbr/img need capacity total = offsets[n]. */
+int spx_run_neighbors(spx_run* run, int64_t* offsets, int64_t* nbr, int32_t* img,
+                      int64_t capacity);
+/* SPHBase::preStepInitialize with RigorousSumDensity: mass density <- SPH sum
+   (SPH/computeSPHSumMassDensity.cc, SPH/SPHBase.cc:336-353). */
+int spx_run_sum_density(spx_run* run);
+/* EOS policies on the current state: pressure, sound speed
+   (Hydro/PressurePolicy.cc, Hydro/SoundSpeedPolicy.cc, Material/GammaLawGas.cc). */
+int spx_run_update_eos(spx_run* run);
+/* SPHBase::postStateUpdate: grad-h Omega (SPH/computeSPHOmegaGradhCorrection.cc). */
+int spx_run_post_state_update(spx_run* run);
+/* SPH::evaluateDerivatives + SPHSmoothingScale::evaluateDerivatives (+ TreeGravity
+   initialize/evaluateDerivatives if enabled) on the current state
+   (SPH/SPH.cc:165-555, SmoothingScale/SPHSmoothingScale.cc:101-275,
+   Gravity/TreeGravity.cc:143-328).  Derivatives are zeroed first. */
+int spx_run_evaluate_derivatives(spx_run* run, double t, double dt);
+/* Physics::dt votes: GenericHydro::dt (Physics/GenericHydro.cc:112-381, includes the
+   cfl factor) and TreeGravity::dt (Gravity/TreeGravity.cc:483-497). */
+int spx_run_dt(spx_run* run, double* hydro_dt, double* gravity_dt);
+/* SpecificThermalEnergyPolicy::update (Hydro/SpecificThermalEnergyPolicy.cc:50-175):
+   eps <- eps + multiplier * compatible DepsDt, using the current derivatives. */
+int spx_run_compatible_energy(spx_run* run, double multiplier);
+
+/* ---- diagnostics --------------------------------------------------------- */
+/* SpheralConservation totals (SimulationControl/SpheralConservation.py:37-150):
+   out[0]=mass, out[1..3]=linear momentum, out[4]=kinetic, out[5]=thermal,
+   out[6]=gravitational extra energy (0.5 sum m phi). */
+int spx_run_conservation(spx_run* run, double* out7);
+/* Device time of the last step's kernels, per kernel class (ms), measured with
+   HIP events on the run's stream.  names: comma-separated labels written to buf. */
+int spx_run_kernel_times(spx_run* run, double* ms, int32_t max_entries,
+                         int32_t* n_entries, char* names, int64_t names_cap);
+/* Unique pairs of the current connectivity (NodePairList size). */
+int spx_run_num_pairs(spx_run* run, int64_t* npairs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPX_ABI_H */

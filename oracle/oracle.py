@@ -1,0 +1,197 @@
+"""ctypes front end of the oracle (``oracle/spx_oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, always as the checker or the CPU baseline, never as
+part of the engine.  Build with ``make -C oracle`` (done by __graft_entry__.build()).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from spheral_amd.abi import SpxConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libspx_oracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.POINTER(C.c_double)
+        I64 = C.POINTER(C.c_int64)
+        I32 = C.POINTER(C.c_int32)
+        L.spxo_create.restype = P
+        L.spxo_create.argtypes = [C.POINTER(SpxConfig), C.c_int64]
+        L.spxo_destroy.argtypes = [P]
+        L.spxo_set_state.argtypes = [P, D, D, D, D, D, D]
+        for name in ("spxo_initialize", "spxo_update_connectivity", "spxo_sum_density", "spxo_update_eos",
+                     "spxo_post_state_update", "spxo_evaluate_derivatives"):
+            getattr(L, name).argtypes = [P]
+        L.spxo_dt.argtypes = [P, D, D]
+        L.spxo_compatible_energy.argtypes = [P, C.c_double]
+        L.spxo_step.argtypes = [P, C.c_double, D]
+        L.spxo_time.argtypes = [P, D, I64, D]
+        L.spxo_num_pairs.restype = C.c_int64
+        L.spxo_num_pairs.argtypes = [P]
+        L.spxo_num_ghosts.restype = C.c_int64
+        L.spxo_num_ghosts.argtypes = [P]
+        L.spxo_neighbors.restype = C.c_int64
+        L.spxo_neighbors.argtypes = [P, I64, I64, I32, C.c_int64]
+        L.spxo_get_field.restype = C.c_int64
+        L.spxo_get_field.argtypes = [P, C.c_char_p, D]
+        L.spxo_table_eval.argtypes = [C.POINTER(SpxConfig), D, C.c_int64, D, D, D]
+        L.spxo_table_coefficients.argtypes = [C.POINTER(SpxConfig), I64, D, D, D]
+        L.spxo_base_kernel.argtypes = [C.c_int, C.c_int, D, C.c_int64, D, D]
+        L.spxo_conservation.argtypes = [P, D]
+        L.spxo_last_error.restype = C.c_char_p
+        L.spxo_set_threads.argtypes = [C.c_int]
+        L.spxo_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def set_threads(n):
+    lib().spxo_set_threads(int(n))
+
+
+def max_threads():
+    return lib().spxo_max_threads()
+
+
+FIELD_COMPONENTS = {
+    "position": "v", "velocity": "v", "mass": "s", "mass density": "s", "specific thermal energy": "s",
+    "H": "t", "pressure": "s", "sound speed": "s", "grad h corrections": "s", "delta velocity hydro": "v",
+    "delta velocity": "v", "delta position": "v", "delta specific thermal energy": "s",
+    "delta mass density": "s", "velocity gradient": "t", "delta H": "t", "new H": "t",
+    "max viscous pressure": "s", "effective viscous pressure": "s", "mass zeroth moment": "s",
+    "mass first moment": "v", "M SPH correction": "t", "mass density gradient": "v",
+    "normalization": "s", "new mass density": "s", "gravitational potential": "s",
+}
+
+
+def field_shape(name, n, dim):
+    k = FIELD_COMPONENTS[name]
+    return (n,) if k == "s" else (n, dim) if k == "v" else (n, dim, dim)
+
+
+class OracleRun:
+    """The oracle driven through the same sequence as the engine's C ABI."""
+
+    def __init__(self, cfg: SpxConfig, n):
+        self.cfg = cfg.copy()
+        self.n = int(n)
+        self.dim = cfg.dim
+        self.h = lib().spxo_create(C.byref(self.cfg), self.n)
+        if not self.h:
+            raise RuntimeError(lib().spxo_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().spxo_destroy(self.h)
+            self.h = None
+
+    def set_state(self, x, v, m, rho, eps, H):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, v, m, rho, eps, H)]
+        self._keep = arrs
+        lib().spxo_set_state(self.h, *[_dp(a) for a in arrs])
+
+    def initialize(self):
+        lib().spxo_initialize(self.h)
+
+    def update_connectivity(self):
+        lib().spxo_update_connectivity(self.h)
+
+    def sum_density(self):
+        lib().spxo_sum_density(self.h)
+
+    def update_eos(self):
+        lib().spxo_update_eos(self.h)
+
+    def post_state_update(self):
+        lib().spxo_post_state_update(self.h)
+
+    def evaluate_derivatives(self):
+        lib().spxo_evaluate_derivatives(self.h)
+
+    def dt(self):
+        a, b = C.c_double(), C.c_double()
+        lib().spxo_dt(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def compatible_energy(self, multiplier):
+        lib().spxo_compatible_energy(self.h, float(multiplier))
+
+    def step(self, max_time=1e300):
+        dt = C.c_double()
+        lib().spxo_step(self.h, float(max_time), C.byref(dt))
+        return dt.value
+
+    def time(self):
+        t, c, l = C.c_double(), C.c_int64(), C.c_double()
+        lib().spxo_time(self.h, C.byref(t), C.byref(c), C.byref(l))
+        return t.value, c.value, l.value
+
+    def num_pairs(self):
+        return lib().spxo_num_pairs(self.h)
+
+    def num_ghosts(self):
+        return lib().spxo_num_ghosts(self.h)
+
+    def neighbors(self):
+        total = lib().spxo_neighbors(self.h, None, None, None, 0)
+        off = np.zeros(self.n + 1, np.int64)
+        nbr = np.zeros(max(total, 1), np.int64)
+        img = np.zeros(max(total, 1), np.int32)
+        lib().spxo_neighbors(self.h, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                             nbr.ctypes.data_as(C.POINTER(C.c_int64)),
+                             img.ctypes.data_as(C.POINTER(C.c_int32)), total)
+        return off, nbr[:total], img[:total]
+
+    def field(self, name):
+        out = np.zeros(field_shape(name, self.n, self.dim), np.float64)
+        cnt = lib().spxo_get_field(self.h, name.encode(), _dp(out))
+        if cnt < 0:
+            raise KeyError(name)
+        return out
+
+    def conservation(self):
+        out = np.zeros(7)
+        lib().spxo_conservation(self.h, _dp(out))
+        return out
+
+
+def table_eval(cfg, eta):
+    eta = np.ascontiguousarray(eta, np.float64)
+    W = np.zeros_like(eta)
+    gW = np.zeros_like(eta)
+    nph = np.zeros_like(eta)
+    lib().spxo_table_eval(C.byref(cfg), _dp(eta), eta.size, _dp(W), _dp(gW), _dp(nph))
+    return W, gW, nph
+
+
+def table_coefficients(cfg):
+    n = C.c_int64()
+    lib().spxo_table_coefficients(C.byref(cfg), C.byref(n), None, None, None)
+    wc = np.zeros(n.value)
+    gwc = np.zeros(n.value)
+    xs = C.c_double()
+    lib().spxo_table_coefficients(C.byref(cfg), C.byref(n), _dp(wc), _dp(gwc), C.byref(xs))
+    return wc, gwc, xs.value
+
+
+def base_kernel(kind, dim, eta):
+    eta = np.ascontiguousarray(eta, np.float64)
+    W = np.zeros_like(eta)
+    gW = np.zeros_like(eta)
+    lib().spxo_base_kernel(kind, dim, _dp(eta), eta.size, _dp(W), _dp(gW))
+    return W, gW

@@ -1,0 +1,199 @@
+// spx_device.hpp -- device-side types and helpers shared by the engine's kernels.
+//
+// Data layout in HBM: structure of arrays, one double array per scalar component,
+// particles in cell-Morton order (slot p holds original node oid[p]).  H is
+// isotropic (H = hinv * I, as SPH + IdealH keep it), so one double per node.
+// Neighbour lists are ELLPACK: entry k of slot i at nbr[k*ld + i] (ld >= n), a
+// uint32 holding the neighbour slot in the low 26 bits and a 6-bit image code
+// (2 bits per axis: 0 none, 1 beyond the low plane, 2 beyond the high plane).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spx {
+
+constexpr int kImgShift = 26;
+constexpr uint32_t kIdxMask = (1u << kImgShift) - 1u;
+constexpr int kBlock = 256;
+
+enum Bc { BC_NONE = 0, BC_PERIODIC = 1, BC_REFLECTING = 2 };
+
+// Tabulated kernel: per interval 6 coefficients (W: c0 c1 c2, dW: c0 c1 c2),
+// plus the Wsum -> nPerh cubic Hermite table (N values then N knot gradients).
+struct DevTable {
+  const double* coef;
+  const double* herm;
+  int n1;
+  int hn;
+  double inv_xstep, extent, W0;
+  double hxmin, hxmax, hxstep;
+};
+
+// Boundary planes and the cell grid of this step.
+struct Geo {
+  int bc_lo[3], bc_hi[3];
+  double xmin[3], xmax[3];
+  double origin[3];
+  double cell[3], inv_cell[3];
+  int nc[3];
+};
+
+// Physics constants of the run.
+struct Phys {
+  double gamma, g1, Cl, Cq, cfl, nperh, ext2, hmin, hmax, rho_min, rho_max;
+  double hminInv, hmaxInv;
+  int correctVG, useVmagDt, gravity;
+};
+
+struct Fields {
+  // state (start of step, then end of step)
+  double *x[3], *v[3], *m, *rho, *eps, *hinv, *P, *cs, *omega;
+  // midpoint state
+  double *xm[3], *vm[3], *rhom, *epsm, *hinvm, *Pm, *csm, *omegam;
+  // end-of-step velocity (swapped with v after the full update)
+  double *v1[3];
+  // derivatives carried across steps
+  double *DxDt[3], *DvDt[3], *DvDtG[3], *DepsDt, *DrhoDt, *DHDt, *DvDx, *maxQ;
+  // derivatives consumed within a step
+  double *Hideal, *m0;
+  // evaluateDerivatives outputs not consumed by the integration
+  double *rhoSum, *norm, *effQ, *gradRho[3], *m1[3], *Mcorr;
+  double *ns;        // nodeScale = 1/(lambda_max(H) nPerh) of the connectivity state
+  double *pot;       // gravitational potential
+  uint32_t* nbr;
+  int32_t* cnt;
+  int64_t* oid;
+  int ld, cap;
+};
+
+// Device scalars of a run (one struct in device memory).
+struct Scalars {
+  double t, lastDt, dt, hdt;
+  double hydroDt, gravDt, dtMinAcc, extraEnergy;
+  long long cycle;
+  int overflow, pad;
+};
+
+__device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double dsub(double a, double b) { return __dsub_rn(a, b); }
+
+// Ghost coordinate of an image along one axis, written in the reference's
+// mapPositionThroughPlanes arithmetic for axis-aligned planes through the origin of
+// the other axes (Boundary/mapPositionThroughPlanes.hh:17-25, Geometry/GeomPlane.cc:240-259,
+// Utilities/pointDistances.hh:69-74).  Adds only: no contraction possible.
+template <int D>
+__device__ __forceinline__ double image_coord(int bc, int code, double x, double lo, double hi) {
+  if (bc == BC_PERIODIC) {
+    if (code == 2) {
+      if (D == 3) { const double cl = dadd(x, dsub(hi, x)); return dadd(cl, dsub(x, lo)); }
+      return dadd(hi, dsub(x, lo));
+    }
+    if (D == 3) { const double cl = dadd(x, dsub(lo, x)); return dadd(cl, dsub(x, hi)); }
+    return dadd(lo, dsub(x, hi));
+  }
+  if (code == 1) {
+    if (D == 3) { const double cl = dadd(x, dsub(lo, x)); return dsub(cl, dsub(x, lo)); }
+    return dsub(lo, dsub(x, lo));
+  }
+  if (D == 3) { const double cl = dadd(x, dsub(hi, x)); return dsub(cl, dsub(x, hi)); }
+  return dsub(hi, dsub(x, hi));
+}
+
+template <int D>
+__device__ __forceinline__ double hdet_iso(double h) {
+  // GeomSymmetricTensor Determinant with zero off-diagonals
+  if (D == 1) return h;
+  if (D == 2) return dmul(h, h);
+  return dmul(dmul(h, h), h);
+}
+
+// Largest eigenvalue of hinv * I by the reference's eigenValues formulas
+// (Geometry/GeomSymmetricTensorInline.hh:2197-2256).
+template <int D>
+__device__ __forceinline__ double max_eigen_iso(double h) {
+  if (D < 3) return h;
+  const double fscale = fmax(10.0*2.220446049250313e-16, fabs(h));
+  const double fi = 1.0/fscale;
+  const double a = h*fi;
+  const double onethird = 1.0/3.0;
+  const double c0 = a*a*a;
+  const double c1 = a*a + a*a + a*a;
+  const double c2 = a + a + a;
+  const double c2Div3 = c2*onethird;
+  const double aDiv3 = fmin(0.0, onethird*(c1 - c2*c2Div3));
+  const double mbDiv2 = 0.5*(c0 + c2Div3*(2.0*c2Div3*c2Div3 - c1));
+  const double q = fmin(0.0, mbDiv2*mbDiv2 + aDiv3*aDiv3*aDiv3);
+  const double mag = sqrt(-aDiv3);
+  const double ang = atan2(sqrt(-q), mbDiv2)*onethird;
+  const double cs = cos(ang), sn = sin(ang);
+  const double s3 = 1.7320508075688772;
+  const double e0 = fscale*(c2Div3 + 2.0*mag*cs);
+  const double e1 = fscale*(c2Div3 - mag*(cs + s3*sn));
+  const double e2 = fscale*(c2Div3 - mag*(cs - s3*sn));
+  return fmax(e0, fmax(e1, e2));
+}
+
+// Reference rdiv: Tensor/Vector /= s multiplies by 1/s in 2-D and 3-D, divides in 1-D.
+template <int D>
+__device__ __forceinline__ double rdiv(double x, double s) { return D == 1 ? x/s : x*(1.0/s); }
+
+__device__ __forceinline__ void kernel_wgw(const DevTable& t, double eta, double hdet, double& w, double& gw) {
+  if (eta < t.extent) {
+    int b = (int)(eta*t.inv_xstep);
+    b = b < t.n1 ? b : t.n1;
+    const double* c = t.coef + 6*b;
+    w = hdet*(c[0] + (c[1] + c[2]*eta)*eta);
+    gw = hdet*(c[3] + (c[4] + c[5]*eta)*eta);
+  } else {
+    w = 0.0;
+    gw = 0.0;
+  }
+}
+
+// kernelValueSPH: |dW/deta| at Hdet = 1 (Kernel/TableKernelViewInline.hh:112-120)
+__device__ __forceinline__ double kernel_wsph(const DevTable& t, double eta) {
+  if (eta < t.extent) {
+    int b = (int)(eta*t.inv_xstep);
+    b = b < t.n1 ? b : t.n1;
+    const double* c = t.coef + 6*b;
+    return fabs(c[3] + (c[4] + c[5]*eta)*eta);
+  }
+  return 0.0;
+}
+
+// equivalentNodesPerSmoothingScale: cubic Hermite lookup (CubicHermiteInterpolatorViewInline.hh:9-33,120-125)
+__device__ __forceinline__ double nperh_lookup(const DevTable& t, double x) {
+  const double* v = t.herm;
+  const int N = t.hn;
+  double r;
+  if (x < t.hxmin) {
+    r = v[0] + v[N]*(x - t.hxmin);
+  } else if (x > t.hxmax) {
+    r = v[N - 1] + v[2*N - 1]*(x - t.hxmin);
+  } else {
+    int i0 = (int)(fmax(0.0, x - t.hxmin)/t.hxstep);
+    i0 = i0 < N - 2 ? i0 : N - 2;
+    const double tt = fmax(0.0, fmin(1.0, (x - t.hxmin - i0*t.hxstep)/t.hxstep));
+    const double t2 = tt*tt, t3 = tt*t2;
+    r = ((2.0*t3 - 3.0*t2 + 1.0)*v[i0] + (-2.0*t3 + 3.0*t2)*v[i0 + 1] +
+         t.hxstep*((t3 - 2.0*t2 + tt)*v[N + i0] + (t3 - t2)*v[N + i0 + 1]));
+  }
+  return fmax(0.0, r);
+}
+
+template <int D>
+__device__ __forceinline__ double rootnu(double x) {
+  if (D == 1) return x;
+  if (D == 2) return sqrt(x);
+  return (x < 0.0 ? -1.0 : 1.0)*pow(fabs(x), 0.3333333333333333);
+}
+
+__device__ __forceinline__ uint64_t spread_bits(uint64_t v, int D) {
+  uint64_t r = 0;
+  for (int b = 0; b < 21; ++b) r |= ((v >> b) & 1ull) << (D*b);
+  return r;
+}
+
+}  // namespace spx

@@ -1,0 +1,224 @@
+"""ctypes binding of libspheral_amd.so (the C ABI in include/spx_abi.h).
+
+The engine runs only on the GPU: if the library is missing or no HIP device is
+visible, ``lib()`` / ``Run(...)`` raise -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import SpxConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspheral_amd.so")
+
+# Every C entry point declared by include/spx_abi.h (checked by the CPU test suite).
+EXPORTS = (
+    "spx_last_error", "spx_config_default", "spx_device_count", "spx_table_eval", "spx_table_coefficients",
+    "spx_run_create", "spx_run_destroy", "spx_run_set_state", "spx_run_get_field", "spx_run_initialize",
+    "spx_run_step", "spx_run_advance", "spx_run_time", "spx_run_update_connectivity", "spx_run_neighbor_counts",
+    "spx_run_neighbors", "spx_run_sum_density", "spx_run_update_eos", "spx_run_post_state_update",
+    "spx_run_evaluate_derivatives", "spx_run_dt", "spx_run_compatible_energy", "spx_run_conservation",
+    "spx_run_kernel_times", "spx_run_num_pairs",
+)
+
+_lib = None
+
+
+class SpxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"spx error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m spheral_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.POINTER(C.c_double)
+        I64 = C.POINTER(C.c_int64)
+        I32 = C.POINTER(C.c_int32)
+        cfgp = C.POINTER(SpxConfig)
+        sig = {
+            "spx_last_error": (C.c_char_p, []),
+            "spx_config_default": (C.c_int, [C.c_int32, cfgp]),
+            "spx_device_count": (C.c_int, [I32]),
+            "spx_table_eval": (C.c_int, [cfgp, D, C.c_int64, D, D, D]),
+            "spx_table_coefficients": (C.c_int, [cfgp, I64, D, D, D]),
+            "spx_run_create": (C.c_int, [cfgp, C.c_int64, C.c_int32, C.POINTER(P)]),
+            "spx_run_destroy": (None, [P]),
+            "spx_run_set_state": (C.c_int, [P, D, D, D, D, D, D]),
+            "spx_run_get_field": (C.c_int, [P, C.c_char_p, D, C.c_int64, I64]),
+            "spx_run_initialize": (C.c_int, [P]),
+            "spx_run_step": (C.c_int, [P, C.c_double, D]),
+            "spx_run_advance": (C.c_int, [P, C.c_double, C.c_int64, I64, D]),
+            "spx_run_time": (C.c_int, [P, D, I64, D]),
+            "spx_run_update_connectivity": (C.c_int, [P]),
+            "spx_run_neighbor_counts": (C.c_int, [P, I32]),
+            "spx_run_neighbors": (C.c_int, [P, I64, I64, I32, C.c_int64]),
+            "spx_run_sum_density": (C.c_int, [P]),
+            "spx_run_update_eos": (C.c_int, [P]),
+            "spx_run_post_state_update": (C.c_int, [P]),
+            "spx_run_evaluate_derivatives": (C.c_int, [P, C.c_double, C.c_double]),
+            "spx_run_dt": (C.c_int, [P, D, D]),
+            "spx_run_compatible_energy": (C.c_int, [P, C.c_double]),
+            "spx_run_conservation": (C.c_int, [P, D]),
+            "spx_run_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
+            "spx_run_num_pairs": (C.c_int, [P, I64]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        raise SpxError(code, lib().spx_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def device_count():
+    n = C.c_int32(0)
+    lib().spx_device_count(C.byref(n))
+    return n.value
+
+
+FIELD_KIND = {
+    "position": "v", "velocity": "v", "mass": "s", "mass density": "s", "specific thermal energy": "s",
+    "H": "t", "pressure": "s", "sound speed": "s", "grad h corrections": "s", "delta velocity hydro": "v",
+    "delta velocity": "v", "delta position": "v", "delta specific thermal energy": "s",
+    "delta mass density": "s", "velocity gradient": "t", "delta H": "t", "new H": "t",
+    "max viscous pressure": "s", "effective viscous pressure": "s", "mass zeroth moment": "s",
+    "mass first moment": "v", "M SPH correction": "t", "mass density gradient": "v",
+    "normalization": "s", "new mass density": "s", "gravitational potential": "s",
+}
+
+
+class Run:
+    """One resident simulation (spx_run) on a GPU."""
+
+    def __init__(self, cfg: SpxConfig, n, device=0):
+        self.cfg = cfg.copy()
+        self.n = int(n)
+        self.dim = cfg.dim
+        h = C.c_void_p()
+        _check(lib().spx_run_create(C.byref(self.cfg), self.n, int(device), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spx_run_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, x, v, m, rho, eps, H):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, v, m, rho, eps, H)]
+        _check(lib().spx_run_set_state(self.h, *[_dp(a) for a in arrs]))
+
+    def field(self, name):
+        k = FIELD_KIND[name]
+        shape = (self.n,) if k == "s" else (self.n, self.dim) if k == "v" else (self.n, self.dim, self.dim)
+        out = np.zeros(shape, np.float64)
+        cnt = C.c_int64()
+        _check(lib().spx_run_get_field(self.h, name.encode(), _dp(out), out.size, C.byref(cnt)))
+        return out
+
+    def initialize(self):
+        _check(lib().spx_run_initialize(self.h))
+
+    def step(self, max_time=1.0e300):
+        dt = C.c_double()
+        _check(lib().spx_run_step(self.h, float(max_time), C.byref(dt)))
+        return dt.value
+
+    def advance(self, max_time, nsteps):
+        k = C.c_int64()
+        t = C.c_double()
+        _check(lib().spx_run_advance(self.h, float(max_time), int(nsteps), C.byref(k), C.byref(t)))
+        return k.value, t.value
+
+    def time(self):
+        t, c, l = C.c_double(), C.c_int64(), C.c_double()
+        _check(lib().spx_run_time(self.h, C.byref(t), C.byref(c), C.byref(l)))
+        return t.value, c.value, l.value
+
+    def update_connectivity(self):
+        _check(lib().spx_run_update_connectivity(self.h))
+
+    def neighbor_counts(self):
+        out = np.zeros(self.n, np.int32)
+        _check(lib().spx_run_neighbor_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def neighbors(self):
+        counts = self.neighbor_counts()
+        total = int(counts.sum())
+        off = np.zeros(self.n + 1, np.int64)
+        nbr = np.zeros(max(total, 1), np.int64)
+        img = np.zeros(max(total, 1), np.int32)
+        _check(lib().spx_run_neighbors(self.h, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       nbr.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       img.ctypes.data_as(C.POINTER(C.c_int32)), total))
+        return off, nbr[:total], img[:total]
+
+    def sum_density(self):
+        _check(lib().spx_run_sum_density(self.h))
+
+    def update_eos(self):
+        _check(lib().spx_run_update_eos(self.h))
+
+    def post_state_update(self):
+        _check(lib().spx_run_post_state_update(self.h))
+
+    def evaluate_derivatives(self, t=0.0, dt=0.0):
+        _check(lib().spx_run_evaluate_derivatives(self.h, float(t), float(dt)))
+
+    def dt(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().spx_run_dt(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def compatible_energy(self, multiplier):
+        _check(lib().spx_run_compatible_energy(self.h, float(multiplier)))
+
+    def conservation(self):
+        out = np.zeros(7)
+        _check(lib().spx_run_conservation(self.h, _dp(out)))
+        return out
+
+    def num_pairs(self):
+        n = C.c_int64()
+        _check(lib().spx_run_num_pairs(self.h, C.byref(n)))
+        return n.value
+
+    def enable_timing(self):
+        _check(lib().spx_run_kernel_times(self.h, None, -1, None, None, 0))
+
+    def kernel_times(self):
+        ms = np.zeros(32)
+        k = C.c_int32()
+        buf = C.create_string_buffer(1024)
+        _check(lib().spx_run_kernel_times(self.h, _dp(ms), 32, C.byref(k), buf, 1024))
+        names = buf.value.decode().split(",") if k.value else []
+        return dict(zip(names, ms[:k.value].tolist()))
+
+
+def make_run(cfg, state, device=0):
+    r = Run(cfg, len(state["m"]), device)
+    r.set_state(state["x"], state["v"], state["m"], state["rho"], state["eps"], state["H"])
+    return r

@@ -1,0 +1,158 @@
+"""Synthetic problem generators for the workloads the engine is measured on.
+
+Each returns ``(cfg, state)`` with ``state`` a dict of host numpy arrays in the
+reference's Field record layout (x, v: (n, dim); m, rho, eps: (n,); H: (n, dim, dim)).
+Lattice H follows GenerateNodeDistribution3d (NodeGenerators/GenerateNodeDistribution3d.py:585-597):
+H = I/(nPerh*dx).  Seeds use numpy's PCG64 so the same seed gives the same inputs on
+any host.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+
+
+def _iso_H(n, dim, hinv):
+    H = np.zeros((n, dim, dim))
+    hinv = np.broadcast_to(np.asarray(hinv, np.float64), (n,))
+    for a in range(dim):
+        H[:, a, a] = hinv
+    return H
+
+
+def taylor_green_3d(n1d=128, jitter=0.1, seed=12345, nperh=2.0, rho0=1.0, P0=100.0, gamma=5.0 / 3.0,
+                    vamp=0.1, kernel=abi.SPX_KERNEL_WENDLANDC4, precision=abi.SPX_PREC_FP64):
+    """3-D periodic Taylor-Green box: n1d^3 equal-mass particles on a unit-cube lattice
+    with uniform jitter of +-jitter spacing, rho=1, P=100, gamma=5/3,
+    v = vamp*(sin X cos Y cos Z, -cos X sin Y cos Z, 0) with X = 2 pi x."""
+    dx = 1.0 / n1d
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = (np.arange(n1d) + 0.5) * dx
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    x += rng.uniform(-jitter, jitter, size=x.shape) * dx
+    x = np.mod(x, 1.0)
+    n = x.shape[0]
+    tp = 2.0 * math.pi
+    v = np.zeros_like(x)
+    v[:, 0] = vamp * np.sin(tp * x[:, 0]) * np.cos(tp * x[:, 1]) * np.cos(tp * x[:, 2])
+    v[:, 1] = -vamp * np.cos(tp * x[:, 0]) * np.sin(tp * x[:, 1]) * np.cos(tp * x[:, 2])
+    m = np.full(n, rho0 * dx ** 3)
+    rho = np.full(n, rho0)
+    eps = np.full(n, P0 / ((gamma - 1.0) * rho0))
+    H = _iso_H(n, 3, 1.0 / (nperh * dx))
+    cfg = abi.default_config(3, kernel=kernel, nperh=nperh, gamma=gamma, precision=precision)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    abi.set_periodic(cfg, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    return cfg, dict(x=x, v=v, m=m, rho=rho, eps=eps, H=H)
+
+
+def sod_1d(n=400, nperh=1.51, gamma=1.4, rho1=1.0, P1=1.0, rho2=0.125, P2=0.1, x0=-0.5, x1=0.0, x2=0.5,
+           kernel=abi.SPX_KERNEL_BSPLINE):
+    """1-D Sod tube, n equal-mass particles on [x0, x2] placed by inverting the mass
+    coordinate of the step profile (as GenerateNodeProfile1d does), reflecting walls."""
+    Mleft = rho1 * (x1 - x0)
+    Mright = rho2 * (x2 - x1)
+    Mtot = Mleft + Mright
+    mi = Mtot / n
+    mc = (np.arange(n) + 0.5) * mi
+    x = np.where(mc < Mleft, x0 + mc / rho1, x1 + (mc - Mleft) / rho2)
+    rho = np.where(x < x1, rho1, rho2)
+    P = np.where(x < x1, P1, P2)
+    eps = P / ((gamma - 1.0) * rho)
+    dxl = mi / rho
+    H = _iso_H(n, 1, 1.0 / (nperh * dxl))
+    cfg = abi.default_config(1, kernel=kernel, nperh=nperh, gamma=gamma)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    cfg.bc_lo[0] = abi.SPX_BC_REFLECTING
+    cfg.bc_hi[0] = abi.SPX_BC_REFLECTING
+    cfg.xmin[0] = x0
+    cfg.xmax[0] = x2
+    cfg.last_dt = 1.0e-6
+    cfg.dt_min = 1.0e-8
+    return cfg, dict(x=x.reshape(n, 1), v=np.zeros((n, 1)), m=np.full(n, mi), rho=rho, eps=eps, H=H)
+
+
+def kelvin_helmholtz_2d(n_target=4096, seed=7, nperh=2.0, gamma=5.0 / 3.0, P0=2.5,
+                        kernel=abi.SPX_KERNEL_BSPLINE, precision=abi.SPX_PREC_MIXED):
+    """2-D periodic KH: lattice of spacing sqrt(1/N) with the band |y-0.5|<0.25 at twice the
+    particle density (rho=2, vx=+0.5), outside rho=1, vx=-0.5; P=2.5; vy=0.01 sin(4 pi x)."""
+    nx = max(4, int(round(math.sqrt(n_target / 1.5))))
+    dx = 1.0 / nx
+    g = (np.arange(nx) + 0.5) * dx
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    outer = np.abs(Y.ravel() - 0.5) >= 0.25
+    xo = np.stack([X.ravel()[outer], Y.ravel()[outer]], 1)
+    # inner band: spacing dx/sqrt(2)
+    ni = int(round(nx * math.sqrt(2.0)))
+    dxi = 1.0 / ni
+    gi = (np.arange(ni) + 0.5) * dxi
+    nyi = int(round(0.5 / dxi))
+    gy = 0.25 + (np.arange(nyi) + 0.5) * (0.5 / nyi)
+    Xi, Yi = np.meshgrid(gi, gy, indexing="ij")
+    xi = np.stack([Xi.ravel(), Yi.ravel()], 1)
+    x = np.concatenate([xo, xi], 0)
+    n = x.shape[0]
+    band = np.abs(x[:, 1] - 0.5) < 0.25
+    rho = np.where(band, 2.0, 1.0)
+    vol = np.where(band, 0.5 * (1.0 / nyi) * dxi * 2.0 / 1.0, dx * dx)
+    vol = np.where(band, dxi * (0.5 / nyi), dx * dx)
+    m = rho * vol
+    v = np.zeros((n, 2))
+    v[:, 0] = np.where(band, 0.5, -0.5)
+    v[:, 1] = 0.01 * np.sin(4.0 * math.pi * x[:, 0])
+    eps = P0 / ((gamma - 1.0) * rho)
+    hloc = np.sqrt(vol)
+    H = _iso_H(n, 2, 1.0 / (nperh * hloc))
+    cfg = abi.default_config(2, kernel=kernel, nperh=nperh, gamma=gamma, precision=precision)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    abi.set_periodic(cfg, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    return cfg, dict(x=x, v=v, m=m, rho=rho, eps=eps, H=H)
+
+
+def evrard_3d(n=250_000, seed=4242, nperh=1.51, gamma=5.0 / 3.0, u0=0.05, M=1.0, R=1.0,
+              kernel=abi.SPX_KERNEL_BSPLINE, precision=abi.SPX_PREC_MIXED):
+    """Evrard collapse: rho(r) = M/(2 pi R^2 r) sampled by r = R*U (mass fraction M(r)/M = (r/R)^2
+    -> r = R sqrt(u)), isotropic directions, u=0.05, v=0; TreeGravity G=1, opening 0.5,
+    softening 0.01."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = R * np.sqrt(rng.uniform(0.0, 1.0, n))
+    mu = rng.uniform(-1.0, 1.0, n)
+    ph = rng.uniform(0.0, 2.0 * math.pi, n)
+    st = np.sqrt(1.0 - mu * mu)
+    x = np.stack([r * st * np.cos(ph), r * st * np.sin(ph), r * mu], 1)
+    m = np.full(n, M / n)
+    rho = M / (2.0 * math.pi * R * R * np.maximum(r, 1e-3 * R))
+    eps = np.full(n, u0)
+    hloc = (m / rho) ** (1.0 / 3.0)
+    H = _iso_H(n, 3, 1.0 / (nperh * hloc))
+    cfg = abi.default_config(3, kernel=kernel, nperh=nperh, gamma=gamma, precision=precision)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    cfg.gravity = 1
+    cfg.G = 1.0
+    cfg.opening = 0.5
+    cfg.softening = 0.01
+    cfg.ftimestep = 0.1
+    return cfg, dict(x=x, v=np.zeros((n, 3)), m=m, rho=rho, eps=eps, H=H)
+
+
+def uniform_lattice(dim, n1d, nperh=2.0, kernel=abi.SPX_KERNEL_WENDLANDC4, rho0=1.0, P0=1.0, gamma=5.0 / 3.0,
+                    periodic=True):
+    """Perfect periodic lattice at rest (uniform pressure): a closed-form test state."""
+    dx = 1.0 / n1d
+    g = (np.arange(n1d) + 0.5) * dx
+    grids = np.meshgrid(*([g] * dim), indexing="ij")
+    x = np.stack([gg.ravel() for gg in grids], 1)
+    n = x.shape[0]
+    cfg = abi.default_config(dim, kernel=kernel, nperh=nperh, gamma=gamma)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    if periodic:
+        abi.set_periodic(cfg, (0.0,) * 3, (1.0,) * 3)
+    return cfg, dict(x=x, v=np.zeros((n, dim)), m=np.full(n, rho0 * dx ** dim), rho=np.full(n, rho0),
+                     eps=np.full(n, P0 / ((gamma - 1.0) * rho0)), H=_iso_H(n, dim, 1.0 / (nperh * dx)))
