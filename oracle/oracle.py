@@ -48,6 +48,8 @@ def lib():
         L.spxo_table_eval.argtypes = [C.POINTER(SpxConfig), D, C.c_int64, D, D, D]
         L.spxo_table_coefficients.argtypes = [C.POINTER(SpxConfig), I64, D, D, D]
         L.spxo_base_kernel.argtypes = [C.c_int, C.c_int, D, C.c_int64, D, D]
+        L.spxo_quadinterp.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, D,
+                                      C.c_int64, D]
         L.spxo_conservation.argtypes = [P, D]
         L.spxo_last_error.restype = C.c_char_p
         L.spxo_set_threads.argtypes = [C.c_int]
@@ -76,6 +78,7 @@ FIELD_COMPONENTS = {
     "max viscous pressure": "s", "effective viscous pressure": "s", "mass zeroth moment": "s",
     "mass first moment": "v", "M SPH correction": "t", "mass density gradient": "v",
     "normalization": "s", "new mass density": "s", "gravitational potential": "s",
+    "abs delta velocity hydro": "v", "abs delta specific thermal energy": "s",
 }
 
 
@@ -195,3 +198,10 @@ def base_kernel(kind, dim, eta):
     gW = np.zeros_like(eta)
     lib().spxo_base_kernel(kind, dim, _dp(eta), eta.size, _dp(W), _dp(gW))
     return W, gW
+
+
+def quadinterp(c, xmin, xmax, n, xs):
+    xs = np.ascontiguousarray(xs, np.float64)
+    out = np.zeros_like(xs)
+    lib().spxo_quadinterp(float(c[0]), float(c[1]), float(c[2]), float(xmin), float(xmax), int(n), _dp(xs), xs.size, _dp(out))
+    return out

@@ -408,6 +408,7 @@ template<int D> struct Sim {
   std::vector<V> DxDt, DvDt, DvDtG, gradRho, XSPHDeltaV, m1;
   std::vector<double> rhoSum, normalization, DrhoDt, DepsDt, maxQ, effQ, XSPHWeightSum, m0, potential;
   std::vector<T> DvDx, localDvDx, Mc, localM, DHDt, Hideal;
+  std::vector<V> absDvDt; std::vector<double> absDepsDt;   // parity scales: sum_j |pair term|
   double t = 0, lastDt = 1e-5; int64_t cycle = 0; bool haveState = false, initialized = false;
   // gravity
   std::vector<std::unordered_map<uint64_t, Cell<D>>> tree; V gxmin, gxmax; double boxLength = 0;
@@ -431,6 +432,7 @@ template<int D> struct Sim {
     rs(rhoSum, n, 0.0); rs(normalization, n, 0.0); rs(DrhoDt, n, 0.0); rs(DepsDt, n, 0.0); rs(maxQ, n, 0.0);
     rs(effQ, n, 0.0); rs(XSPHWeightSum, n, 0.0); rs(m0, n, 0.0); rs(potential, n, 0.0);
     rs(DvDx, n, zt); rs(localDvDx, n, zt); rs(Mc, n, zt); rs(localM, n, zt); rs(DHDt, n, zt); rs(Hideal, n, zt);
+    rs(absDvDt, n, z); rs(absDepsDt, n, 0.0);
   }
 
   // ---- ghosts -------------------------------------------------------------
@@ -655,6 +657,7 @@ template<int D> struct Sim {
     std::fill(gradRho.begin(), gradRho.end(), z); std::fill(XSPHDeltaV.begin(), XSPHDeltaV.end(), z); std::fill(m1.begin(), m1.end(), z);
     for (auto* f : {&rhoSum, &normalization, &DrhoDt, &DepsDt, &maxQ, &effQ, &XSPHWeightSum, &m0}) std::fill(f->begin(), f->end(), 0.0);
     for (auto* f : {&DvDx, &localDvDx, &Mc, &localM, &DHDt, &Hideal}) std::fill(f->begin(), f->end(), zt);
+    std::fill(absDvDt.begin(), absDvDt.end(), z); std::fill(absDepsDt.begin(), absDepsDt.end(), 0.0);
   }
 
   // ---- SPH::evaluateDerivativesImpl (SPH/SPH.cc:165-555) + MG Q + SPHSmoothingScale
@@ -664,7 +667,8 @@ template<int D> struct Sim {
     const int nth = omp_get_max_threads();
     const V z = vzero<D>(); const T zt = mzero<D>();
     TC<double> t_rhoSum(nth, nTot, 0.0), t_norm(nth, nTot, 0.0), t_DepsDt(nth, nTot, 0.0), t_maxQ(nth, nTot, 0.0), t_effQ(nth, nTot, 0.0), t_m0(nth, nTot, 0.0);
-    TC<V> t_DvDt(nth, nTot, z), t_gradRho(nth, nTot, z), t_m1(nth, nTot, z);
+    TC<V> t_DvDt(nth, nTot, z), t_gradRho(nth, nTot, z), t_m1(nth, nTot, z), t_aV(nth, nTot, z);
+    TC<double> t_aE(nth, nTot, 0.0);
     TC<T> t_DvDx(nth, nTot, zt), t_M(nth, nTot, zt);
     const int64_t np = (int64_t)pairs.size();
     pacc.assign(np, z);
@@ -673,7 +677,7 @@ template<int D> struct Sim {
       const int tid = omp_get_thread_num();
       auto& rS = t_rhoSum.a[tid]; auto& nm = t_norm.a[tid]; auto& dE = t_DepsDt.a[tid]; auto& mq = t_maxQ.a[tid]; auto& eq = t_effQ.a[tid];
       auto& dV = t_DvDt.a[tid]; auto& gR = t_gradRho.a[tid]; auto& dX = t_DvDx.a[tid]; auto& Ma = t_M.a[tid];
-      auto& z0 = t_m0.a[tid]; auto& z1 = t_m1.a[tid];
+      auto& z0 = t_m0.a[tid]; auto& z1 = t_m1.a[tid]; auto& aV = t_aV.a[tid]; auto& aE = t_aE.a[tid];
 #pragma omp for
       for (int64_t kk = 0; kk < np; ++kk) {
         const int64_t i = pairs[kk].first, j = pairs[kk].second;
@@ -711,6 +715,9 @@ template<int D> struct Sim {
         const V deltaDvDt = add(add(add(scale(Prhoi, gradWi), scale(Prhoj, gradWj)), Qacci), Qaccj);
         dV[i] = sub(dV[i], scale(mj, deltaDvDt)); dV[j] = add(dV[j], scale(mi, deltaDvDt));
         pacc[kk] = scale(-mj, deltaDvDt);                                    // SPH.cc:430
+        for (int a = 0; a < D; ++a) { aV[i].c[a] += std::abs(mj*deltaDvDt.c[a]); aV[j].c[a] += std::abs(mi*deltaDvDt.c[a]); }
+        aE[i] += std::abs(mj*Prhoi*dot(vij, gradWi)) + std::abs(mj*workQi);
+        aE[j] += std::abs(mi*Prhoj*dot(vij, gradWj)) + std::abs(mi*workQj);
         dE[i] += mj*(Prhoi*dot(vij, gradWi) + workQi);
         dE[j] += mi*(Prhoj*dot(vij, gradWj) + workQj);
         const T dDvDxi = mscale(mj, dyad(vij, gradWi)), dDvDxj = mscale(mi, dyad(vij, gradWj));
@@ -731,6 +738,7 @@ template<int D> struct Sim {
         maxQ[i] = std::max(maxQ[i], t_maxQ.a[t][i]); effQ[i] += t_effQ.a[t][i]; m0[i] += t_m0.a[t][i];
         DvDt[i] = add(DvDt[i], t_DvDt.a[t][i]); gradRho[i] = add(gradRho[i], t_gradRho.a[t][i]); m1[i] = add(m1[i], t_m1.a[t][i]);
         madd(DvDx[i], t_DvDx.a[t][i]); madd(Mc[i], t_M.a[t][i]);
+        absDvDt[i] = add(absDvDt[i], t_aV.a[t][i]); absDepsDt[i] += t_aE.a[t][i];
       }
       localDvDx[i] = DvDx[i]; localM[i] = Mc[i];
     }
@@ -1176,6 +1184,7 @@ int64_t spxo_get_field(void* p, const char* name, double* out) {
     else if (n == "M SPH correction") te(s->Mc); else if (n == "mass density gradient") ve(s->gradRho);
     else if (n == "normalization") sc(s->normalization); else if (n == "new mass density") sc(s->rhoSum);
     else if (n == "gravitational potential") sc(s->potential);
+    else if (n == "abs delta velocity hydro") ve(s->absDvDt); else if (n == "abs delta specific thermal energy") sc(s->absDepsDt);
   });
   if (cnt < 0) fail(SPX_ERR_ARG, "unknown field " + n);
   return cnt;
@@ -1196,6 +1205,13 @@ int spxo_table_coefficients(const spx_config* cfg, int64_t* ncoef, double* wc, d
   if (wc) std::memcpy(wc, t.Wi.c.data(), t.Wi.c.size()*sizeof(double));
   if (gwc) std::memcpy(gwc, t.gWi.c.data(), t.gWi.c.size()*sizeof(double));
   if (xstep) *xstep = t.Wi.xstep;
+  return 0;
+}
+// QuadraticInterpolator on an arbitrary quadratic c0 + c1 x + c2 x^2 over [xmin, xmax] with n
+// samples, evaluated at xs (the reference's own exactness test, quadraticinterpolator_tests.cc:17-80).
+int spxo_quadinterp(double c0, double c1, double c2, double xmin, double xmax, int64_t n, const double* xs, int64_t m, double* out) {
+  QuadInterp q; q.init(xmin, xmax, (size_t)n, [&](double x) { return c0 + c1*x + c2*x*x; });
+  for (int64_t k = 0; k < m; ++k) out[k] = q(xs[k]);
   return 0;
 }
 // Analytic base kernel (closed forms) for pinning the table.

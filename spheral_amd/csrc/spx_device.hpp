@@ -37,6 +37,8 @@ struct Geo {
   double origin[3];
   double cell[3], inv_cell[3];
   int nc[3];
+  // bit a: axis a periodic; bit 2a + side - 1 (side 1 low, 2 high): that side reflecting
+  int permask, reflmask;
 };
 
 // Physics constants of the run.
@@ -50,7 +52,7 @@ struct Fields {
   // state (start of step, then end of step)
   double *x[3], *v[3], *m, *rho, *eps, *hinv, *P, *cs, *omega;
   // midpoint state
-  double *xm[3], *vm[3], *rhom, *epsm, *hinvm, *Pm, *csm, *omegam;
+  double *xm[3], *vm[3], *rhom, *epsm, *hinvm, *Pm, *csm, *omegam, *prhom;
   // end-of-step velocity (swapped with v after the full update)
   double *v1[3];
   // derivatives carried across steps

@@ -188,6 +188,13 @@ class Run {
       geo.inv_cell[a] = 1.0;
       geo.origin[a] = 0.0;
     }
+    geo.permask = 0;
+    geo.reflmask = 0;
+    for (int a = 0; a < D; ++a) {
+      if (geo.bc_lo[a] == BC_PERIODIC) geo.permask |= 1 << a;
+      if (geo.bc_lo[a] == BC_REFLECTING) geo.reflmask |= 1 << (2*a);
+      if (geo.bc_hi[a] == BC_REFLECTING) geo.reflmask |= 1 << (2*a + 1);
+    }
   }
 
   // Carried arrays (persist across steps, follow the reorder).
@@ -210,7 +217,7 @@ class Run {
         all.push_back(p);
     }
     for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.rhom, &F.epsm, &F.hinvm, &F.Pm,
-                       &F.csm, &F.omegam, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.Hideal, &F.m0, &F.rhoSum,
+                       &F.csm, &F.omegam, &F.prhom, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.Hideal, &F.m0, &F.rhoSum,
                        &F.norm, &F.effQ, &F.ns, &F.pot})
       all.push_back(p);
     const size_t tensorComps = 2*(size_t)D*D;   // DvDx, Mcorr
@@ -592,6 +599,7 @@ class Run {
   }
   void sphPass(bool full) {
     const int nb = nblocks(n);
+    hipLaunchKernelGGL(k_prho, dim3(nb), dim3(kBlock), 0, stream, f, (int)n);
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (full) hipLaunchKernelGGL((k_sph<DD, true>), dim3(nb), dim3(kBlock), 0, stream, f, geo, ph, dtab, (int)n);

@@ -22,14 +22,14 @@ __device__ __forceinline__ void image_pos(const Geo& g, int code, double* xj) {
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     const int c = (code >> (2*a)) & 3;
-    if (c) xj[a] = image_coord<D>(c == 1 ? g.bc_lo[a] : g.bc_hi[a], c, xj[a], g.xmin[a], g.xmax[a]);
+    if (c) xj[a] = image_coord<D>(((g.permask >> a) & 1) ? BC_PERIODIC : BC_REFLECTING, c, xj[a], g.xmin[a], g.xmax[a]);
   }
 }
 template <int D>
 __device__ __forceinline__ double refl_sign(const Geo& g, int code, int a) {
   const int c = (code >> (2*a)) & 3;
   if (!c) return 1.0;
-  return ((c == 1 ? g.bc_lo[a] : g.bc_hi[a]) == BC_REFLECTING) ? -1.0 : 1.0;
+  return ((g.reflmask >> (2*a + c - 1)) & 1) ? -1.0 : 1.0;
 }
 
 // ---------------------------------------------------------------------------

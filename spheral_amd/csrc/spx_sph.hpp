@@ -9,79 +9,90 @@ namespace spx {
 template <int D>
 struct Nbr {
   double x[D], v[D];
-  double m, rho, P, cs, h, somega;
+  double m, rho, cs, h, prho;
 };
 
 template <int D>
 __device__ __forceinline__ void load_mid(const Fields& f, const Geo& g, int j, int code, Nbr<D>& q) {
 #pragma unroll
   for (int a = 0; a < D; ++a) { q.x[a] = f.xm[a][j]; q.v[a] = refl_sign<D>(g, code, a)*f.vm[a][j]; }
-  image_pos<D>(g, code, q.x);
+  if (code) image_pos<D>(g, code, q.x);
   q.m = f.m[j];
   q.rho = f.rhom[j];
-  q.P = f.Pm[j];
   q.cs = f.csm[j];
   q.h = f.hinvm[j];
-  const double om = f.omegam[j];
-  q.somega = om/(om*om + 1.0e-30);           // safeInv(omega, 1e-30)
+  q.prho = f.prhom[j];
+}
+template <int D>
+__device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D>& s) {
+#pragma unroll
+  for (int a = 0; a < D; ++a) { s.x[a] = f.xm[a][i]; s.v[a] = f.vm[a][i]; }
+  s.m = f.m[i];
+  s.rho = f.rhom[i];
+  s.cs = f.csm[i];
+  s.h = f.hinvm[i];
+  s.prho = f.prhom[i];
 }
 
-// Per-pair quantities of SPH::evaluateDerivativesImpl for the canonical pair (a, b)
-// (SPH/SPH.cc:362-430, MonaghanGingoldViscosity::QPiij MonaghanGingoldViscosity.cc:64-100).
+// Pair quantities of SPH::evaluateDerivativesImpl (SPH/SPH.cc:362-430) and
+// MonaghanGingoldViscosity::QPiij (MonaghanGingoldViscosity.cc:64-100), evaluated
+// from the gathering node s toward its neighbour o (r = r_s - r_o).  The canonical
+// pair (a, b) of the reference has a = s when aFirst, else a = o; every term below
+// equals its canonical value up to an exact sign flip, and the pairwise acceleration
+// dd is summed in the canonical order so both lanes produce the same bits:
+//   dd_canonical = aFirst ? dd : -dd.
 template <int D>
-struct PairEval {
-  double rab[D], vab[D], etaA[D], etaB[D], gWa[D], gWb[D], dd[D];
-  double Wa, Wb, Qa, Qb, workQa, workQb, PrhoA, PrhoB, etaMagA, etaMagB;
+struct PairS {
+  double r[D], vso[D], gs[D], go[D], es[D], dd[D];
+  double Ws, Wo, Qs, workQs, etaMagS;
 };
 
 template <int D>
-__device__ __forceinline__ void eval_pair(const DevTable& tab, const Phys& ph, const Nbr<D>& A, const Nbr<D>& B,
-                                          PairEval<D>& e) {
-  double e2a = 0.0, e2b = 0.0;
+__device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, const Nbr<D>& S, const Nbr<D>& O,
+                                          bool aFirst, PairS<D>& e) {
+  double eo[D], e2s = 0.0, e2o = 0.0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
-    e.rab[c] = A.x[c] - B.x[c];
-    e.etaA[c] = A.h*e.rab[c];
-    e.etaB[c] = B.h*e.rab[c];
-    e2a += e.etaA[c]*e.etaA[c];
-    e2b += e.etaB[c]*e.etaB[c];
-    e.vab[c] = A.v[c] - B.v[c];
+    e.r[c] = S.x[c] - O.x[c];
+    e.es[c] = S.h*e.r[c];
+    eo[c] = O.h*e.r[c];
+    e2s += e.es[c]*e.es[c];
+    e2o += eo[c]*eo[c];
+    e.vso[c] = S.v[c] - O.v[c];
   }
-  e.etaMagA = sqrt(e2a);
-  e.etaMagB = sqrt(e2b);
-  const double ia = 1.0/fmax(1.0e-30, e.etaMagA), ib = 1.0/fmax(1.0e-30, e.etaMagB);   // safeInvVar
-  double gwa, gwb;
-  kernel_wgw(tab, e.etaMagA, hdet_iso<D>(A.h), e.Wa, gwa);
-  kernel_wgw(tab, e.etaMagB, hdet_iso<D>(B.h), e.Wb, gwb);
-  const double sa = gwa*A.h, sb = gwb*B.h;                  // (gW*H)*etaUnit
-  double mua = 0.0, mub = 0.0;
+  e.etaMagS = sqrt(e2s);
+  const double etaMagO = sqrt(e2o);
+  const double is = 1.0/fmax(1.0e-30, e.etaMagS), io = 1.0/fmax(1.0e-30, etaMagO);   // safeInvVar
+  double gws, gwo;
+  kernel_wgw(tab, e.etaMagS, hdet_iso<D>(S.h), e.Ws, gws);
+  kernel_wgw(tab, etaMagO, hdet_iso<D>(O.h), e.Wo, gwo);
+  const double ss = gws*S.h, so = gwo*O.h;                   // (gW*H)*etaUnit
+  double mus = 0.0, muo = 0.0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
-    e.gWa[c] = sa*(e.etaA[c]*ia);
-    e.gWb[c] = sb*(e.etaB[c]*ib);
-    mua += e.vab[c]*e.etaA[c];
-    mub += e.vab[c]*e.etaB[c];
+    e.gs[c] = ss*(e.es[c]*is);
+    e.go[c] = so*(eo[c]*io);
+    mus += e.vso[c]*e.es[c];
+    muo += e.vso[c]*eo[c];
   }
-  mua = mua/(e2a + 1.0e-2);
-  mub = mub/(e2b + 1.0e-2);
-  const double mma = fmin(0.0, mua), mmb = fmin(0.0, mub);
-  const double ea = -ph.Cl*A.cs*mma + ph.Cq*(mma*mma);
-  const double eb = -ph.Cl*B.cs*mmb + ph.Cq*(mmb*mmb);
-  const double QPa = ea/A.rho, QPb = eb/B.rho;
-  e.Qa = A.rho*ea;
-  e.Qb = B.rho*eb;
-  e.PrhoA = A.somega*A.P/(A.rho*A.rho);
-  e.PrhoB = B.somega*B.P/(B.rho*B.rho);
-  double wqa = 0.0, wqb = 0.0;
+  mus = mus/(e2s + 1.0e-2);
+  muo = muo/(e2o + 1.0e-2);
+  const double mms = fmin(0.0, mus), mmo = fmin(0.0, muo);
+  const double es_ = -ph.Cl*S.cs*mms + ph.Cq*(mms*mms);
+  const double eo_ = -ph.Cl*O.cs*mmo + ph.Cq*(mmo*mmo);
+  const double QPs = es_/S.rho, QPo = eo_/O.rho;
+  e.Qs = S.rho*es_;
+  double wqs = 0.0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
-    const double qa = 0.5*(QPa*e.gWa[c]), qb = 0.5*(QPb*e.gWb[c]);
-    wqa += e.vab[c]*qa;
-    wqb += e.vab[c]*qb;
-    e.dd[c] = ((e.PrhoA*e.gWa[c] + e.PrhoB*e.gWb[c]) + qa) + qb;
+    const double qs = 0.5*(QPs*e.gs[c]), qo = 0.5*(QPo*e.go[c]);
+    wqs += e.vso[c]*qs;
+    const double p1 = aFirst ? S.prho : O.prho, p2 = aFirst ? O.prho : S.prho;
+    const double g1 = aFirst ? e.gs[c] : e.go[c], g2 = aFirst ? e.go[c] : e.gs[c];
+    const double q1 = aFirst ? qs : qo, q2 = aFirst ? qo : qs;
+    e.dd[c] = ((p1*g1 + p2*g2) + q1) + q2;
   }
-  e.workQa = wqa;
-  e.workQb = wqb;
+  e.workQs = wqs;
 }
 
 // ---------------------------------------------------------------------------
@@ -96,10 +107,7 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
   if (i >= n) return;
   const size_t ld = (size_t)f.ld;
   Nbr<D> S;
-#pragma unroll
-  for (int a = 0; a < D; ++a) { S.x[a] = f.xm[a][i]; S.v[a] = f.vm[a][i]; }
-  S.m = f.m[i]; S.rho = f.rhom[i]; S.P = f.Pm[i]; S.cs = f.csm[i]; S.h = f.hinvm[i];
-  { const double om = f.omegam[i]; S.somega = om/(om*om + 1.0e-30); }
+  load_self<D>(f, i, S);
 
   double dvdt[D], dvdx[D*D], Mc[D*D], depsdt = 0.0, maxq = 0.0, m0 = 0.0;
   double rhosum = 0.0, nrm = 0.0, effq = 0.0, grho[D], m1[D];
@@ -113,44 +121,34 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
     const uint32_t ent = f.nbr[(size_t)k*ld + i];
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
-    Nbr<D> Q;
-    load_mid<D>(f, g, j, code, Q);
-    const bool selfA = code != 0 || i < j;
-    PairEval<D> e;
-    if (selfA) eval_pair<D>(tab, ph, S, Q, e); else eval_pair<D>(tab, ph, Q, S, e);
-    // this node's side of the pair
-    const double Ws = selfA ? e.Wa : e.Wb;
-    const double Qs = selfA ? e.Qa : e.Qb;
-    const double wQs = selfA ? e.workQa : e.workQb;
-    const double Prs = selfA ? e.PrhoA : e.PrhoB;
-    const double mo = Q.m, rhoo = Q.rho;
-    const double dsg = selfA ? -1.0 : 1.0;
+    Nbr<D> O;
+    load_mid<D>(f, g, j, code, O);
+    PairS<D> e;
+    pair_eval<D>(tab, ph, S, O, code != 0 || i < j, e);
+    const double mo = O.m;
     double vdg = 0.0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const double gs = selfA ? e.gWa[c] : e.gWb[c];
-      dvdt[c] += dsg*(mo*e.dd[c]);
-      vdg += e.vab[c]*gs;
+      dvdt[c] -= mo*e.dd[c];
+      vdg += e.vso[c]*e.gs[c];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const double gd = selfA ? e.gWa[d] : e.gWb[d];
-        dvdx[c*D + d] -= mo*(e.vab[c]*gd);
-        Mc[c*D + d] -= mo*(e.rab[c]*gd);
+        dvdx[c*D + d] -= mo*(e.vso[c]*e.gs[d]);
+        Mc[c*D + d] -= mo*(e.r[c]*e.gs[d]);
       }
     }
-    depsdt += mo*(Prs*vdg + wQs);
-    maxq = fmax(maxq, Qs);
-    const double wsph = kernel_wsph(tab, selfA ? e.etaMagA : e.etaMagB);
+    depsdt += mo*(S.prho*vdg + e.workQs);
+    maxq = fmax(maxq, e.Qs);
+    const double wsph = kernel_wsph(tab, e.etaMagS);
     m0 += wsph;
     if (FULL) {
-      rhosum += mo*Ws;
-      nrm += S.m/S.rho*Ws;
-      effq += mo*Qs*Ws/rhoo;
-      const double drho = selfA ? (rhoo - S.rho) : (S.rho - rhoo);
+      rhosum += mo*e.Ws;
+      nrm += S.m/S.rho*e.Ws;
+      effq += mo*e.Qs*e.Ws/O.rho;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
-        grho[c] += mo*drho*(selfA ? e.gWa[c] : e.gWb[c]);
-        m1[c] += dsg*(wsph*(selfA ? e.etaA[c] : e.etaB[c]));
+        grho[c] += mo*(O.rho - S.rho)*e.gs[c];
+        m1[c] -= wsph*e.es[c];
       }
     }
   }
@@ -243,10 +241,7 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   const double hdt = 0.5*dt;
   const size_t ld = (size_t)f.ld;
   Nbr<D> S;
-#pragma unroll
-  for (int a = 0; a < D; ++a) { S.x[a] = f.xm[a][i]; S.v[a] = f.vm[a][i]; }
-  S.m = f.m[i]; S.rho = f.rhom[i]; S.P = f.Pm[i]; S.cs = f.csm[i]; S.h = f.hinvm[i];
-  { const double om = f.omegam[i]; S.somega = om/(om*om + 1.0e-30); }
+  load_self<D>(f, i, S);
   double v12s[D];
 #pragma unroll
   for (int a = 0; a < D; ++a) v12s[a] = f.v[a][i] + f.DvDt[a][i]*hdt;
@@ -258,30 +253,29 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
     const uint32_t ent = f.nbr[(size_t)k*ld + i];
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
-    Nbr<D> Q;
-    load_mid<D>(f, g, j, code, Q);
+    Nbr<D> O;
+    load_mid<D>(f, g, j, code, O);
+    const bool aFirst = code != 0 || i < j;
+    PairS<D> e;
+    pair_eval<D>(tab, ph, S, O, aFirst, e);
+    // canonical: pacc_ab = (-m_b) dd_ab ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
+    const double mb = aFirst ? O.m : S.m, ma = aFirst ? S.m : O.m;
     double v12o[D];
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      const double sg = refl_sign<D>(g, code, a);
-      v12o[a] = sg*f.v[a][j] + (sg*f.DvDt[a][j])*hdt;
-    }
-    const double de0o = f.DepsDt[j];
-    const bool selfA = code != 0 || i < j;
-    PairEval<D> e;
-    if (selfA) eval_pair<D>(tab, ph, S, Q, e); else eval_pair<D>(tab, ph, Q, S, e);
-    // pacc_ab = -m_b * deltaDvDt ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
-    const double mb = selfA ? Q.m : S.m, ma = selfA ? S.m : Q.m;
+    if (D >= 1) { const double sg = refl_sign<D>(g, code, 0); v12o[0] = sg*f.v[0][j] + (sg*f.DvDt[0][j])*hdt; }
+    if (D >= 2) { const double sg = refl_sign<D>(g, code, 1); v12o[1] = sg*f.v[1][j] + (sg*f.DvDt[1][j])*hdt; }
+    if (D >= 3) { const double sg = refl_sign<D>(g, code, 2); v12o[2] = sg*f.v[2][j] + (sg*f.DvDt[2][j])*hdt; }
     double duab = 0.0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const double va = selfA ? v12s[c] : v12o[c], vb = selfA ? v12o[c] : v12s[c];
-      duab += (vb - va)*(-mb*e.dd[c]);
+      const double va = aFirst ? v12s[c] : v12o[c], vb = aFirst ? v12o[c] : v12s[c];
+      const double ddc = aFirst ? e.dd[c] : -e.dd[c];
+      duab += (vb - va)*((-mb)*ddc);
     }
+    const double de0o = f.DepsDt[j];
     const double sgn = duab < 0.0 ? -1.0 : 1.0;
-    const double wa = fmax(epsm, (selfA ? de0s : de0o)*sgn), wb = fmax(epsm, (selfA ? de0o : de0s)*sgn);
+    const double wa = fmax(epsm, (aFirst ? de0s : de0o)*sgn), wb = fmax(epsm, (aFirst ? de0o : de0s)*sgn);
     const double w = wa/(wa + wb);
-    du += selfA ? w*duab : (1.0 - w)*duab*ma/mb;
+    du += aFirst ? w*duab : (1.0 - w)*duab*ma/mb;
   }
   const double eps1 = f.eps[i] + du*dt;
   f.eps[i] = eps1;
@@ -298,6 +292,14 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   f.hinv[i] = fmin(ph.hminInv, fmax(ph.hmaxInv, f.Hideal[i]));
   f.P[i] = ph.g1*rho1*eps1;
   f.cs[i] = sqrt(fmax(0.0, ph.gamma*ph.g1*eps1));
+}
+
+// Per-node pressure term Prho = safeInv(omega)*P/rho^2 of the midpoint state (SPH.cc:425-426).
+__global__ void __launch_bounds__(kBlock) k_prho(Fields f, int n) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double om = f.omegam[i], rho = f.rhom[i];
+  f.prhom[i] = (om/(om*om + 1.0e-30))*f.Pm[i]/(rho*rho);
 }
 
 // Reductions for the next step's grid: min hinv and the bounding box (per block).

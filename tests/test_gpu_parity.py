@@ -1,0 +1,174 @@
+"""GPU engine vs the oracle on the same seeded inputs, through the C ABI.
+
+Bars: neighbour sets bit-exact (after sorting, images mapped to (control node, image code));
+FP64 derivatives within |gpu - oracle| <= 1e-10 * S, S = the oracle's per-node sum of
+|pair terms| for the cancelling sums (DvDt, DepsDt), relative 1e-10 elsewhere."""
+import numpy as np
+import pytest
+
+from spheral_amd import abi, problems
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def cases():
+    out = []
+    cfg, st = problems.sod_1d(400)
+    out.append(("sod1d_reflecting", cfg, st))
+    cfg, st = problems.taylor_green_3d(16)
+    out.append(("taylor_green_16^3_periodic", cfg, st))
+    cfg, st = problems.kelvin_helmholtz_2d(3000)
+    cfg.precision = abi.SPX_PREC_FP64
+    out.append(("kelvin_helmholtz_2d_periodic", cfg, st))
+    rng = np.random.default_rng(17)
+    n = 3000
+    x = rng.normal(0, 0.3, (n, 3))
+    cfg = abi.default_config(3, nperh=1.51)
+    h = np.full(n, 0.08)
+    H = np.zeros((n, 3, 3))
+    for a in range(3):
+        H[:, a, a] = 1.0/h
+    st = dict(x=x, v=rng.normal(0, 0.1, (n, 3)), m=np.full(n, 1.0/n), rho=np.ones(n), eps=np.full(n, 0.5), H=H)
+    out.append(("open_3d_gaussian_blob", cfg, st))
+    cfg, st = problems.uniform_lattice(2, 24, kernel=abi.SPX_KERNEL_BSPLINE)
+    cfg.bc_lo[0] = cfg.bc_hi[0] = abi.SPX_BC_REFLECTING
+    out.append(("lattice_2d_reflecting_x_periodic_y", cfg, st))
+    return out
+
+
+CASES = cases()
+IDS = [c[0] for c in CASES]
+
+
+def pair(gpu, oracle_mod, cfg, st):
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    o = oracle_mod.OracleRun(cfg, len(st["m"]))
+    o.set_state(**st)
+    o.initialize()
+    return g, o
+
+
+def rel_ok(a, b, tol=TOL):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.all(np.abs(a - b) <= tol*np.maximum(np.abs(b), np.abs(b).max()*1e-3 + 1e-300))
+
+
+@pytest.mark.parametrize("name,cfg,st", CASES, ids=IDS)
+def test_neighbor_sets_bit_exact(gpu, oracle_mod, name, cfg, st):
+    print(f"[case] {name}", flush=True)
+    g, o = pair(gpu, oracle_mod, cfg, st)
+    go, gn, gi = g.neighbors()
+    oo, on_, oi = o.neighbors()
+    assert np.array_equal(go, oo)
+    assert np.array_equal(gn, on_)
+    assert np.array_equal(gi, oi)
+
+
+@pytest.mark.parametrize("name,cfg,st", CASES, ids=IDS)
+def test_derivatives_match_oracle(gpu, oracle_mod, name, cfg, st):
+    print(f"[case] {name}", flush=True)
+    g, o = pair(gpu, oracle_mod, cfg, st)
+    for f in ("pressure", "sound speed", "grad h corrections"):
+        assert rel_ok(g.field(f), o.field(f)), f
+    g.sum_density()
+    o.sum_density()
+    assert rel_ok(g.field("mass density"), o.field("mass density"))
+    g.post_state_update()
+    o.post_state_update()
+    assert rel_ok(g.field("grad h corrections"), o.field("grad h corrections"))
+    g.evaluate_derivatives()
+    o.evaluate_derivatives()
+    S = o.field("abs delta velocity hydro")
+    assert np.all(np.abs(g.field("delta velocity hydro") - o.field("delta velocity hydro")) <= TOL*S + 1e-300)
+    S = o.field("abs delta specific thermal energy")
+    assert np.all(np.abs(g.field("delta specific thermal energy") - o.field("delta specific thermal energy"))
+                  <= TOL*S + 1e-300)
+    for f in ("velocity gradient", "delta mass density", "delta H"):
+        a, b = g.field(f), o.field(f)
+        assert np.all(np.abs(a - b) <= TOL*max(np.abs(b).max(), 1e-300)), f
+    for f in ("max viscous pressure", "mass zeroth moment", "new H", "new mass density", "normalization",
+              "effective viscous pressure", "delta position"):
+        assert rel_ok(g.field(f), o.field(f)), f
+    # cancelling sums whose natural scale is not their own value: M ~ identity, the density
+    # gradient ~ rho*|H|, the first moment ~ the zeroth moment
+    rho, hinv = o.field("mass density"), o.field("H")
+    scales = {"M SPH correction": 1.0,
+              "mass density gradient": np.abs(rho).max()*np.abs(hinv).max(),
+              "mass first moment": max(o.field("mass zeroth moment").max(), 1.0)}
+    for f, sc in scales.items():
+        a, b = g.field(f), o.field(f)
+        assert np.all(np.abs(a - b) <= TOL*max(np.abs(b).max(), sc)), f
+    gh, _ = g.dt()
+    oh, _ = o.dt()
+    assert abs(gh - oh) <= TOL*oh
+
+
+@pytest.mark.parametrize("name,cfg,st", CASES, ids=IDS)
+def test_compatible_energy_matches_oracle(gpu, oracle_mod, name, cfg, st):
+    print(f"[case] {name}", flush=True)
+    g, o = pair(gpu, oracle_mod, cfg, st)
+    g.evaluate_derivatives()
+    o.evaluate_derivatives()
+    dt = 1e-4
+    e0 = o.field("specific thermal energy").copy()
+    g.compatible_energy(dt)
+    o.compatible_energy(dt)
+    de_g = g.field("specific thermal energy") - e0
+    de_o = o.field("specific thermal energy") - e0
+    scale = dt*o.field("abs delta specific thermal energy") + 1e-300
+    assert np.all(np.abs(de_g - de_o) <= 1e-9*scale + 4*np.spacing(e0))
+
+
+@pytest.mark.parametrize("name,cfg,st", CASES, ids=IDS)
+def test_rk2_steps_track_oracle(gpu, oracle_mod, name, cfg, st):
+    print(f"[case] {name}", flush=True)
+    g, o = pair(gpu, oracle_mod, cfg, st)
+    for k in range(5):
+        dg = g.step()
+        do = o.step()
+        assert abs(dg - do) <= 1e-10*do
+    cs = o.field("sound speed").max()        # velocity scale of a state at rest
+    for f in ("position", "velocity", "mass density", "specific thermal energy", "H"):
+        a, b = g.field(f), o.field(f)
+        sc = max(np.abs(b).max(), cs if f == "velocity" else 0.0)
+        assert np.all(np.abs(a - b) <= 1e-9*sc), f
+    cg, co = g.conservation(), o.conservation()
+    assert abs((cg[4] + cg[5]) - (co[4] + co[5])) <= 1e-12*abs(co[4] + co[5])
+
+
+def test_taylor_green_full_size_conservation(gpu):
+    """128^3 = 2,097,152 particles: momentum and total energy conserved to roundoff over
+    3 compatible-energy RK2 steps (size-independent invariants at the benchmark size)."""
+    cfg, st = problems.taylor_green_3d(128)
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    c0 = g.conservation()
+    for _ in range(3):
+        g.step()
+    c1 = g.conservation()
+    pscale = np.abs(st["m"][:, None]*st["v"]).sum()
+    assert np.all(np.abs(c1[1:4] - c0[1:4]) <= 1e-12*pscale)
+    E0, E1 = c0[4] + c0[5], c1[4] + c1[5]
+    assert abs(E1 - E0) <= 1e-12*E0
+
+
+def test_neighbor_symmetry_at_scale(gpu):
+    """64^3 jittered periodic box: j in N(i) <=> i in N(j) with the mirrored image code."""
+    cfg, st = problems.taylor_green_3d(64)
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    off, nb, im = g.neighbors()
+    n = len(st["m"])
+    rows = np.repeat(np.arange(n), np.diff(off))
+    flip = np.zeros_like(im)
+    for a in range(3):
+        c = (im >> (2*a)) & 3
+        flip |= np.where(c == 0, 0, 3 - c) << (2*a)
+    fwd = np.stack([rows, nb, im], 1)
+    bwd = np.stack([nb, rows, flip], 1)
+    fwd = fwd[np.lexsort(fwd.T[::-1])]
+    bwd = bwd[np.lexsort(bwd.T[::-1])]
+    assert np.array_equal(fwd, bwd)

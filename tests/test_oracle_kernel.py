@@ -1,0 +1,108 @@
+"""Pins of the oracle's kernel tables (Kernel/TableKernel.cc) against what the mathematics fixes."""
+import math
+
+import numpy as np
+import pytest
+
+from spheral_amd import abi
+
+
+def analytic(kind, dim, eta):
+    """Closed forms written from the kernels' definitions (not from the oracle's code)."""
+    eta = np.asarray(eta, float)
+    if kind == abi.SPX_KERNEL_BSPLINE:
+        A = {1: 2.0/3.0, 2: 10.0/(7.0*math.pi), 3: 1.0/math.pi}[dim]
+        q = eta
+        W = np.where(q < 1, 1 - 1.5*q**2 + 0.75*q**3, np.where(q < 2, 0.25*(2 - q)**3, 0.0))
+        dW = np.where(q < 1, -3*q + 2.25*q**2, np.where(q < 2, -0.75*(2 - q)**2, 0.0))
+        return A*W, A*dW
+    A = {1: 1.5, 2: 9.0/math.pi, 3: 495.0/(32.0*math.pi)}[dim]
+    q = eta
+    if dim == 1:
+        W = np.where(q < 1, (1 - q)**5*(1 + 5*q + 8*q**2), 0.0)
+        dW = np.where(q < 1, -14*q*(1 + 4*q)*(1 - q)**4, 0.0)
+    else:
+        W = np.where(q < 1, (1 - q)**6*(1 + 6*q + 35.0/3.0*q**2), 0.0)
+        dW = np.where(q < 1, -56.0/3.0*q*(1 + 5*q)*(1 - q)**5, 0.0)
+    return A*W, A*dW
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("kind", [abi.SPX_KERNEL_BSPLINE, abi.SPX_KERNEL_WENDLANDC4])
+def test_table_matches_closed_form(oracle_mod, dim, kind):
+    cfg = abi.default_config(dim, kernel=kind)
+    cfg.kernel = kind
+    ext = abi.KERNEL_EXTENT[kind]
+    rng = np.random.default_rng(3)
+    eta = rng.uniform(0.0, 1.1*ext, 5000)
+    W, gW, _ = oracle_mod.table_eval(cfg, eta)
+    Wa, gWa = analytic(kind, dim, eta)
+    W0 = analytic(kind, dim, [0.0])[0][0]
+    # reference tolerances (tests/unit/Kernel/testTableKernel.py:35-38): 1e-3 on W, 1e-2 on grad W
+    assert np.max(np.abs(W - Wa)) <= 1e-3*W0
+    assert np.max(np.abs(gW - gWa)) <= 1e-2*np.max(np.abs(gWa))
+    # with 1000 points the parabolas are far better than the test tolerance
+    assert np.max(np.abs(W - Wa)) <= 1e-8*W0
+    # zero at and beyond the extent
+    Wx, gx, _ = oracle_mod.table_eval(cfg, np.array([ext, ext*1.5]))
+    assert np.all(Wx == 0.0) and np.all(gx == 0.0)
+
+
+def test_base_kernel_closed_form(oracle_mod):
+    eta = np.linspace(0, 2.2, 301)
+    for dim in (1, 2, 3):
+        for kind in (abi.SPX_KERNEL_BSPLINE, abi.SPX_KERNEL_WENDLANDC4):
+            W, gW = oracle_mod.base_kernel(kind, dim, eta)
+            Wa, gWa = analytic(kind, dim, eta)
+            assert np.allclose(W, Wa, rtol=1e-13, atol=1e-15)
+            assert np.allclose(gW, gWa, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_kernel_normalisation(oracle_mod, dim):
+    """Volume integral of W is one in every dimension (fixes the normalisation constants)."""
+    for kind in (abi.SPX_KERNEL_BSPLINE, abi.SPX_KERNEL_WENDLANDC4):
+        ext = abi.KERNEL_EXTENT[kind]
+        r = np.linspace(0, ext, 200001)
+        W, _ = oracle_mod.base_kernel(kind, dim, r)
+        shell = {1: 2.0*np.ones_like(r), 2: 2*math.pi*r, 3: 4*math.pi*r**2}[dim]
+        assert abs(np.trapezoid(W*shell, r) - 1.0) < 1e-6
+
+
+def test_quadratic_interpolator_is_exact_for_quadratics(oracle_mod):
+    """The reference's own check (tests/cpp/Utilities/quadraticinterpolator_tests.cc:17-107):
+    f = 1 + x(2 + 3x) on [10, 100] with 41 samples, compared at the samples with
+    ASSERT_FLOAT_EQ (4 float ulps); plus a dense check inside [0, 2]."""
+    NV, xmin, xmax = 41, 10.0, 100.0
+    xs = xmin + (xmax - xmin)/(NV - 1.0)*np.arange(NV)
+    y = oracle_mod.quadinterp((1.0, 2.0, 3.0), xmin, xmax, NV, xs)
+    ref = 1.0 + xs*(2.0 + 3.0*xs)
+    assert np.all(np.abs(y - ref) <= 4*np.spacing(ref.astype(np.float32)).astype(float))
+    xs = np.linspace(0.0, 2.0, 4001)
+    y = oracle_mod.quadinterp((1.0, 2.0, 3.0), 0.0, 2.0, 100, xs)
+    assert np.max(np.abs(y - (1 + 2*xs + 3*xs**2))) < 1e-10*17.0
+
+
+def test_nperh_lookup_is_monotone_and_inverts_wsum(oracle_mod):
+    """Wsum(nPerh) -> nPerh round trip on the lattice sums (testTableKernel.py nPerh checks)."""
+    for dim in (1, 2, 3):
+        cfg = abi.default_config(dim, kernel=abi.SPX_KERNEL_WENDLANDC4)
+        cfg.kernel = abi.SPX_KERNEL_WENDLANDC4
+        ext = 1.0
+        for nperh in (1.5, 2.0, 3.0, 4.0):
+            # the lattice kernel sum as TableKernel's sumKernelValues defines it
+            _, gW, _ = oracle_mod.table_eval(cfg, np.arange(1, int(ext*nperh) + 2)/nperh)
+            k = np.arange(1, int(ext*nperh) + 2)
+            eta = k/nperh
+            w = np.abs(gW)*(eta < ext)
+            if dim == 1:
+                ws = np.sum(2*w)
+            elif dim == 2:
+                ws = math.sqrt(np.sum(2*math.pi*k*w))
+            else:
+                ws = np.sum(4*math.pi*k**2*w)**(1/3)
+            _, _, nph = oracle_mod.table_eval(cfg, np.array([ws]))
+            assert abs(nph[0] - nperh) < 2e-2*nperh
+        grid = np.linspace(0.5, 5.0, 400)
+        _, _, nph = oracle_mod.table_eval(cfg, grid)
+        assert np.all(np.diff(nph) >= -1e-12)
