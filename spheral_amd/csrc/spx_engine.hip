@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/spx_abi.h"
+#include "spx_error.hpp"
 #include "spx_gravity.hpp"
 #include "spx_sph.hpp"
 #include "spx_table.hpp"
@@ -26,19 +27,6 @@
 namespace spx {
 
 static thread_local std::string g_last_error;
-
-struct SpxError {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] static void raise(int code, const std::string& msg) { throw SpxError{code, msg}; }
-
-#define HIP_CHECK(expr)                                                                          \
-  do {                                                                                           \
-    hipError_t e_ = (expr);                                                                      \
-    if (e_ != hipSuccess) raise(SPX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1)/kBlock); }
 
@@ -101,7 +89,7 @@ class Run {
     allocate();
     t = 0.0;
     lastDt = cfg.last_dt;
-    grav.init(D, n, ld, stream);
+    grav.init(D, n, ld, stream, cfg.gravity != 0);
   }
   ~Run() {
     (void)hipSetDevice(device);
@@ -607,15 +595,12 @@ class Run {
     });
     HIP_CHECK(hipGetLastError());
   }
+  // TreeGravity::initialize (tree on the given positions) + evaluateDerivatives + dt vote
   void gravityPass(bool mid) {
     if (!cfg.gravity) return;
     double* xs[3] = {mid ? f.xm[0] : f.x[0], mid ? f.xm[1] : f.x[1], mid ? f.xm[2] : f.x[2]};
     grav.build(xs, f.m, (int)n);
-    grav.evaluate(xs, f.m, f.DvDtG, f.pot, (int)n, cfg.G, cfg.softening, cfg.opening, dsc);
-  }
-  void gravityDtVote() {
-    if (!cfg.gravity) return;
-    grav.finishDt(dsc, cfg.G, cfg.softening, cfg.ftimestep);
+    grav.evaluate(f.DvDtG, f.pot, (int)n, cfg.G, cfg.softening, cfg.opening, dsc, cfg.ftimestep);
   }
 
   void mark(const char* name) {
@@ -656,10 +641,7 @@ class Run {
     zeroDerivs(true);
     eos();
     omegaPass(false);
-    if (cfg.gravity) {
-      gravityPass(false);
-      gravityDtVote();
-    }
+    if (cfg.gravity) gravityPass(false);
     HIP_CHECK(hipStreamSynchronize(stream));
   }
 
@@ -689,9 +671,7 @@ class Run {
     mark("omega");
     // derivatives at the midpoint
     if (cfg.gravity) {
-      for (int a = 0; a < D; ++a) HIP_CHECK(hipMemsetAsync(f.DvDtG[a], 0, n*sizeof(double), stream));
       gravityPass(true);
-      gravityDtVote();
       mark("gravity");
     }
     sphPass(false);
@@ -723,11 +703,7 @@ class Run {
     if (!connValid) raise(SPX_ERR_STATE, "no connectivity: call spx_run_update_connectivity or spx_run_initialize");
     copyStateToMid();
     zeroDerivs(false);
-    if (cfg.gravity) {
-      for (int a = 0; a < D; ++a) HIP_CHECK(hipMemsetAsync(f.DvDtG[a], 0, n*sizeof(double), stream));
-      gravityPass(true);
-      gravityDtVote();
-    }
+    if (cfg.gravity) gravityPass(true);
     sphPass(true);
     HIP_CHECK(hipStreamSynchronize(stream));
   }

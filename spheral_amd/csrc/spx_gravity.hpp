@@ -1,12 +1,408 @@
-// spx_gravity.hpp -- TreeGravity on the device (placeholder until the octree lands).
+// spx_gravity.hpp -- TreeGravity on the device (3-D).
+//
+// Tree: the reference's octree (Gravity/TreeGravityInline.hh:37-160, TreeGravity.cc:362-477)
+// rebuilt from 60-bit Morton keys.  The level-l cell of a point is the l-level prefix of its
+// key (floor(u*2^l) with u = (x - xmin)/boxLength is the prefix of floor(u*2^20), since the
+// scaling is by a power of two), and the reference's split rule -- a cell is subdivided when a
+// second point arrives, down to level 20 where points are listed -- gives: cells at level l are
+// the l-prefix groups of points whose (l-1)-group holds at least two points.  Point p owns the
+// cells (p, l) for l from 1 + lcp(p-1, p) to its leaf level min(20, 1 + max lcp with its
+// sorted neighbours); numbering them by (p, l) is a pre-order walk, so each cell's subtree is
+// the contiguous index range [c, skip[c]) and the force walk needs no stack.
+//
+// Walk: TreeGravity::applyTreeForces (TreeGravity.cc:818-950) per point from the root's
+// daughters: accept a cell's monopole when |xcm - xi|^2 > cellsize^2/opening^2 + rcm2cc2,
+// otherwise sum a leaf's points directly (self screened by r^2/soft^2 > 1e-10) or descend.
+// Force M/(r^2+soft^2) along the unit vector, potential -M/sqrt(r^2+soft^2); then
+// DvDt *= G, potential *= G, dt vote ftimestep*min sqrt(soft/|a|)/sqrt(G) (TreeGravity.cc:143-328,483-497).
 #pragma once
+
+#include <rocprim/rocprim.hpp>
+
+#include <vector>
+
 #include "spx_device.hpp"
+#include "spx_error.hpp"
+
 namespace spx {
-struct GravityTree {
-  void init(int, int64_t, int, hipStream_t) {}
-  void release() {}
-  void build(double* const*, const double*, int) {}
-  void evaluate(double* const*, const double*, double* const*, double*, int, double, double, double, Scalars*) {}
-  void finishDt(Scalars*, double, double, double) {}
+
+constexpr int kTreeBits = 20;   // deepest level (num1dbits - 1)
+
+struct TreeView {
+  const double* tx[3];     // tree-order positions
+  const double* tm;        // tree-order masses
+  const uint64_t* code;    // sorted keys
+  const int* first;        // cell point range [first, last]
+  const int* last;
+  const int* skip;
+  const unsigned char* level;
+  const double* M;
+  const double* cx[3];     // centre of mass
+  const double* rc2;       // |xcm - cell centre|^2
+  int ncells, n;
+  double xmin[3], box;
 };
+
+__device__ __forceinline__ int lcp_levels(uint64_t a, uint64_t b) {
+  const uint64_t x = a ^ b;
+  if (x == 0) return kTreeBits;
+  const int lz = __clzll(x) - (64 - 3*kTreeBits);   // leading equal bits within the 60-bit key
+  return lz/3;
+}
+
+__global__ void __launch_bounds__(kBlock) k_tree_bbox(double* const* xs, int n, double* part) {
+  __shared__ double red[kBlock];
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  for (int q = 0; q < 6; ++q) {
+    const int a = q >> 1;
+    double v = 1.7976931348623157e308;
+    if (i < n) v = (q & 1) ? -xs[a][i] : xs[a][i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock/2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x*6 + q] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_tree_codes(double* const* xs, int n, double x0, double y0, double z0, double box,
+                                                       uint64_t* codes, uint32_t* idx) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double o[3] = {x0, y0, z0};
+  const uint64_t ncell = 1ull << kTreeBits;
+  uint64_t key = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    uint64_t c = 0;
+    if (box > 0.0) {
+      const double u = (xs[a][i] - o[a])/box*(double)ncell;      // TreeGravityInline.hh buildCellKey
+      c = u <= 0.0 ? 0 : (uint64_t)u;
+      c = c < ncell - 1 ? c : ncell - 1;
+    }
+    key |= spread_bits(c, 3) << a;
+  }
+  codes[i] = key;
+  idx[i] = (uint32_t)i;
+}
+
+// Leaf level and first owned level of each sorted point -> number of cells it heads.
+__global__ void __launch_bounds__(kBlock) k_tree_heads(const uint64_t* code, int n, int* nheads, unsigned char* lstart) {
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int lp = p > 0 ? lcp_levels(code[p - 1], code[p]) : -1;
+  const int ln = p + 1 < n ? lcp_levels(code[p], code[p + 1]) : -1;
+  int leaf = 1 + (lp > ln ? lp : ln);
+  leaf = leaf < kTreeBits ? leaf : kTreeBits;
+  if (n == 1) leaf = 0;
+  const int ls = p == 0 ? 0 : 1 + lp;
+  nheads[p] = leaf >= ls ? leaf - ls + 1 : 0;
+  lstart[p] = (unsigned char)(ls > 63 ? 63 : ls);
+}
+
+// Emit the cells (p, l): level, point range, skip pointer.
+__global__ void __launch_bounds__(kBlock) k_tree_cells(const uint64_t* code, int n, const int* cstart, const int* nheads,
+                                                       const unsigned char* lstart, int ncells, int* first, int* last,
+                                                       int* skip, unsigned char* level) {
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int k0 = cstart[p], nh = nheads[p], ls = lstart[p];
+  for (int k = 0; k < nh; ++k) {
+    const int l = ls + k;
+    const int c = k0 + k;
+    // last point sharing the l-prefix: binary search over the sorted keys
+    const int sh = 3*(kTreeBits - l);
+    const uint64_t pre = l == 0 ? 0 : (code[p] >> sh);
+    int lo = p, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      const uint64_t pm = l == 0 ? 0 : (code[mid] >> sh);
+      if (pm == pre) lo = mid; else hi = mid - 1;
+    }
+    first[c] = p;
+    last[c] = lo;
+    level[c] = (unsigned char)l;
+    skip[c] = lo + 1 < n ? cstart[lo + 1] : ncells;
+  }
+}
+
+// Monopoles of the cells at one level, bottom-up (children already done).
+__global__ void __launch_bounds__(kBlock) k_tree_moments(TreeView t, int lvl, double* M, double* cx0, double* cx1, double* cx2,
+                                                         double* rc2) {
+  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  if (c >= t.ncells || t.level[c] != lvl) return;
+  const int f = t.first[c], l = t.last[c];
+  double m = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, xc[3];
+  if (f == l) {
+    m = t.tm[f];
+    xc[0] = t.tx[0][f]; xc[1] = t.tx[1][f]; xc[2] = t.tx[2][f];
+  } else {
+    if (lvl == kTreeBits) {
+      for (int q = f; q <= l; ++q) {
+        const double mq = t.tm[q];
+        m += mq; sx += mq*t.tx[0][q]; sy += mq*t.tx[1][q]; sz += mq*t.tx[2][q];
+      }
+    } else {
+      const int end = t.skip[c];
+      for (int d = c + 1; d < end; d = t.skip[d]) {
+        const double md = M[d];
+        m += md; sx += md*cx0[d]; sy += md*cx1[d]; sz += md*cx2[d];
+      }
+    }
+    xc[0] = sx/m; xc[1] = sy/m; xc[2] = sz/m;
+  }
+  M[c] = m;
+  cx0[c] = xc[0]; cx1[c] = xc[1]; cx2[c] = xc[2];
+  const double cs = t.box/(double)(1u << lvl);
+  const uint64_t key = t.code[f];
+  double r2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    uint64_t ia = 0;
+    for (int b = 0; b < lvl; ++b) ia |= ((key >> (3*(kTreeBits - lvl + b) + a)) & 1ull) << b;
+    const double cc = t.xmin[a] + ((double)ia + 0.5)*cs;
+    const double d = xc[a] - cc;
+    r2 += d*d;
+  }
+  rc2[c] = r2;
+}
+
+__global__ void __launch_bounds__(kBlock) k_tree_gather(double* const* xs, const double* m, const uint32_t* perm, int n,
+                                                        double* tx0, double* tx1, double* tx2, double* tm) {
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t q = perm[p];
+  tx0[p] = xs[0][q]; tx1[p] = xs[1][q]; tx2[p] = xs[2][q];
+  tm[p] = m[q];
+}
+
+// applyTreeForces for the point in tree slot p; results scattered to its hydro slot.
+__global__ void __launch_bounds__(kBlock) k_tree_walk(TreeView t, const uint32_t* perm, double soft2, double open2,
+                                                      double soft, double G, const double* cs2, double* const* acc,
+                                                      double* pot, double* blockMin) {
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  double dtv = 1.7976931348623157e308;
+  if (p < t.n) {
+    const double xi = t.tx[0][p], yi = t.tx[1][p], zi = t.tx[2][p];
+    double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
+    int c = 1;
+    while (c < t.ncells) {
+      const int lvl = t.level[c];
+      const double dx = t.cx[0][c] - xi, dy = t.cx[1][c] - yi, dz = t.cx[2][c] - zi;
+      double r2 = dx*dx + dy*dy + dz*dz;
+      if (r2 > cs2[lvl]/open2 + t.rc2[c]) {
+        const double mag = sqrt(r2);
+        const double inv = mag > 1.0e-50 ? 1.0/mag : 0.0;
+        const double nx = mag > 1.0e-50 ? dx*inv : 1.0, ny = dy*inv, nz = dz*inv;
+        const double rs = r2 + soft2;
+        const double Mc = t.M[c];
+        const double f = Mc*(1.0/rs);
+        ax += f*nx; ay += f*ny; az += f*nz;
+        phi -= Mc*(1.0/sqrt(rs));
+        c = t.skip[c];
+      } else if (t.first[c] == t.last[c] || lvl == kTreeBits) {
+        for (int q = t.first[c]; q <= t.last[c]; ++q) {
+          const double ex = t.tx[0][q] - xi, ey = t.tx[1][q] - yi, ez = t.tx[2][q] - zi;
+          const double q2 = ex*ex + ey*ey + ez*ez;
+          if (q2/soft2 > 1.0e-10) {
+            const double mag = sqrt(q2);
+            const double inv = 1.0/mag;
+            const double rs = q2 + soft2;
+            const double mq = t.tm[q];
+            const double f = mq*(1.0/rs);
+            ax += f*(ex*inv); ay += f*(ey*inv); az += f*(ez*inv);
+            phi -= mq*(1.0/sqrt(rs));
+          }
+        }
+        c = t.skip[c];
+      } else {
+        c = c + 1;
+      }
+    }
+    const double amag = sqrt(ax*ax + ay*ay + az*az);
+    if (amag > 0.0) dtv = sqrt(soft/amag);
+    const uint32_t s = perm[p];
+    acc[0][s] = G*ax;
+    acc[1][s] = G*ay;
+    acc[2][s] = G*az;
+    pot[s] = G*phi;
+  }
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = dtv;
+  __syncthreads();
+  for (int s = kBlock/2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockMin[blockIdx.x] = red[0];
+}
+
+__global__ void k_tree_dt(const double* blockMin, int nb, Scalars* sc, double G, double ftimestep) {
+  __shared__ double red[kBlock];
+  double m = 1.7976931348623157e308;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) m = fmin(m, blockMin[b]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock/2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sc->dtMinAcc = red[0]/sqrt(G);
+    sc->gravDt = ftimestep*sc->dtMinAcc;
+  }
+}
+
+class GravityTree {
+ public:
+  int D = 3;
+  int64_t n = 0;
+  hipStream_t stream = nullptr;
+  bool active = false;
+  std::vector<void*> allocs;
+  uint64_t *codes = nullptr, *codesSorted = nullptr;
+  uint32_t *idx = nullptr, *perm = nullptr;
+  double *tx[3] = {nullptr, nullptr, nullptr}, *tm = nullptr;
+  int *nheads = nullptr, *cstart = nullptr;
+  unsigned char* lstart = nullptr;
+  void* tmp = nullptr;
+  size_t tmpBytes = 0;
+  double* part = nullptr;
+  double* hpart = nullptr;
+  double* cs2 = nullptr;
+  double** accPtrs = nullptr;
+  double** xsPtrs = nullptr;
+  // cells
+  int64_t cellCap = 0;
+  int ncells = 0;
+  int *first = nullptr, *last = nullptr, *skip = nullptr;
+  unsigned char* level = nullptr;
+  double *M = nullptr, *cx[3] = {nullptr, nullptr, nullptr}, *rc2 = nullptr;
+  TreeView view{};
+
+  template <typename T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1)*sizeof(T)));
+    allocs.push_back(p);
+    return static_cast<T*>(p);
+  }
+
+  void init(int dim, int64_t n_, int ld, hipStream_t s, bool enabled) {
+    D = dim;
+    n = n_;
+    stream = s;
+    active = enabled;
+    if (!active) return;
+    if (D != 3) raise(SPX_ERR_UNSUPPORTED, "TreeGravity on the GPU is implemented for 3-D");
+    (void)ld;
+    codes = alloc<uint64_t>(n);
+    codesSorted = alloc<uint64_t>(n);
+    idx = alloc<uint32_t>(n);
+    perm = alloc<uint32_t>(n);
+    for (int a = 0; a < 3; ++a) tx[a] = alloc<double>(n);
+    tm = alloc<double>(n);
+    nheads = alloc<int>(n + 1);
+    cstart = alloc<int>(n + 1);
+    lstart = alloc<unsigned char>(n);
+    const int nb = (int)((n + kBlock - 1)/kBlock);
+    part = alloc<double>((size_t)nb*6 + 8);
+    HIP_CHECK(hipHostMalloc((void**)&hpart, ((size_t)nb*6 + 8)*sizeof(double), hipHostMallocDefault));
+    cs2 = alloc<double>(32);
+    accPtrs = alloc<double*>(3);
+    xsPtrs = alloc<double*>(3);
+    size_t t1 = 0, t2 = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, t1, codes, codesSorted, idx, perm, (size_t)n, 0, 60, stream));
+    HIP_CHECK(rocprim::exclusive_scan(nullptr, t2, nheads, cstart, 0, (size_t)n + 1, rocprim::plus<int>(), stream));
+    tmpBytes = std::max(t1, t2);
+    tmp = alloc<char>(tmpBytes);
+  }
+
+  void release() {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    if (hpart) (void)hipHostFree(hpart);
+    hpart = nullptr;
+  }
+
+  void ensureCells(int64_t need) {
+    if (need <= cellCap) return;
+    const int64_t cap = need + need/4 + 1024;
+    first = alloc<int>(cap);
+    last = alloc<int>(cap);
+    skip = alloc<int>(cap);
+    level = alloc<unsigned char>(cap);
+    M = alloc<double>(cap);
+    for (int a = 0; a < 3; ++a) cx[a] = alloc<double>(cap);
+    rc2 = alloc<double>(cap);
+    cellCap = cap;
+  }
+
+  void build(double* const* xs, const double* m, int nn) {
+    const int nb = (int)((nn + kBlock - 1)/kBlock);
+    double* hx[3] = {xs[0], xs[1], xs[2]};
+    HIP_CHECK(hipMemcpyAsync(xsPtrs, hx, sizeof(hx), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_tree_bbox, dim3(nb), dim3(kBlock), 0, stream, xsPtrs, nn, part);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(hpart, part, (size_t)nb*6*sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) { lo[a] = 1.7976931348623157e308; hi[a] = -1.7976931348623157e308; }
+    for (int b = 0; b < nb; ++b)
+      for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], hpart[b*6 + 2*a]); hi[a] = std::max(hi[a], -hpart[b*6 + 2*a + 1]); }
+    double box = 0.0;
+    for (int a = 0; a < 3; ++a) box = std::max(box, std::abs(hi[a] - lo[a]));
+    hipLaunchKernelGGL(k_tree_codes, dim3(nb), dim3(kBlock), 0, stream, xsPtrs, nn, lo[0], lo[1], lo[2], box, codes, idx);
+    HIP_CHECK(hipGetLastError());
+    size_t t = tmpBytes;
+    HIP_CHECK(rocprim::radix_sort_pairs(tmp, t, codes, codesSorted, idx, perm, (size_t)nn, 0, 60, stream));
+    hipLaunchKernelGGL(k_tree_gather, dim3(nb), dim3(kBlock), 0, stream, xsPtrs, m, perm, nn, tx[0], tx[1], tx[2], tm);
+    hipLaunchKernelGGL(k_tree_heads, dim3(nb), dim3(kBlock), 0, stream, codesSorted, nn, nheads, lstart);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemsetAsync(nheads + nn, 0, sizeof(int), stream));
+    t = tmpBytes;
+    HIP_CHECK(rocprim::exclusive_scan(tmp, t, nheads, cstart, 0, (size_t)nn + 1, rocprim::plus<int>(), stream));
+    int total = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, cstart + nn, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    ncells = total;
+    ensureCells(total);
+    hipLaunchKernelGGL(k_tree_cells, dim3(nb), dim3(kBlock), 0, stream, codesSorted, nn, cstart, nheads, lstart, total,
+                       first, last, skip, level);
+    HIP_CHECK(hipGetLastError());
+    view.tx[0] = tx[0]; view.tx[1] = tx[1]; view.tx[2] = tx[2];
+    view.tm = tm;
+    view.code = codesSorted;
+    view.first = first; view.last = last; view.skip = skip; view.level = level;
+    view.M = M; view.cx[0] = cx[0]; view.cx[1] = cx[1]; view.cx[2] = cx[2]; view.rc2 = rc2;
+    view.ncells = total;
+    view.n = nn;
+    for (int a = 0; a < 3; ++a) view.xmin[a] = lo[a];
+    view.box = box;
+    const int cb = (int)((total + kBlock - 1)/kBlock);
+    for (int l = kTreeBits; l >= 0; --l) {
+      hipLaunchKernelGGL(k_tree_moments, dim3(cb), dim3(kBlock), 0, stream, view, l, M, cx[0], cx[1], cx[2], rc2);
+    }
+    HIP_CHECK(hipGetLastError());
+    double h2[32];
+    for (int l = 0; l < 32; ++l) h2[l] = l <= kTreeBits ? (box*box)/(double)(1ull << (2*l)) : 0.0;
+    HIP_CHECK(hipMemcpyAsync(cs2, h2, sizeof(h2), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  void evaluate(double* const* acc, double* pot, int nn, double G, double soft, double opening, Scalars* sc,
+                double ftimestep) {
+    const int nb = (int)((nn + kBlock - 1)/kBlock);
+    double* ha[3] = {acc[0], acc[1], acc[2]};
+    HIP_CHECK(hipMemcpyAsync(accPtrs, ha, sizeof(ha), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_tree_walk, dim3(nb), dim3(kBlock), 0, stream, view, perm, soft*soft, opening*opening, soft, G, cs2,
+                       accPtrs, pot, part);
+    hipLaunchKernelGGL(k_tree_dt, dim3(1), dim3(kBlock), 0, stream, part, nb, sc, G, ftimestep);
+    HIP_CHECK(hipGetLastError());
+  }
+};
+
 }  // namespace spx

@@ -172,3 +172,25 @@ def test_neighbor_symmetry_at_scale(gpu):
     fwd = fwd[np.lexsort(fwd.T[::-1])]
     bwd = bwd[np.lexsort(bwd.T[::-1])]
     assert np.array_equal(fwd, bwd)
+
+
+@pytest.mark.parametrize("n,opening", [(2000, 0.5), (20000, 0.5), (3000, 0.05)])
+def test_tree_gravity_matches_oracle(gpu, oracle_mod, n, opening):
+    """Same octree, same acceptance decisions: accelerations and potential at roundoff."""
+    print(f"[case] evrard n={n} opening={opening}", flush=True)
+    cfg, st = problems.evrard_3d(n)
+    cfg.opening = opening
+    g, o = pair(gpu, oracle_mod, cfg, st)
+    g.evaluate_derivatives()
+    o.evaluate_derivatives()
+    ag, ao = g.field("delta velocity"), o.field("delta velocity")
+    assert np.all(np.abs(ag - ao) <= TOL*np.abs(ao).max())
+    pg, po = g.field("gravitational potential"), o.field("gravitational potential")
+    assert np.all(np.abs(pg - po) <= TOL*np.abs(po).max())
+    gh, gg = g.dt()
+    oh, og = o.dt()
+    assert abs(gg - og) <= TOL*og
+    for _ in range(3):
+        assert abs(g.step() - o.step()) <= TOL*o.time()[2]
+    a, b = g.field("velocity"), o.field("velocity")
+    assert np.all(np.abs(a - b) <= 1e-9*max(np.abs(b).max(), 1e-300))

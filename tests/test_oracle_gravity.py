@@ -1,0 +1,75 @@
+"""Pins of the oracle's TreeGravity (Gravity/TreeGravity.cc): with the opening angle -> 0 no
+cell monopole is ever accepted (TreeGravity.cc:882), so the walk reduces to the direct sum
+of the Plummer-like force law over all other points (TreeGravity.cc:60-90)."""
+import numpy as np
+import pytest
+
+from spheral_amd import problems
+
+
+def direct(x, m, G, soft):
+    d = x[None, :, :] - x[:, None, :]               # x_j - x_i
+    r2 = np.sum(d*d, 2)
+    mask = r2/(soft*soft) > 1e-10
+    r = np.sqrt(np.where(mask, r2, 1.0))
+    rs = r2 + soft*soft
+    f = np.where(mask, m[None, :]/rs/r, 0.0)
+    a = G*np.sum(f[:, :, None]*d, 1)
+    phi = -G*np.sum(np.where(mask, m[None, :]/np.sqrt(rs), 0.0), 1)
+    return a, phi
+
+
+@pytest.mark.parametrize("n", [300, 1500])
+def test_zero_opening_is_direct_summation(oracle_mod, n):
+    cfg, st = problems.evrard_3d(n)
+    cfg.opening = 1.0e-4
+    cfg.G = 2.0
+    r = oracle_mod.OracleRun(cfg, n)
+    r.set_state(**st)
+    r.initialize()
+    r.evaluate_derivatives()
+    a = r.field("delta velocity")
+    phi = r.field("gravitational potential")
+    ad, pd = direct(st["x"], st["m"], cfg.G, cfg.softening)
+    assert np.allclose(a, ad, rtol=1e-10, atol=1e-10*np.abs(ad).max())
+    assert np.allclose(phi, pd, rtol=1e-12)
+
+
+def test_monopole_error_is_small_and_shrinks_with_opening(oracle_mod):
+    n = 2000
+    cfg, st = problems.evrard_3d(n)
+    ad, _ = direct(st["x"], st["m"], cfg.G, cfg.softening)
+    errs = []
+    for th in (1.0, 0.5, 0.25):
+        cfg.opening = th
+        r = oracle_mod.OracleRun(cfg, n)
+        r.set_state(**st)
+        r.initialize()
+        r.evaluate_derivatives()
+        a = r.field("delta velocity")
+        errs.append(np.median(np.linalg.norm(a - ad, axis=1)/np.linalg.norm(ad, axis=1)))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[1] < 2e-2
+
+
+def test_uniform_sphere_potential(oracle_mod):
+    """Uniform-density ball, centre potential -1.5 GM/R (softening small)."""
+    rng = np.random.default_rng(1)
+    n = 6000
+    u = rng.uniform(0, 1, n)**(1/3)
+    d = rng.normal(size=(n, 3))
+    x = d/np.linalg.norm(d, axis=1)[:, None]*u[:, None]
+    cfg, st = problems.evrard_3d(10)
+    st = dict(x=x, v=np.zeros_like(x), m=np.full(n, 1.0/n), rho=np.ones(n), eps=np.full(n, 0.05),
+              H=np.tile(np.eye(3)*20.0, (n, 1, 1)))
+    cfg.softening = 1e-3
+    cfg.opening = 0.3
+    r = oracle_mod.OracleRun(cfg, n)
+    r.set_state(**st)
+    r.initialize()
+    r.evaluate_derivatives()
+    phi = r.field("gravitational potential")
+    rr = np.linalg.norm(x, axis=1)
+    inner = rr < 0.2
+    expect = -0.5*(3.0 - rr[inner]**2)          # -GM(3R^2 - r^2)/(2R^3)
+    assert np.median(np.abs(phi[inner] - expect)) < 0.03
