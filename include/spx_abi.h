@@ -85,11 +85,12 @@ typedef struct spx_config {
   int32_t bc_lo[3], bc_hi[3];        /* SPX_BC_* per axis side                       */
   double  xmin[3], xmax[3];          /* boundary plane positions per axis            */
   int32_t gravity;                   /* 1: TreeGravity package appended              */
-  int32_t reorder_interval;          /* steps between Morton reorders (GPU layout)   */
+  int32_t reorder_interval;          /* 0 or 1: particles are re-sorted every step   */
   double  G, softening, opening, ftimestep;
 } spx_config;
 
 typedef struct spx_run spx_run;
+typedef struct spx_batch spx_batch;
 
 /* ---- library / errors ---------------------------------------------------- */
 const char* spx_last_error(void);
@@ -133,7 +134,11 @@ int spx_run_set_state(spx_run* run, const double* x, const double* v,
    "max viscous pressure", "effective viscous pressure", "mass zeroth moment",
    "mass first moment", "M SPH correction", "mass density gradient",
    "normalization", "new mass density", "gravitational potential".
-   *count receives the number of doubles written (n * components). */
+   *count receives the number of doubles written (n * components).
+   "new H" and "mass zeroth moment" come from the last step or derivative
+   evaluation; "effective viscous pressure", "mass first moment", "M SPH correction",
+   "mass density gradient", "normalization" and "new mass density" only from
+   spx_run_evaluate_derivatives (SPX_ERR_STATE when not in place). */
 int spx_run_get_field(spx_run* run, const char* name, double* out, int64_t capacity,
                       int64_t* count);
 
@@ -181,7 +186,7 @@ int spx_run_post_state_update(spx_run* run);
 /* SPH::evaluateDerivatives + SPHSmoothingScale::evaluateDerivatives (+ TreeGravity
    initialize/evaluateDerivatives if enabled) on the current state
    (SPH/SPH.cc:165-555, SmoothingScale/SPHSmoothingScale.cc:101-275,
-   Gravity/TreeGravity.cc:143-328).  Derivatives are zeroed first. */
+   Gravity/TreeGravity.cc:143-328).  Every derivative output is overwritten. */
 int spx_run_evaluate_derivatives(spx_run* run, double t, double dt);
 /* Physics::dt votes: GenericHydro::dt (Physics/GenericHydro.cc:112-381, includes the
    cfl factor) and TreeGravity::dt (Gravity/TreeGravity.cc:483-497). */
@@ -201,6 +206,50 @@ int spx_run_kernel_times(spx_run* run, double* ms, int32_t max_entries,
                          int32_t* n_entries, char* names, int64_t names_cap);
 /* Unique pairs of the current connectivity (NodePairList size). */
 int spx_run_num_pairs(spx_run* run, int64_t* npairs);
+
+/* ---- batches: many independent runs of one problem type --------------------
+   The reference advances one problem per process: SpheralController.advance
+   (SimulationControl/SpheralController.py:383-426) loops Integrator::step on one
+   DataBase.  A batch holds many such problems in one HBM-resident set of arrays
+   and steps all of them together, each with its own dt (Integrator::selectDt per
+   run), neighbour grid and end time; pairs never cross runs.  This serves
+   ensembles and request streams of small/medium problems that cannot fill the GPU
+   alone.  Every run of a batch shares cfg (kernel, EOS, viscosity, boundary
+   planes); TreeGravity batches hold one run.  A run is stepped until its time
+   reaches t_end (the last step is clamped to land on t_end), then it is DONE and
+   its state stays readable until it is removed. */
+#define SPX_RUN_ACTIVE 1
+#define SPX_RUN_DONE   2
+
+/* capacity: total particles resident at once; max_runs: run slots (<= 32768). */
+int  spx_batch_create(const spx_config* cfg, int64_t capacity, int32_t max_runs,
+                      int32_t device, spx_batch** out);
+void spx_batch_destroy(spx_batch* batch);
+/* Admit a run (state layout as spx_run_set_state).  Its start-up
+   (reinitializeProblem subset, as spx_run_initialize) happens at the next step.
+   *run receives the run id.  SPX_ERR_CAPACITY when particles or slots run out. */
+int spx_batch_add_run(spx_batch* batch, const double* x, const double* v,
+                      const double* m, const double* rho, const double* eps,
+                      const double* H, int64_t n, double t_end, int32_t* run);
+/* Release a run's slot and particles (reclaimed at the next step or admission). */
+int spx_batch_remove_run(spx_batch* batch, int32_t run);
+/* One Integrator::step of every active run; *active_runs receives the number of
+   runs still active afterwards. */
+int spx_batch_step(spx_batch* batch, int32_t* active_runs);
+/* State of one run: SPX_RUN_ACTIVE or SPX_RUN_DONE, time, cycle, last dt, nodes. */
+int spx_batch_run_status(spx_batch* batch, int32_t run, int32_t* state, double* t,
+                         int64_t* cycle, double* last_dt, int64_t* n);
+/* As spx_run_get_field for one run.  "new H" and "mass zeroth moment" are kept
+   only until the batch's next step; the evaluate-only outputs are not produced. */
+int spx_batch_get_field(spx_batch* batch, int32_t run, const char* name, double* out,
+                        int64_t capacity, int64_t* count);
+/* As spx_run_conservation for one run. */
+int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7);
+/* Occupancy for admission control: particles resident and run slots in use. */
+int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs);
+/* As spx_run_kernel_times for the batch's last step (max_entries < 0 enables timing). */
+int spx_batch_kernel_times(spx_batch* batch, double* ms, int32_t max_entries,
+                           int32_t* n_entries, char* names, int64_t names_cap);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,1032 @@
+// spx_batch.hpp -- host orchestration: a batch of independent runs of one problem type,
+// resident in HBM and advanced together by CheapSynchronousRK2, each with its own dt.
+//
+// A single spx_run is a batch with one slot whose end time follows the caller's
+// maxTime.  Runs of a batch share the physics configuration (kernel, EOS, viscosity,
+// boundary planes) and differ in particle count, state, cell grid, dt and end time.
+// Per step the host enqueues, on the batch's stream, the reference's call order
+// (Integrator/Integrator.cc:67-111, Integrator/CheapSynchronousRK2.cc:40-132) and
+// reads back three small things: the per-run extents (to size the cell grids), the
+// neighbour-capacity overflow word, and the per-run times.
+//
+// Particle order in the arrays: the active runs first, sorted by (run, cell Morton
+// code), then the finished runs, then removed runs until they are squeezed out.
+#pragma once
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "spx_error.hpp"
+#include "spx_gravity.hpp"
+#include "spx_sph.hpp"
+#include "spx_table.hpp"
+
+namespace spx {
+
+static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1)/kBlock); }
+
+// A run has reached its end time (a clamped last step may land an ulp short).
+static inline bool reached_end(double t, double tEnd) { return t >= tEnd || tEnd - t <= 1.0e-15*std::fabs(tEnd); }
+
+// EOS of the nodes of newly admitted runs (runMask[run] != 0).
+__global__ void __launch_bounds__(kBlock) k_eos_masked(Fields f, Phys ph, const int* runMask, int n) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n || !runMask[f.run[i]]) return;
+  f.P[i] = ph.g1*f.rho[i]*f.eps[i];
+  f.cs[i] = sqrt(fmax(0.0, ph.gamma*ph.g1*f.eps[i]));
+}
+
+class Batch {
+ public:
+  struct RunInfo {
+    int state = RUN_FREE;
+    int64_t n = 0;
+    bool needInit = false;
+    bool stepOut = false;   // "new H", "mass zeroth moment" of this run's last evaluation are in place
+    bool fullOut = false;   // the evaluate-only outputs (sum density, M, ...) are in place
+  };
+
+  spx_config cfg;
+  int D;
+  int device;
+  int64_t cap;           // particle capacity of the batch
+  int maxRuns;
+  bool retire;           // batch mode: a run that reaches its end time stops; single-run mode: never
+  hipStream_t stream = nullptr;
+  KernelTable table;
+  DevTable dtab{};
+  Phys ph{};
+  Geo geo{};
+  Fields f{};
+  Fields alt{};          // reorder targets of the carried arrays
+  GravityTree grav;
+  std::vector<RunInfo> runs;
+  std::vector<void*> allocs;
+  // device side
+  RunGrid* dgrids = nullptr;
+  Scalars* dsc = nullptr;
+  unsigned long long* dext = nullptr;
+  int* dstate = nullptr;
+  int* dinit = nullptr;
+  int* dOverflow = nullptr;
+  double** xsMid = nullptr;
+  double** xsState = nullptr;
+  uint64_t *keys = nullptr, *keysAlt = nullptr;
+  uint32_t *vals = nullptr, *valsAlt = nullptr;
+  void* sortTmp = nullptr;
+  size_t sortTmpBytes = 0;
+  int* cellStart = nullptr;
+  int* cellEnd = nullptr;
+  int64_t cellsAlloc = 0;
+  int64_t* oidAlt = nullptr;
+  int32_t* runAlt = nullptr;
+  // pinned host mirrors
+  RunGrid* hgrids = nullptr;
+  Scalars* hsc = nullptr;
+  unsigned long long* hext = nullptr;
+  int* hstate = nullptr;
+  int* hinit = nullptr;
+  int* hOverflow = nullptr;
+  double** hptrs = nullptr;
+  int ld = 0;
+  int mortonBits = 0, runBits = 0;
+  int64_t nActive = 0;    // particles of active runs (front of the arrays after a sort)
+  int64_t nTotal = 0;     // particles resident (active, done, removed-but-not-yet-dropped)
+  bool initialized = false, connValid = false, wrapPending = false;
+  int64_t npairsCache = -1;
+  bool mixed = false;
+  // per-class kernel time of the last step (ms)
+  bool timing = false;
+  std::vector<std::string> tnames;
+  std::vector<double> tms;
+  std::vector<hipEvent_t> evs;
+
+  Batch(const spx_config& c, int64_t capacity, int nruns, int dev, bool retireDone)
+      : cfg(c), D(c.dim), device(dev), cap(capacity), maxRuns(nruns), retire(retireDone) {
+    validate();
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    table.build(cfg.kernel, D, (unsigned)cfg.kernel_points, cfg.min_nperh, cfg.max_nperh);
+    uploadTable();
+    setPhys();
+    mixed = cfg.precision == SPX_PREC_MIXED;
+    ld = (int)((cap + 255)/256*256);
+    runs.resize(maxRuns);
+    runBits = 0;
+    while ((1 << runBits) < maxRuns) ++runBits;
+    allocate();
+    grav.init(D, cap, ld, stream, cfg.gravity != 0);
+  }
+  ~Batch() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : evs) (void)hipEventDestroy(e);
+    for (void* p : allocs) (void)hipFree(p);
+    for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow, (void*)hptrs})
+      if (p) (void)hipHostFree(p);
+    grav.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void validate() {
+    if (cfg.dim < 1 || cfg.dim > 3) raise(SPX_ERR_ARG, "dim must be 1, 2 or 3");
+    if (cap < 1 || cap > (int64_t)kIdxMask) raise(SPX_ERR_CAPACITY, "particle capacity must be in [1, 2^26)");
+    if (maxRuns < 1 || maxRuns > 32768) raise(SPX_ERR_CAPACITY, "run slots must be in [1, 32768]");
+    if (cfg.kernel != SPX_KERNEL_BSPLINE && cfg.kernel != SPX_KERNEL_WENDLANDC4) raise(SPX_ERR_ARG, "unknown kernel");
+    if (cfg.kernel_points < 3) raise(SPX_ERR_ARG, "kernel_points must be >= 3");
+    if (!cfg.compatible_energy) raise(SPX_ERR_UNSUPPORTED, "only compatibleEnergyEvolution=True is implemented");
+    if (cfg.precision != SPX_PREC_FP64 && cfg.precision != SPX_PREC_MIXED) raise(SPX_ERR_ARG, "unknown precision");
+    if (cfg.neighbor_extent <= 0.0) raise(SPX_ERR_ARG, "neighbor_extent must be positive");
+    if (cfg.reorder_interval > 1)
+      raise(SPX_ERR_UNSUPPORTED, "the engine re-sorts particles every step (reorder_interval must be 0 or 1)");
+    for (int a = 0; a < 3; ++a) {
+      const bool per = cfg.bc_lo[a] == SPX_BC_PERIODIC || cfg.bc_hi[a] == SPX_BC_PERIODIC;
+      if (per && cfg.bc_lo[a] != cfg.bc_hi[a]) raise(SPX_ERR_ARG, "periodic must be set on both sides of an axis");
+      if (a < cfg.dim && (cfg.bc_lo[a] != SPX_BC_NONE || cfg.bc_hi[a] != SPX_BC_NONE) && !(cfg.xmax[a] > cfg.xmin[a]))
+        raise(SPX_ERR_ARG, "boundary planes need xmax > xmin");
+    }
+    if (cfg.gravity && (cfg.G <= 0.0 || cfg.softening <= 0.0 || cfg.opening <= 0.0 || cfg.ftimestep <= 0.0))
+      raise(SPX_ERR_ARG, "TreeGravity needs G, softening, opening, ftimestep > 0");
+    if (cfg.gravity && maxRuns != 1) raise(SPX_ERR_UNSUPPORTED, "a TreeGravity batch holds one run");
+  }
+
+  template <typename T>
+  T* dalloc(size_t count) {
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1)*sizeof(T)));
+    allocs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* halloc(size_t count) {
+    void* p = nullptr;
+    HIP_CHECK(hipHostMalloc(&p, std::max<size_t>(count, 1)*sizeof(T), hipHostMallocDefault));
+    std::memset(p, 0, std::max<size_t>(count, 1)*sizeof(T));
+    return static_cast<T*>(p);
+  }
+  // Host -> device on the batch's stream, complete on return.
+  void upload(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+  void download(void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  void uploadTable() {
+    const size_t nint = table.W.N1 + 1;
+    std::vector<double> coef(6*nint);
+    std::vector<float> coef32(6*nint);
+    for (size_t k = 0; k < nint; ++k)
+      for (int c = 0; c < 3; ++c) { coef[6*k + c] = table.W.coef[3*k + c]; coef[6*k + 3 + c] = table.gW.coef[3*k + c]; }
+    for (size_t k = 0; k < coef.size(); ++k) coef32[k] = (float)coef[k];
+    double* dc = dalloc<double>(coef.size());
+    float* dc32 = dalloc<float>(coef32.size());
+    double* dh = dalloc<double>(table.nperh.vals.size());
+    upload(dc, coef.data(), coef.size()*sizeof(double));
+    upload(dc32, coef32.data(), coef32.size()*sizeof(float));
+    upload(dh, table.nperh.vals.data(), table.nperh.vals.size()*sizeof(double));
+    dtab.coef = dc;
+    dtab.coef32 = dc32;
+    dtab.herm = dh;
+    dtab.n1 = (int)table.W.N1;
+    dtab.inv_xstep = 1.0/table.W.xstep;
+    dtab.extent = table.extent;
+    dtab.W0 = table.W(0.0);
+    dtab.hn = (int)table.nperh.N;
+    dtab.hxmin = table.nperh.xmin;
+    dtab.hxmax = table.nperh.xmax;
+    dtab.hxstep = table.nperh.xstep;
+  }
+
+  void setPhys() {
+    ph.gamma = cfg.gamma;
+    ph.g1 = cfg.gamma - 1.0;
+    ph.Cl = cfg.Cl;
+    ph.Cq = cfg.Cq;
+    ph.cfl = cfg.cfl;
+    ph.nperh = cfg.nperh;
+    ph.ext2 = cfg.neighbor_extent*cfg.neighbor_extent;
+    ph.hmin = cfg.hmin;
+    ph.hmax = cfg.hmax;
+    ph.rho_min = cfg.rho_min;
+    ph.rho_max = cfg.rho_max;
+    ph.hmaxInv = 1.0/cfg.hmax;
+    ph.hminInv = 1.0/cfg.hmin;
+    ph.correctVG = cfg.correct_velocity_gradient;
+    ph.useVmagDt = cfg.use_velocity_magnitude_for_dt;
+    ph.gravity = cfg.gravity;
+    geo.permask = 0;
+    geo.reflmask = 0;
+    for (int a = 0; a < 3; ++a) {
+      geo.bc_lo[a] = a < D ? cfg.bc_lo[a] : BC_NONE;
+      geo.bc_hi[a] = a < D ? cfg.bc_hi[a] : BC_NONE;
+      geo.xmin[a] = cfg.xmin[a];
+      geo.xmax[a] = cfg.xmax[a];
+      if (geo.bc_lo[a] == BC_PERIODIC) geo.permask |= 1 << a;
+      if (geo.bc_lo[a] == BC_REFLECTING) geo.reflmask |= 1 << (2*a);
+      if (geo.bc_hi[a] == BC_REFLECTING) geo.reflmask |= 1 << (2*a + 1);
+    }
+  }
+
+  // Arrays that persist across steps and follow the reorder.
+  std::vector<double**> carried(Fields& F) {
+    std::vector<double**> v;
+    for (int a = 0; a < D; ++a) { v.push_back(&F.x[a]); v.push_back(&F.v[a]); }
+    for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.pot})
+      v.push_back(p);
+    for (int a = 0; a < D; ++a) { v.push_back(&F.DxDt[a]); v.push_back(&F.DvDt[a]); v.push_back(&F.DvDtG[a]); }
+    return v;
+  }
+
+  void allocate() {
+    std::vector<double**> all;
+    Fields& F = f;
+    for (int a = 0; a < D; ++a)
+      for (double** p : {&F.x[a], &F.v[a], &F.xm[a], &F.vm[a], &F.v1[a], &F.DxDt[a], &F.DvDt[a], &F.DvDtG[a],
+                         &F.gradRho[a], &F.m1[a]})
+        all.push_back(p);
+    for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.rhom, &F.epsm, &F.hinvm, &F.Pm,
+                       &F.csm, &F.omegam, &F.prhom, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.Hideal, &F.m0,
+                       &F.rhoSum, &F.norm, &F.effQ, &F.ns, &F.pot})
+      all.push_back(p);
+    const size_t total = (all.size() + 2*(size_t)D*D)*(size_t)ld;   // + DvDx, Mcorr
+    double* slab = dalloc<double>(total);
+    HIP_CHECK(hipMemsetAsync(slab, 0, total*sizeof(double), stream));
+    size_t off = 0;
+    for (double** p : all) { *p = slab + off; off += ld; }
+    F.DvDx = slab + off; off += (size_t)D*D*ld;
+    F.Mcorr = slab + off;
+    for (int a = D; a < 3; ++a)
+      F.x[a] = F.v[a] = F.xm[a] = F.vm[a] = F.v1[a] = F.DxDt[a] = F.DvDt[a] = F.DvDtG[a] = F.gradRho[a] = F.m1[a] = nullptr;
+    alt = f;
+    auto cl = carried(alt);
+    const size_t altTotal = (cl.size() + (size_t)D*D)*(size_t)ld;
+    double* slabAlt = dalloc<double>(altTotal);
+    HIP_CHECK(hipMemsetAsync(slabAlt, 0, altTotal*sizeof(double), stream));
+    off = 0;
+    for (double** p : cl) { *p = slabAlt + off; off += ld; }
+    alt.DvDx = slabAlt + off;
+    f.oid = dalloc<int64_t>(ld);
+    oidAlt = dalloc<int64_t>(ld);
+    f.run = dalloc<int32_t>(ld);
+    runAlt = dalloc<int32_t>(ld);
+    f.cnt = dalloc<int32_t>(ld);
+    HIP_CHECK(hipMemsetAsync(f.cnt, 0, ld*sizeof(int32_t), stream));
+    f.ld = ld;
+    f.cap = 0;
+    f.nbr = nullptr;
+    keys = dalloc<uint64_t>(ld);
+    keysAlt = dalloc<uint64_t>(ld);
+    vals = dalloc<uint32_t>(ld);
+    valsAlt = dalloc<uint32_t>(ld);
+    size_t tmp = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys, keysAlt, vals, valsAlt, (size_t)ld, 0, 64, stream));
+    sortTmpBytes = tmp;
+    sortTmp = dalloc<char>(tmp);
+    dgrids = dalloc<RunGrid>(maxRuns);
+    dsc = dalloc<Scalars>(maxRuns);
+    dext = dalloc<unsigned long long>((size_t)maxRuns*7);
+    dstate = dalloc<int>(maxRuns);
+    dinit = dalloc<int>(maxRuns);
+    dOverflow = dalloc<int>(1);
+    xsMid = dalloc<double*>(3);
+    xsState = dalloc<double*>(3);
+    hgrids = halloc<RunGrid>(maxRuns);
+    hsc = halloc<Scalars>(maxRuns);
+    hext = halloc<unsigned long long>((size_t)maxRuns*7);
+    hstate = halloc<int>(maxRuns);
+    hinit = halloc<int>(maxRuns);
+    hOverflow = halloc<int>(1);
+    hptrs = halloc<double*>(6);
+    upload(dsc, hsc, sizeof(Scalars)*maxRuns);
+    ensureNbrCapacity(D == 1 ? 16 : (D == 2 ? 32 : 64));
+  }
+
+  void ensureNbrCapacity(int c) {
+    if (c <= f.cap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    uint32_t* p = nullptr;
+    HIP_CHECK(hipMalloc((void**)&p, (size_t)c*ld*sizeof(uint32_t)));
+    if (f.nbr) {
+      auto it = std::find(allocs.begin(), allocs.end(), (void*)f.nbr);
+      if (it != allocs.end()) allocs.erase(it);
+      HIP_CHECK(hipFree(f.nbr));
+    }
+    allocs.push_back(p);
+    f.nbr = p;
+    f.cap = c;
+  }
+
+  template <typename F>
+  void launchD(F&& fn) {
+    switch (D) {
+      case 1: fn(std::integral_constant<int, 1>{}); break;
+      case 2: fn(std::integral_constant<int, 2>{}); break;
+      default: fn(std::integral_constant<int, 3>{}); break;
+    }
+  }
+
+  // ---------------------------------------------------------------- run admission / retirement
+  int addRun(const double* x, const double* v, const double* m, const double* rho, const double* eps, const double* H,
+             int64_t n, double tEnd, int slot = -1) {
+    if (!x || !v || !m || !rho || !eps || !H) raise(SPX_ERR_ARG, "null state pointer");
+    if (n < 1) raise(SPX_ERR_ARG, "a run needs at least one node");
+    if (!(tEnd >= 0.0)) raise(SPX_ERR_ARG, "end time must be >= 0");
+    // validate the whole input before touching the device
+    std::vector<double> hbuf((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+      const double* Hi = H + i*D*D;
+      const double h = Hi[0];
+      for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+          if (Hi[a*D + b] != (a == b ? h : 0.0))
+            raise(SPX_ERR_UNSUPPORTED, "H must be isotropic (h^-1 I); ASPH smoothing tensors are not implemented");
+      if (!(h > 0.0) || !std::isfinite(h)) raise(SPX_ERR_ARG, "H must be positive definite");
+      if (!(m[i] > 0.0) || !(rho[i] > 0.0)) raise(SPX_ERR_ARG, "mass and mass density must be positive");
+      hbuf[i] = h;
+    }
+    bool anyRemoved = false;
+    for (auto& r : runs) anyRemoved |= r.state == RUN_REMOVED;
+    if (slot < 0) {
+      for (int s = 0; s < maxRuns && slot < 0; ++s) if (runs[s].state == RUN_FREE) slot = s;
+      if (slot < 0 && anyRemoved) {
+        compact();
+        anyRemoved = false;
+        for (int s = 0; s < maxRuns && slot < 0; ++s) if (runs[s].state == RUN_FREE) slot = s;
+      }
+      if (slot < 0) raise(SPX_ERR_CAPACITY, "no free run slot");
+    } else if (slot >= maxRuns || runs[slot].state != RUN_FREE) {
+      raise(SPX_ERR_STATE, "run slot in use");
+    }
+    if (nTotal + n > cap && anyRemoved) compact();
+    if (nTotal + n > cap) raise(SPX_ERR_CAPACITY, "particle capacity of the batch exceeded");
+    HIP_CHECK(hipStreamSynchronize(stream));
+    const int64_t o = nTotal;
+    std::vector<double> buf((size_t)n);
+    auto up = [&](double* dst, const double* src) { upload(dst + o, src, n*sizeof(double)); };
+    auto zero = [&](double* dst) { HIP_CHECK(hipMemsetAsync(dst + o, 0, n*sizeof(double), stream)); };
+    for (int a = 0; a < D; ++a) {
+      for (int64_t i = 0; i < n; ++i) buf[i] = x[i*D + a];
+      up(f.x[a], buf.data());
+      for (int64_t i = 0; i < n; ++i) buf[i] = v[i*D + a];
+      up(f.v[a], buf.data());
+      zero(f.DxDt[a]); zero(f.DvDt[a]); zero(f.DvDtG[a]);
+    }
+    up(f.hinv, hbuf.data());
+    up(f.m, m);
+    up(f.rho, rho);
+    up(f.eps, eps);
+    for (int64_t i = 0; i < n; ++i) buf[i] = 1.0;
+    up(f.omega, buf.data());
+    // derivatives start at zero (SPHBase::initializeProblemStartup)
+    for (double* p : {f.DepsDt, f.DrhoDt, f.DHDt, f.maxQ, f.pot, f.P, f.cs}) zero(p);
+    for (int c = 0; c < D*D; ++c) HIP_CHECK(hipMemsetAsync(f.DvDx + (size_t)c*ld + o, 0, n*sizeof(double), stream));
+    std::vector<int64_t> ids(n);
+    std::vector<int32_t> rr(n, slot);
+    for (int64_t i = 0; i < n; ++i) ids[i] = ((int64_t)slot << 40) | i;
+    upload(f.oid + o, ids.data(), n*sizeof(int64_t));
+    upload(f.run + o, rr.data(), n*sizeof(int32_t));
+    RunInfo info;
+    info.state = RUN_ACTIVE;
+    info.n = n;
+    info.needInit = true;
+    runs[slot] = info;
+    Scalars s;
+    std::memset(&s, 0, sizeof(Scalars));
+    s.lastDt = cfg.last_dt;
+    s.tEnd = tEnd;
+    s.active = 1;
+    hsc[slot] = s;
+    upload(dsc + slot, &hsc[slot], sizeof(Scalars));
+    nTotal += n;
+    if (!retire) nActive = nTotal;     // a single run is always active, in upload order until the first sort
+    connValid = false;
+    if (retire && tEnd <= 0.0) markDone(slot);
+    return slot;
+  }
+
+  void markDone(int slot) {
+    runs[slot].state = RUN_DONE;
+    hsc[slot].active = 0;
+    upload(&dsc[slot].active, &hsc[slot].active, sizeof(int));
+  }
+
+  void removeRun(int slot) {
+    checkSlot(slot);
+    runs[slot].state = RUN_REMOVED;
+    hsc[slot].active = 0;
+    upload(&dsc[slot].active, &hsc[slot].active, sizeof(int));
+  }
+
+  void checkSlot(int slot) const {
+    if (slot < 0 || slot >= maxRuns || runs[slot].state == RUN_FREE || runs[slot].state == RUN_REMOVED)
+      raise(SPX_ERR_ARG, "no such run");
+  }
+
+  // Squeeze removed runs out of the arrays (no cell grids needed).
+  void compact() {
+    for (int r = 0; r < maxRuns; ++r) {
+      RunGrid& g = hgrids[r];
+      for (int a = 0; a < 3; ++a) { g.origin[a] = 0.0; g.inv_cell[a] = 0.0; g.nc[a] = 1; }
+      g.cellBase = 0;
+    }
+    HIP_CHECK(hipMemcpyAsync(dgrids, hgrids, sizeof(RunGrid)*maxRuns, hipMemcpyHostToDevice, stream));
+    sortAndGather(0);
+    connValid = false;
+  }
+
+  // ---------------------------------------------------------------- connectivity
+  // Per-run cell grids from the current extents; returns the Morton bits per axis needed.
+  int setupGrids() {
+    hipLaunchKernelGGL(k_reset_extent, dim3(nblocks((int64_t)maxRuns*7)), dim3(kBlock), 0, stream, dext, maxRuns);
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_extent<DD>, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, (int)nTotal, dext);
+    });
+    HIP_CHECK(hipGetLastError());
+    download(hext, dext, sizeof(unsigned long long)*maxRuns*7);
+    const int64_t maxAxis = D == 3 ? (1 << 15) : (D == 2 ? (1 << 20) : (1 << 26));
+    int64_t base = 0;
+    int bits = 0;
+    for (int r = 0; r < maxRuns; ++r) {
+      RunGrid& gr = hgrids[r];
+      for (int a = 0; a < 3; ++a) { gr.origin[a] = 0.0; gr.inv_cell[a] = 0.0; gr.nc[a] = 1; }
+      gr.cellBase = (int)base;
+      if (runs[r].state != RUN_ACTIVE) continue;
+      const double hmaxInv = ord_value(hext[r*7]);
+      if (!(hmaxInv > 0.0) || !std::isfinite(hmaxInv)) raise(SPX_ERR_NUMERIC, "non-positive or non-finite H on the device");
+      const double support = cfg.neighbor_extent/hmaxInv*(1.0 + 1.0e-9);
+      int64_t total = 1;
+      for (int a = 0; a < D; ++a) {
+        const double lo = ord_value(hext[r*7 + 1 + 2*a]), hi = ord_value(hext[r*7 + 2 + 2*a]);
+        if (!std::isfinite(lo) || !std::isfinite(hi)) raise(SPX_ERR_NUMERIC, "non-finite position on the device");
+        const int blo = geo.bc_lo[a], bhi = geo.bc_hi[a];
+        const double L = cfg.xmax[a] - cfg.xmin[a];
+        int64_t nc;
+        double cell, origin;
+        if (blo == BC_PERIODIC) {
+          nc = (int64_t)std::floor(L/support);
+          if (nc < 3) raise(SPX_ERR_UNSUPPORTED, "periodic box must span at least 3 support radii on every axis");
+          nc = std::min<int64_t>(nc, maxAxis);
+          cell = L/nc;
+          origin = cfg.xmin[a];
+        } else if (blo == BC_REFLECTING && bhi == BC_REFLECTING) {
+          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor(L/support), maxAxis));
+          cell = L/nc;
+          origin = cfg.xmin[a];
+        } else if (blo == BC_REFLECTING) {
+          const double top = std::max(hi, cfg.xmin[a]);
+          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor((top - cfg.xmin[a])/support) + 1, maxAxis));
+          cell = std::max(support, (top - cfg.xmin[a])/nc*(1.0 + 1e-12));
+          origin = cfg.xmin[a];
+        } else if (bhi == BC_REFLECTING) {
+          const double bot = std::min(lo, cfg.xmax[a]);
+          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor((cfg.xmax[a] - bot)/support) + 1, maxAxis));
+          cell = std::max(support, (cfg.xmax[a] - bot)/nc*(1.0 + 1e-12));
+          origin = cfg.xmax[a] - nc*cell;
+        } else {
+          const double span = std::max(0.0, hi - lo);
+          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor(span/support) + 1, maxAxis));
+          cell = std::max(support, span/nc*(1.0 + 1e-12));
+          origin = lo;
+        }
+        gr.nc[a] = (int)nc;
+        gr.inv_cell[a] = 1.0/cell;
+        gr.origin[a] = origin;
+        total *= nc;
+        int b = 0;
+        while (((int64_t)1 << b) < nc) ++b;
+        bits = std::max(bits, b);
+      }
+      base += total;
+      if (base > ((int64_t)1 << 28)) raise(SPX_ERR_CAPACITY, "cell grids too large (support radius tiny relative to the domain)");
+    }
+    if (base > cellsAlloc) {
+      HIP_CHECK(hipStreamSynchronize(stream));
+      const int64_t c = base + base/4 + 1024;
+      for (void* p : {(void*)cellStart, (void*)cellEnd}) {
+        if (!p) continue;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) allocs.erase(it);
+        HIP_CHECK(hipFree(p));
+      }
+      cellStart = dalloc<int>(c);
+      cellEnd = dalloc<int>(c);
+      cellsAlloc = c;
+    }
+    HIP_CHECK(hipMemcpyAsync(dgrids, hgrids, sizeof(RunGrid)*maxRuns, hipMemcpyHostToDevice, stream));
+    if (base > 0) {
+      HIP_CHECK(hipMemsetAsync(cellStart, 0, base*sizeof(int), stream));
+      HIP_CHECK(hipMemsetAsync(cellEnd, 0, base*sizeof(int), stream));
+    }
+    return bits;
+  }
+
+  // Sort by [rank | run | Morton] and gather the carried arrays; drops removed runs.
+  void sortAndGather(int bitsPerAxis) {
+    mortonBits = D*bitsPerAxis;
+    const int endBit = mortonBits + runBits + 2;
+    if (endBit > 64) raise(SPX_ERR_CAPACITY, "sort key exceeds 64 bits");
+    for (int r = 0; r < maxRuns; ++r) hstate[r] = runs[r].state;
+    HIP_CHECK(hipMemcpyAsync(dstate, hstate, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
+    const int nb = nblocks(nTotal);
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_keys<DD>, dim3(nb), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)dgrids, (const int*)dstate,
+                         (int)nTotal, wrapPending ? 1 : 0, 1, mortonBits, runBits, keys, vals);
+    });
+    HIP_CHECK(hipGetLastError());
+    wrapPending = false;
+    size_t tmp = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys, keysAlt, vals, valsAlt, (size_t)nTotal, 0, endBit, stream));
+    if (tmp > sortTmpBytes) {
+      HIP_CHECK(hipStreamSynchronize(stream));
+      auto it = std::find(allocs.begin(), allocs.end(), sortTmp);
+      if (it != allocs.end()) allocs.erase(it);
+      HIP_CHECK(hipFree(sortTmp));
+      sortTmp = dalloc<char>(tmp);
+      sortTmpBytes = tmp;
+    }
+    tmp = sortTmpBytes;
+    HIP_CHECK(rocprim::radix_sort_pairs(sortTmp, tmp, keys, keysAlt, vals, valsAlt, (size_t)nTotal, 0, endBit, stream));
+    auto src = carried(f);
+    auto dst = carried(alt);
+    GatherSet gs{};
+    gs.count = 0;
+    for (size_t k = 0; k < src.size(); ++k) { gs.src[gs.count] = *src[k]; gs.dst[gs.count] = *dst[k]; ++gs.count; }
+    for (int c = 0; c < D*D; ++c) { gs.src[gs.count] = f.DvDx + (size_t)c*ld; gs.dst[gs.count] = alt.DvDx + (size_t)c*ld; ++gs.count; }
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, stream, gs, valsAlt, (int)nTotal);
+    hipLaunchKernelGGL(k_gather_ids, dim3(nb), dim3(kBlock), 0, stream, f.oid, oidAlt, f.run, runAlt, valsAlt, (int)nTotal);
+    HIP_CHECK(hipGetLastError());
+    for (size_t k = 0; k < src.size(); ++k) std::swap(*src[k], *dst[k]);
+    std::swap(f.DvDx, alt.DvDx);
+    std::swap(f.oid, oidAlt);
+    std::swap(f.run, runAlt);
+    nActive = 0;
+    int64_t live = 0;
+    for (int r = 0; r < maxRuns; ++r) {
+      RunInfo& ri = runs[r];
+      if (ri.state == RUN_ACTIVE) nActive += ri.n;
+      if (ri.state == RUN_ACTIVE || ri.state == RUN_DONE) live += ri.n;
+      if (ri.state == RUN_REMOVED) ri = RunInfo();
+      ri.stepOut = ri.fullOut = false;    // the per-step outputs were not carried through the reorder
+    }
+    nTotal = live;
+    npairsCache = -1;
+  }
+
+  void buildConnectivity() {
+    if (nTotal == 0) raise(SPX_ERR_STATE, "no state uploaded");
+    const int bits = setupGrids();
+    sortAndGather(bits);
+    connValid = false;
+    if (nActive == 0) return;
+    const int nb = nblocks(nActive);
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_cell_bounds<DD>, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t*)keysAlt, (int)nActive,
+                         (const RunGrid*)dgrids, mortonBits, cellStart, cellEnd);
+    });
+    HIP_CHECK(hipGetLastError());
+    bool ok = false;
+    for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
+      HIP_CHECK(hipMemsetAsync(dOverflow, 0, sizeof(int), stream));
+      launchD([&](auto d) {
+        constexpr int DD = decltype(d)::value;
+        hipLaunchKernelGGL(k_neighbors<DD>, dim3(nb), dim3(kBlock), 0, stream, f, geo, ph, (const RunGrid*)dgrids,
+                           (int)nActive, (const int*)cellStart, (const int*)cellEnd, (const uint32_t*)nullptr, dOverflow);
+      });
+      HIP_CHECK(hipGetLastError());
+      download(hOverflow, dOverflow, sizeof(int));
+      if (*hOverflow == 0) { ok = true; break; }
+      const int need = *hOverflow;
+      if (need > 4096) raise(SPX_ERR_CAPACITY, "more than 4096 neighbours for one node");
+      ensureNbrCapacity(((need + need/4) + 7)/8*8);
+    }
+    if (!ok) raise(SPX_ERR_CAPACITY, "neighbour list capacity not reached");
+    for (int a = 0; a < 3; ++a) { hptrs[a] = f.xm[a]; hptrs[3 + a] = f.x[a]; }
+    HIP_CHECK(hipMemcpyAsync(xsMid, hptrs, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(xsState, hptrs + 3, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
+    connValid = true;
+  }
+
+  // Boundary enforcement without a neighbour rebuild (before a state download).
+  void applyPendingBoundaries() {
+    if (!wrapPending) return;
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_keys<DD>, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)nullptr,
+                         (const int*)nullptr, (int)nTotal, 1, 0, 0, 0, keys, vals);
+    });
+    HIP_CHECK(hipGetLastError());
+    wrapPending = false;
+  }
+
+  // ---------------------------------------------------------------- passes (over the active runs)
+  template <typename K64, typename K32, typename... Args>
+  void launchPrec(K64 k64, K32 k32, int nb, Args... args) {
+    if (mixed) hipLaunchKernelGGL(k32, dim3(nb), dim3(kBlock), 0, stream, args...);
+    else hipLaunchKernelGGL(k64, dim3(nb), dim3(kBlock), 0, stream, args...);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  void eos() {
+    if (nActive == 0) return;
+    hipLaunchKernelGGL(k_eos, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, ph, (int)nActive);
+    HIP_CHECK(hipGetLastError());
+  }
+  void omegaPass(bool mid, const int* runMask = nullptr) {
+    if (!cfg.gradh_correction || nActive == 0) return;
+    const int nb = nblocks(nActive);
+    double* const* xs = mid ? xsMid : xsState;
+    const double* hs = mid ? f.hinvm : f.hinv;
+    double* om = mid ? f.omegam : f.omega;
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      launchPrec(k_omega<DD, double>, k_omega<DD, float>, nb, f, geo, dtab, (int)nActive, xs, hs, om, runMask);
+    });
+  }
+  void densityDt(bool dens, bool dtv) {
+    if (nActive == 0) return;
+    const int nb = nblocks(nActive);
+    if (dtv) hipLaunchKernelGGL(k_reset_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      if (dens && dtv)
+        launchPrec(k_density_dt<DD, true, true, double>, k_density_dt<DD, true, true, float>, nb, f, geo, ph, dtab, (int)nActive, dsc);
+      else if (dens)
+        launchPrec(k_density_dt<DD, true, false, double>, k_density_dt<DD, true, false, float>, nb, f, geo, ph, dtab, (int)nActive, dsc);
+      else
+        launchPrec(k_density_dt<DD, false, true, double>, k_density_dt<DD, false, true, float>, nb, f, geo, ph, dtab, (int)nActive, dsc);
+    });
+  }
+  void sphPass(bool full) {
+    const int nb = nblocks(nActive);
+    hipLaunchKernelGGL(k_prho, dim3(nb), dim3(kBlock), 0, stream, f, (int)nActive);
+    HIP_CHECK(hipGetLastError());
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      if (full) launchPrec(k_sph<DD, true, double>, k_sph<DD, true, float>, nb, f, geo, ph, dtab, (int)nActive);
+      else launchPrec(k_sph<DD, false, double>, k_sph<DD, false, float>, nb, f, geo, ph, dtab, (int)nActive);
+    });
+  }
+  // TreeGravity::initialize (tree on the given positions) + evaluateDerivatives + dt vote
+  void gravityPass(bool mid) {
+    if (!cfg.gravity || nActive == 0) return;
+    double* xs[3] = {mid ? f.xm[0] : f.x[0], mid ? f.xm[1] : f.x[1], mid ? f.xm[2] : f.x[2]};
+    grav.build(xs, f.m, (int)nActive);
+    grav.evaluate(f.DvDtG, f.pot, (int)nActive, cfg.G, cfg.softening, cfg.opening, dsc, cfg.ftimestep);
+  }
+  void copyStateToMid() {
+    auto cp = [&](double* d, const double* s) {
+      HIP_CHECK(hipMemcpyAsync(d, s, nActive*sizeof(double), hipMemcpyDeviceToDevice, stream));
+    };
+    for (int a = 0; a < D; ++a) { cp(f.xm[a], f.x[a]); cp(f.vm[a], f.v[a]); }
+    cp(f.rhom, f.rho); cp(f.epsm, f.eps); cp(f.hinvm, f.hinv); cp(f.Pm, f.P); cp(f.csm, f.cs); cp(f.omegam, f.omega);
+  }
+
+  void mark(const char* name) {
+    if (!timing) return;
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(e, stream));
+    evs.push_back(e);
+    tnames.push_back(name);
+  }
+  void collectTimes() {
+    if (!timing || evs.empty()) return;
+    HIP_CHECK(hipEventSynchronize(evs.back()));
+    std::vector<std::string> names;
+    std::vector<double> ms;
+    for (size_t k = 1; k < evs.size(); ++k) {
+      float v = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&v, evs[k - 1], evs[k]));
+      auto it = std::find(names.begin(), names.end(), tnames[k]);
+      if (it == names.end()) { names.push_back(tnames[k]); ms.push_back(v); }
+      else ms[it - names.begin()] += v;
+    }
+    for (auto e : evs) (void)hipEventDestroy(e);
+    evs.clear();
+    tnames = names;
+    tms = ms;
+  }
+
+  // ---------------------------------------------------------------- lifecycle
+  // Start-up of runs admitted since the last step (the reinitializeProblem subset:
+  // EOS, grad-h Omega, TreeGravity dry run), on this step's connectivity.
+  void initNewRuns() {
+    bool any = false;
+    for (int r = 0; r < maxRuns; ++r) {
+      hinit[r] = runs[r].needInit && runs[r].state == RUN_ACTIVE ? 1 : 0;
+      any |= hinit[r] != 0;
+    }
+    if (!any) return;
+    HIP_CHECK(hipMemcpyAsync(dinit, hinit, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_eos_masked, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, ph, (const int*)dinit, (int)nActive);
+    HIP_CHECK(hipGetLastError());
+    omegaPass(false, dinit);
+    if (cfg.gravity) gravityPass(false);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (int r = 0; r < maxRuns; ++r) if (hinit[r]) runs[r].needInit = false;
+  }
+
+  // Explicit start-up of a single run (spx_run_initialize).
+  void initialize() {
+    if (nTotal == 0) raise(SPX_ERR_STATE, "no state uploaded");
+    // derivatives start at zero (SPHBase::initializeProblemStartup)
+    auto z = [&](double* p) { HIP_CHECK(hipMemsetAsync(p, 0, nTotal*sizeof(double), stream)); };
+    for (int a = 0; a < D; ++a) { z(f.DxDt[a]); z(f.DvDt[a]); z(f.DvDtG[a]); }
+    for (double* p : {f.DepsDt, f.DrhoDt, f.DHDt, f.maxQ}) z(p);
+    for (int c = 0; c < D*D; ++c) z(f.DvDx + (size_t)c*ld);
+    for (auto& r : runs) if (r.state == RUN_ACTIVE) r.needInit = true;
+    buildConnectivity();
+    initNewRuns();
+    initialized = true;
+  }
+
+  // One Integrator::step of every active run.
+  void step() {
+    if (!retire && !initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called before stepping");
+    if (timing) {
+      for (auto e : evs) (void)hipEventDestroy(e);
+      evs.clear();
+      tnames.clear();
+      mark("start");
+    }
+    // Integrator::step -> boundary enforcement, connectivity
+    buildConnectivity();
+    if (nActive == 0) return;
+    mark("connectivity");
+    initNewRuns();
+    const int nb = nblocks(nActive);
+    // preStepInitialize (RigorousSumDensity) fused with the dt vote; TreeGravity's
+    // vote is the previous evaluation's
+    densityDt(true, true);
+    hipLaunchKernelGGL(k_select_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns, ph, cfg.dt_min,
+                       cfg.dt_max, cfg.dt_growth, cfg.gravity);
+    HIP_CHECK(hipGetLastError());
+    mark("density_dt");
+    // half step with the previous derivatives
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_midstep<DD>, dim3(nb), dim3(kBlock), 0, stream, f, ph, (const Scalars*)dsc, (int)nActive);
+    });
+    HIP_CHECK(hipGetLastError());
+    mark("update");
+    omegaPass(true);
+    mark("omega");
+    // derivatives at the midpoint
+    if (cfg.gravity) { gravityPass(true); mark("gravity"); }
+    sphPass(false);
+    mark("sph");
+    // full step from the start-of-step state, compatible energy first
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      launchPrec(k_full_update<DD, false, double>, k_full_update<DD, false, float>, nb, f, geo, ph, dtab,
+                 (const Scalars*)dsc, 0.0, (int)nActive);
+    });
+    for (int a = 0; a < D; ++a) {
+      std::swap(f.v[a], f.v1[a]);
+      // the update wrote only the active runs: the finished runs behind them keep their velocity
+      if (nTotal > nActive)
+        HIP_CHECK(hipMemcpyAsync(f.v[a] + nActive, f.v1[a] + nActive, (nTotal - nActive)*sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream));
+    }
+    mark("energy_update");
+    omegaPass(false);
+    mark("omega");
+    wrapPending = true;
+    hipLaunchKernelGGL(k_advance_time, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
+    HIP_CHECK(hipGetLastError());
+    download(hsc, dsc, sizeof(Scalars)*maxRuns);
+    collectTimes();
+    for (int r = 0; r < maxRuns; ++r) {
+      if (runs[r].state != RUN_ACTIVE) continue;
+      runs[r].stepOut = true;
+      if (retire && reached_end(hsc[r].t, hsc[r].tEnd)) markDone(r);
+    }
+  }
+
+  int activeRuns() const {
+    int k = 0;
+    for (auto& r : runs) k += r.state == RUN_ACTIVE;
+    return k;
+  }
+
+  void setEndTime(int slot, double tEnd) {
+    checkSlot(slot);
+    hsc[slot].tEnd = tEnd;
+    upload(&dsc[slot].tEnd, &hsc[slot].tEnd, sizeof(double));
+  }
+
+  // ---------------------------------------------------------------- single-run package calls
+  void requireConn() const {
+    if (!connValid) raise(SPX_ERR_STATE, "no connectivity: call spx_run_update_connectivity or spx_run_initialize");
+  }
+  void evaluateDerivatives() {
+    requireConn();
+    copyStateToMid();
+    if (cfg.gravity) gravityPass(true);
+    sphPass(true);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (auto& r : runs) if (r.state == RUN_ACTIVE) r.stepOut = r.fullOut = true;
+  }
+  void sumDensity() {
+    requireConn();
+    densityDt(true, false);
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+  void hydroDtVote(int slot, double* hydro, double* gravity) {
+    requireConn();
+    densityDt(false, true);
+    hipLaunchKernelGGL(k_select_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns, ph, 0.0,
+                       1.7976931348623157e308, 1.7976931348623157e308, cfg.gravity);
+    HIP_CHECK(hipGetLastError());
+    download(hsc, dsc, sizeof(Scalars)*maxRuns);
+    *hydro = hsc[slot].hydroDt;
+    *gravity = cfg.gravity ? hsc[slot].gravDt : 0.0;
+  }
+  void compatibleEnergy(double multiplier) {
+    requireConn();
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      launchPrec(k_full_update<DD, true, double>, k_full_update<DD, true, float>, nblocks(nActive), f, geo, ph, dtab,
+                 (const Scalars*)dsc, multiplier, (int)nActive);
+    });
+    eos();
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  // Slots of one run in the current order and the run-local index of each.
+  void runSlots(int slot, std::vector<int64_t>& local, std::vector<int64_t>& slots) {
+    std::vector<int64_t> oid(nTotal);
+    std::vector<int32_t> rr(nTotal);
+    download(oid.data(), f.oid, nTotal*sizeof(int64_t));
+    download(rr.data(), f.run, nTotal*sizeof(int32_t));
+    local.clear();
+    slots.clear();
+    for (int64_t p = 0; p < nTotal; ++p)
+      if (rr[p] == slot) { slots.push_back(p); local.push_back(oid[p] & (((int64_t)1 << 40) - 1)); }
+  }
+
+  void neighbors(int slot, int64_t* offsets, int64_t* nbr, int32_t* img, int64_t capacity, int32_t* countsOnly) {
+    checkSlot(slot);
+    requireConn();
+    if (runs[slot].state != RUN_ACTIVE) raise(SPX_ERR_STATE, "the run holds no connectivity");
+    std::vector<int64_t> local, slots;
+    runSlots(slot, local, slots);
+    const int64_t n = runs[slot].n;
+    std::vector<int32_t> cnt(nActive);
+    std::vector<int64_t> oid(nActive);
+    download(cnt.data(), f.cnt, nActive*sizeof(int32_t));
+    download(oid.data(), f.oid, nActive*sizeof(int64_t));
+    if (countsOnly) {
+      for (size_t k = 0; k < slots.size(); ++k) countsOnly[local[k]] = cnt[slots[k]];
+      return;
+    }
+    std::vector<uint32_t> ell((size_t)f.cap*ld);
+    download(ell.data(), f.nbr, ell.size()*sizeof(uint32_t));
+    std::vector<int64_t> off(n + 1, 0), slotOf(n);
+    for (size_t k = 0; k < slots.size(); ++k) { off[local[k] + 1] = cnt[slots[k]]; slotOf[local[k]] = slots[k]; }
+    for (int64_t i = 0; i < n; ++i) off[i + 1] += off[i];
+    if (capacity < off[n]) raise(SPX_ERR_ARG, "neighbour capacity too small");
+    const int64_t mask = ((int64_t)1 << 40) - 1;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t p = slotOf[i];
+      std::vector<std::pair<int64_t, int32_t>> l;
+      for (int k = 0; k < cnt[p]; ++k) {
+        const uint32_t e = ell[(size_t)k*ld + p];
+        l.push_back({oid[e & kIdxMask] & mask, (int32_t)(e >> kImgShift)});
+      }
+      std::sort(l.begin(), l.end());
+      for (size_t k = 0; k < l.size(); ++k) { nbr[off[i] + k] = l[k].first; img[off[i] + k] = l[k].second; }
+      offsets[i] = off[i];
+    }
+    offsets[n] = off[n];
+  }
+
+  // Unique pairs of the active runs: internal pairs once, image pairs once per (node, image) entry.
+  int64_t numPairs() {
+    requireConn();
+    if (npairsCache >= 0) return npairsCache;
+    std::vector<int32_t> cnt(nActive);
+    download(cnt.data(), f.cnt, nActive*sizeof(int32_t));
+    std::vector<uint32_t> ell((size_t)f.cap*ld);
+    download(ell.data(), f.nbr, ell.size()*sizeof(uint32_t));
+    int64_t np = 0;
+    for (int64_t p = 0; p < nActive; ++p)
+      for (int k = 0; k < cnt[p]; ++k) {
+        const uint32_t e = ell[(size_t)k*ld + p];
+        if ((e >> kImgShift) != 0 || (int64_t)(e & kIdxMask) > p) ++np;
+      }
+    npairsCache = np;
+    return np;
+  }
+
+  // ---------------------------------------------------------------- field access
+  int64_t getField(int slot, const std::string& name, double* out, int64_t capacity) {
+    checkSlot(slot);
+    applyPendingBoundaries();
+    const RunInfo& ri = runs[slot];
+    auto needStep = [&] {
+      if (!ri.stepOut) raise(SPX_ERR_STATE, "'" + name + "' is not retained after the batch re-sorts; read it right after the run's step");
+    };
+    auto needFull = [&] {
+      if (!ri.fullOut) raise(SPX_ERR_STATE, "'" + name + "' is produced by spx_run_evaluate_derivatives only");
+    };
+    std::vector<int64_t> local, slots;
+    runSlots(slot, local, slots);
+    const int64_t n = ri.n;
+    std::vector<double> tmp(nTotal);
+    auto dl = [&](const double* src) { download(tmp.data(), src, nTotal*sizeof(double)); };
+    auto scalar = [&](const double* src) -> int64_t {
+      if (capacity < n) raise(SPX_ERR_ARG, "output capacity too small");
+      dl(src);
+      for (size_t k = 0; k < slots.size(); ++k) out[local[k]] = tmp[slots[k]];
+      return n;
+    };
+    auto vec = [&](double* const* src) -> int64_t {
+      if (capacity < n*D) raise(SPX_ERR_ARG, "output capacity too small");
+      for (int a = 0; a < D; ++a) {
+        dl(src[a]);
+        for (size_t k = 0; k < slots.size(); ++k) out[local[k]*D + a] = tmp[slots[k]];
+      }
+      return n*D;
+    };
+    auto ten = [&](const double* src) -> int64_t {
+      if (capacity < n*D*D) raise(SPX_ERR_ARG, "output capacity too small");
+      for (int c = 0; c < D*D; ++c) {
+        dl(src + (size_t)c*ld);
+        for (size_t k = 0; k < slots.size(); ++k) out[local[k]*D*D + c] = tmp[slots[k]];
+      }
+      return n*D*D;
+    };
+    auto iso = [&](const double* src) -> int64_t {
+      if (capacity < n*D*D) raise(SPX_ERR_ARG, "output capacity too small");
+      dl(src);
+      for (size_t k = 0; k < slots.size(); ++k)
+        for (int a = 0; a < D; ++a)
+          for (int b = 0; b < D; ++b) out[local[k]*D*D + a*D + b] = a == b ? tmp[slots[k]] : 0.0;
+      return n*D*D;
+    };
+    if (name == "position") return vec(f.x);
+    if (name == "velocity") return vec(f.v);
+    if (name == "mass") return scalar(f.m);
+    if (name == "mass density") return scalar(f.rho);
+    if (name == "specific thermal energy") return scalar(f.eps);
+    if (name == "H") return iso(f.hinv);
+    if (name == "pressure") return scalar(f.P);
+    if (name == "sound speed") return scalar(f.cs);
+    if (name == "grad h corrections") return scalar(f.omega);
+    if (name == "delta velocity hydro") return vec(f.DvDt);
+    if (name == "delta velocity") return vec(f.DvDtG);
+    if (name == "delta position") return vec(f.DxDt);
+    if (name == "delta specific thermal energy") return scalar(f.DepsDt);
+    if (name == "delta mass density") return scalar(f.DrhoDt);
+    if (name == "velocity gradient") return ten(f.DvDx);
+    if (name == "delta H") return iso(f.DHDt);
+    if (name == "max viscous pressure") return scalar(f.maxQ);
+    if (name == "gravitational potential") return scalar(f.pot);
+    if (name == "new H") { needStep(); return iso(f.Hideal); }
+    if (name == "mass zeroth moment") { needStep(); return scalar(f.m0); }
+    if (name == "effective viscous pressure") { needFull(); return scalar(f.effQ); }
+    if (name == "mass first moment") { needFull(); return vec(f.m1); }
+    if (name == "M SPH correction") { needFull(); return ten(f.Mcorr); }
+    if (name == "mass density gradient") { needFull(); return vec(f.gradRho); }
+    if (name == "normalization") { needFull(); return scalar(f.norm); }
+    if (name == "new mass density") { needFull(); return scalar(f.rhoSum); }
+    raise(SPX_ERR_ARG, "unknown field '" + name + "'");
+  }
+
+  // Totals of one run: mass, momentum (3), kinetic, thermal, gravitational potential energy.
+  void conservation(int slot, double* out) {
+    checkSlot(slot);
+    const int64_t n = runs[slot].n;
+    std::vector<double> m(n), e(n), v(n*D), pot(n);
+    getField(slot, "mass", m.data(), n);
+    getField(slot, "specific thermal energy", e.data(), n);
+    getField(slot, "velocity", v.data(), n*D);
+    getField(slot, "gravitational potential", pot.data(), n);
+    double M = 0, P[3] = {0, 0, 0}, KE = 0, TE = 0, PE = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      M += m[i];
+      double v2 = 0.0;
+      for (int a = 0; a < D; ++a) { P[a] += m[i]*v[i*D + a]; v2 += v[i*D + a]*v[i*D + a]; }
+      KE += 0.5*m[i]*v2;
+      TE += m[i]*e[i];
+      PE += 0.5*m[i]*pot[i];
+    }
+    out[0] = M; out[1] = P[0]; out[2] = P[1]; out[3] = P[2]; out[4] = KE; out[5] = TE; out[6] = cfg.gravity ? PE : 0.0;
+  }
+};
+
+}  // namespace spx

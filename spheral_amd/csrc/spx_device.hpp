@@ -19,10 +19,14 @@ constexpr int kBlock = 256;
 
 enum Bc { BC_NONE = 0, BC_PERIODIC = 1, BC_REFLECTING = 2 };
 
+// Life cycle of a run slot in a batch.
+enum RunState { RUN_FREE = 0, RUN_ACTIVE = 1, RUN_DONE = 2, RUN_REMOVED = 3 };
+
 // Tabulated kernel: per interval 6 coefficients (W: c0 c1 c2, dW: c0 c1 c2),
 // plus the Wsum -> nPerh cubic Hermite table (N values then N knot gradients).
 struct DevTable {
   const double* coef;
+  const float* coef32;     // the same coefficients rounded to float (mixed-precision pair math)
   const double* herm;
   int n1;
   int hn;
@@ -30,15 +34,21 @@ struct DevTable {
   double hxmin, hxmax, hxstep;
 };
 
-// Boundary planes and the cell grid of this step.
+// Boundary planes, shared by every run of a batch.
 struct Geo {
   int bc_lo[3], bc_hi[3];
   double xmin[3], xmax[3];
-  double origin[3];
-  double cell[3], inv_cell[3];
-  int nc[3];
   // bit a: axis a periodic; bit 2a + side - 1 (side 1 low, 2 high): that side reflecting
   int permask, reflmask;
+};
+
+// Cell grid of one run for the current step.  Cells of run r occupy the global
+// cell-index range [cellBase, cellBase + nc0*nc1*nc2).
+struct RunGrid {
+  double origin[3];
+  double inv_cell[3];
+  int nc[3];
+  int cellBase;
 };
 
 // Physics constants of the run.
@@ -65,17 +75,32 @@ struct Fields {
   double *pot;       // gravitational potential
   uint32_t* nbr;
   int32_t* cnt;
-  int64_t* oid;
+  int64_t* oid;      // (run slot << 40) | node index within the run
+  int32_t* run;      // run slot of each particle
   int ld, cap;
 };
 
-// Device scalars of a run (one struct in device memory).
+// Device scalars of one run.
 struct Scalars {
   double t, lastDt, dt, hdt;
-  double hydroDt, gravDt, dtMinAcc, extraEnergy;
+  double hydroDt, gravDt, dtMinAcc, tEnd;
   long long cycle;
-  int overflow, pad;
+  unsigned long long dtBits;   // running min of the hydro dt vote (order-preserving bits)
+  int active, overflow;
+  int nonfinite, pad;
 };
+
+// Order-preserving map between doubles and unsigned 64-bit integers (for atomic min/max).
+__device__ __forceinline__ unsigned long long ord_bits(double d) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double ord_value(unsigned long long u) {
+  const unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+  double d;
+  __builtin_memcpy(&d, &b, sizeof(d));
+  return d;
+}
 
 __device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a, b); }
 __device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
@@ -103,12 +128,12 @@ __device__ __forceinline__ double image_coord(int bc, int code, double x, double
   return dsub(hi, dsub(x, hi));
 }
 
-template <int D>
-__device__ __forceinline__ double hdet_iso(double h) {
+template <int D, typename R = double>
+__device__ __forceinline__ R hdet_iso(R h) {
   // GeomSymmetricTensor Determinant with zero off-diagonals
   if (D == 1) return h;
-  if (D == 2) return dmul(h, h);
-  return dmul(dmul(h, h), h);
+  if (D == 2) return h*h;
+  return (h*h)*h;
 }
 
 // Largest eigenvalue of hinv * I by the reference's eigenValues formulas
@@ -141,28 +166,35 @@ __device__ __forceinline__ double max_eigen_iso(double h) {
 template <int D>
 __device__ __forceinline__ double rdiv(double x, double s) { return D == 1 ? x/s : x*(1.0/s); }
 
-__device__ __forceinline__ void kernel_wgw(const DevTable& t, double eta, double hdet, double& w, double& gw) {
-  if (eta < t.extent) {
-    int b = (int)(eta*t.inv_xstep);
+template <typename R> __device__ __forceinline__ const R* table_coef(const DevTable& t);
+template <> __device__ __forceinline__ const double* table_coef<double>(const DevTable& t) { return t.coef; }
+template <> __device__ __forceinline__ const float* table_coef<float>(const DevTable& t) { return t.coef32; }
+
+// TableKernelView::kernelAndGradValue (Kernel/TableKernelViewInline.hh:84-99) in arithmetic R.
+template <typename R = double>
+__device__ __forceinline__ void kernel_wgw(const DevTable& t, R eta, R hdet, R& w, R& gw) {
+  if (eta < (R)t.extent) {
+    int b = (int)(eta*(R)t.inv_xstep);
     b = b < t.n1 ? b : t.n1;
-    const double* c = t.coef + 6*b;
+    const R* c = table_coef<R>(t) + 6*b;
     w = hdet*(c[0] + (c[1] + c[2]*eta)*eta);
     gw = hdet*(c[3] + (c[4] + c[5]*eta)*eta);
   } else {
-    w = 0.0;
-    gw = 0.0;
+    w = (R)0;
+    gw = (R)0;
   }
 }
 
 // kernelValueSPH: |dW/deta| at Hdet = 1 (Kernel/TableKernelViewInline.hh:112-120)
-__device__ __forceinline__ double kernel_wsph(const DevTable& t, double eta) {
-  if (eta < t.extent) {
-    int b = (int)(eta*t.inv_xstep);
+template <typename R = double>
+__device__ __forceinline__ R kernel_wsph(const DevTable& t, R eta) {
+  if (eta < (R)t.extent) {
+    int b = (int)(eta*(R)t.inv_xstep);
     b = b < t.n1 ? b : t.n1;
-    const double* c = t.coef + 6*b;
+    const R* c = table_coef<R>(t) + 6*b;
     return fabs(c[3] + (c[4] + c[5]*eta)*eta);
   }
-  return 0.0;
+  return (R)0;
 }
 
 // equivalentNodesPerSmoothingScale: cubic Hermite lookup (CubicHermiteInterpolatorViewInline.hh:9-33,120-125)

@@ -1,4 +1,4 @@
-// spx_kernels.hpp -- HIP kernels of the SPH hot path (FP64 path).
+// spx_kernels.hpp -- neighbour build and the light pair passes of the SPH hot path.
 //
 // Every pair pass is a gather: one lane per target particle walks its ELLPACK
 // neighbour list and accumulates only its own sums, so no atomics touch the
@@ -6,6 +6,11 @@
 // pairwise acceleration and the compatible-energy partition) both lanes evaluate
 // the pair in one canonical (a, b) order: an image partner makes the gathering
 // node "a" (the reference's internal-ghost pair), otherwise the lower slot is "a".
+//
+// A batch holds independent runs of one problem type; particle i belongs to run
+// f.run[i] with its own cell grid (RunGrid) and time step (Scalars).  Pairs never
+// cross runs.  R is the arithmetic type of the pair math (double, or float in the
+// mixed-precision mode where positions are differenced in double first).
 #pragma once
 
 #include "spx_device.hpp"
@@ -13,9 +18,9 @@
 namespace spx {
 
 // ---------------------------------------------------------------------------
-// Neighbour image of slot j seen from slot i: position and velocity transforms
-// (PeriodicBoundary copies, ReflectingBoundary::reflectOperator flips the normal
-// component: Boundary/ReflectingBoundary.cc:183-197).
+// Neighbour image of a slot: position and velocity transforms (PeriodicBoundary
+// copies, ReflectingBoundary::reflectOperator flips the normal component:
+// Boundary/ReflectingBoundary.cc:183-197).
 // ---------------------------------------------------------------------------
 template <int D>
 __device__ __forceinline__ void image_pos(const Geo& g, int code, double* xj) {
@@ -32,22 +37,34 @@ __device__ __forceinline__ double refl_sign(const Geo& g, int code, int a) {
   return ((g.reflmask >> (2*a + c - 1)) & 1) ? -1.0 : 1.0;
 }
 
+template <int D>
+__device__ __forceinline__ int cell_coord(const RunGrid& gr, int a, double x) {
+  int c = (int)floor((x - gr.origin[a])*gr.inv_cell[a]);
+  return c < 0 ? 0 : (c >= gr.nc[a] ? gr.nc[a] - 1 : c);
+}
+
 // ---------------------------------------------------------------------------
-// Boundary enforcement + cell keys.  Periodic violators are mapped through the
+// Boundary enforcement + sort keys.  Periodic violators are mapped through the
 // opposite plane, reflecting violators mirrored with their velocity
-// (Boundary/PlanarBoundary.cc:153-200, ReflectingBoundary.cc:465-473).
+// (Boundary/PlanarBoundary.cc:153-200, ReflectingBoundary.cc:465-473).  Keys order
+// the particles of active runs by (run, cell Morton code), then the finished runs
+// (kept whole, behind the active ones), then removed runs (largest key).
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, int n, int enforce, uint64_t* keys, uint32_t* vals) {
+__global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, const RunGrid* grids, const int* rstate, int n,
+                                                 int enforce, int withKeys, int mortonBits, int runBits,
+                                                 uint64_t* keys, uint32_t* vals) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int r = f.run[i];
   uint64_t key = 0;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     double xa = f.x[a][i];
     if (enforce) {
       for (int side = 1; side <= 2; ++side) {
-        const int bc = side == 1 ? g.bc_lo[a] : g.bc_hi[a];
+        const int bc = side == 1 ? (((g.permask >> a) & 1) ? BC_PERIODIC : (((g.reflmask >> (2*a)) & 1) ? BC_REFLECTING : BC_NONE))
+                                 : (((g.permask >> a) & 1) ? BC_PERIODIC : (((g.reflmask >> (2*a + 1)) & 1) ? BC_REFLECTING : BC_NONE));
         if (bc == BC_NONE) continue;
         const double thpt = side == 1 ? (xa - g.xmin[a]) : -(xa - g.xmax[a]);
         const bool fuzzy0 = fabs(thpt) <= 1.0e-15*fmax(1.0, fabs(thpt));
@@ -62,15 +79,19 @@ __global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, int n, int enf
       }
       f.x[a][i] = xa;
     }
-    int c = (int)floor((xa - g.origin[a])*g.inv_cell[a]);
-    c = c < 0 ? 0 : (c >= g.nc[a] ? g.nc[a] - 1 : c);
-    key |= spread_bits((uint64_t)c, D) << a;
+    if (withKeys) key |= spread_bits((uint64_t)cell_coord<D>(grids[r], a, xa), D) << a;
   }
-  keys[i] = key;
-  vals[i] = (uint32_t)i;
+  if (withKeys) {
+    // layout: [rank | run | Morton], rank 0 active, 1 done; removed runs take the maximum key
+    const int st = rstate[r];
+    const int shift = mortonBits + runBits;
+    keys[i] = st == RUN_ACTIVE ? (((uint64_t)r << mortonBits) | key)
+            : (st == RUN_DONE ? ((1ull << shift) | ((uint64_t)r << mortonBits)) : ~0ull);
+    vals[i] = (uint32_t)i;
+  }
 }
 
-// Gather a set of double arrays through a permutation (dst[p] = src[perm[p]]).
+// Gather a set of arrays through a permutation (dst[p] = src[perm[p]]).
 struct GatherSet {
   const double* src[48];
   double* dst[48];
@@ -82,35 +103,38 @@ __global__ void __launch_bounds__(kBlock) k_gather(GatherSet s, const uint32_t* 
   const uint32_t q = perm[p];
   for (int k = 0; k < s.count; ++k) s.dst[k][p] = s.src[k][q];
 }
-__global__ void __launch_bounds__(kBlock) k_gather_i64(const int64_t* src, int64_t* dst, const uint32_t* perm, int n) {
+__global__ void __launch_bounds__(kBlock) k_gather_ids(const int64_t* src, int64_t* dst, const int32_t* rsrc, int32_t* rdst,
+                                                       const uint32_t* perm, int n) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
-  if (p < n) dst[p] = src[perm[p]];
+  if (p < n) { dst[p] = src[perm[p]]; rdst[p] = rsrc[perm[p]]; }
 }
 
 template <int D>
-__device__ __forceinline__ int lin_cell(const Geo& g, const int* c) {
+__device__ __forceinline__ int lin_cell(const RunGrid& g, const int* c) {
   int l = c[D - 1];
   for (int a = D - 2; a >= 0; --a) l = l*g.nc[a] + c[a];
-  return l;
+  return g.cellBase + l;
 }
 template <int D>
-__device__ __forceinline__ void unspread(uint64_t key, int* c) {
+__device__ __forceinline__ void unspread(uint64_t key, int mortonBits, int* c) {
   for (int a = 0; a < D; ++a) {
     uint64_t v = 0;
-    for (int b = 0; b < 21; ++b) v |= ((key >> (D*b + a)) & 1ull) << b;
+    for (int b = 0; b < 21 && D*b + a < mortonBits; ++b) v |= ((key >> (D*b + a)) & 1ull) << b;
     c[a] = (int)v;
   }
 }
 
-// Cell ranges of the sorted keys: start[cell], end[cell] (both 0 when empty).
+// Cell ranges of the sorted keys of the active runs: start[cell], end[cell] (both 0 when empty).
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, int n, Geo g, int* start, int* end) {
+__global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, int n, const RunGrid* grids, int mortonBits,
+                                                        int* start, int* end) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint64_t k = keys[p];
+  const int r = (int)(k >> mortonBits);          // active keys carry rank 0 above the run field
   int c[3];
-  if (p == 0 || keys[p - 1] != k) { unspread<D>(k, c); start[lin_cell<D>(g, c)] = p; }
-  if (p == n - 1 || keys[p + 1] != k) { unspread<D>(k, c); end[lin_cell<D>(g, c)] = p + 1; }
+  if (p == 0 || keys[p - 1] != k) { unspread<D>(k, mortonBits, c); start[lin_cell<D>(grids[r], c)] = p; }
+  if (p == n - 1 || keys[p + 1] != k) { unspread<D>(k, mortonBits, c); end[lin_cell<D>(grids[r], c)] = p + 1; }
 }
 
 // ---------------------------------------------------------------------------
@@ -122,17 +146,19 @@ __global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, in
 // sorted: slot of the p-th particle in cell order, or nullptr when slots are sorted.
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, int n, const int* cstart,
-                                                      const int* cend, const uint32_t* sorted, Scalars* sc) {
+__global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, const RunGrid* grids, int n,
+                                                      const int* cstart, const int* cend, const uint32_t* sorted,
+                                                      int* overflow) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int r = f.run[i];
+  const RunGrid gr = grids[r];
   double xi[D];
   int ci[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     xi[a] = f.x[a][i];
-    int c = (int)floor((xi[a] - g.origin[a])*g.inv_cell[a]);
-    ci[a] = c < 0 ? 0 : (c >= g.nc[a] ? g.nc[a] - 1 : c);
+    ci[a] = cell_coord<D>(gr, a, xi[a]);
   }
   const double hi = f.hinv[i];
   f.ns[i] = 1.0/max_eigen_iso<D>(hi)/ph.nperh;
@@ -149,19 +175,20 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
 #pragma unroll
         for (int a = 0; a < D; ++a) {
           int c = ci[a] + off[a];
+          const bool per = (g.permask >> a) & 1;
           if (c < 0) {
-            if (g.bc_lo[a] == BC_PERIODIC) { c += g.nc[a]; code |= 1 << (2*a); }
-            else if (g.bc_lo[a] == BC_REFLECTING) { c = -c - 1; code |= 1 << (2*a); }
+            if (per) { c += gr.nc[a]; code |= 1 << (2*a); }
+            else if ((g.reflmask >> (2*a)) & 1) { c = -c - 1; code |= 1 << (2*a); }
             else skip = true;
-          } else if (c >= g.nc[a]) {
-            if (g.bc_hi[a] == BC_PERIODIC) { c -= g.nc[a]; code |= 2 << (2*a); }
-            else if (g.bc_hi[a] == BC_REFLECTING) { c = 2*g.nc[a] - 1 - c; code |= 2 << (2*a); }
+          } else if (c >= gr.nc[a]) {
+            if (per) { c -= gr.nc[a]; code |= 2 << (2*a); }
+            else if ((g.reflmask >> (2*a + 1)) & 1) { c = 2*gr.nc[a] - 1 - c; code |= 2 << (2*a); }
             else skip = true;
           }
           cc[a] = c;
         }
         if (skip) continue;
-        const int cell = lin_cell<D>(g, cc);
+        const int cell = lin_cell<D>(gr, cc);
         const int p0 = cstart[cell], p1 = cend[cell];
         for (int p = p0; p < p1; ++p) {
           const int j = sorted ? (int)sorted[p] : p;
@@ -169,13 +196,13 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
           double xj[D];
 #pragma unroll
           for (int a = 0; a < D; ++a) xj[a] = f.x[a][j];
-          image_pos<D>(g, code, xj);
+          if (code) image_pos<D>(g, code, xj);
           const double hj = f.hinv[j];
           double e2i = 0.0, e2j = 0.0;
 #pragma unroll
           for (int a = 0; a < D; ++a) {
-            const double r = dsub(xi[a], xj[a]);
-            const double ei = dmul(hi, r), ej = dmul(hj, r);
+            const double rr = dsub(xi[a], xj[a]);
+            const double ei = dmul(hi, rr), ej = dmul(hj, rr);
             e2i = a == 0 ? dmul(ei, ei) : dadd(e2i, dmul(ei, ei));
             e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
           }
@@ -186,32 +213,48 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
         }
       }
   f.cnt[i] = count;
-  if (count > cap) atomicMax(&sc->overflow, count);
+  if (count > cap) atomicMax(overflow, count);
+}
+
+// Running per-run minimum of a positive double: one atomic per wavefront when its lanes share a run.
+__device__ __forceinline__ void run_min(Scalars* sc, int r, double v, bool valid) {
+  const int r0 = __shfl(r, 0);
+  const bool uniform = __all(!valid || r == r0);
+  if (uniform) {
+    double m = valid ? v : 1.7976931348623157e308;
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && __any(valid)) atomicMin(&sc[r0].dtBits, ord_bits(m));
+  } else if (valid) {
+    atomicMin(&sc[r].dtBits, ord_bits(v));
+  }
 }
 
 // ---------------------------------------------------------------------------
 // computeSPHSumMassDensity (SPH/computeSPHSumMassDensity.cc:35-85) in gather form
 // -- node i receives m_j W(|Hj ri - Hj rj|, det Hj) -- fused with the GenericHydro::dt
 // vote (Physics/GenericHydro.cc:112-381): per-node limits on the new density and the
-// pairwise velocity-difference limit, min-reduced per block.
+// pairwise velocity-difference limit, min-reduced per run.
 // ---------------------------------------------------------------------------
-template <int D, bool DENS, bool DT>
-__global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, double* blockMin) {
+template <int D, bool DENS, bool DT, typename R>
+__global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   double dtmin = 1.7976931348623157e308;
-  if (i < n) {
+  const bool valid = i < n;
+  int r = 0;
+  if (valid) {
+    r = f.run[i];
     double xi[D], vi[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) { xi[a] = f.x[a][i]; vi[a] = f.v[a][i]; }
     const double nsi = f.ns[i];
     double rho = 0.0;
     if (DENS) {
-      const double hi = f.hinv[i];
-      // self term: W(|Hi ri - Hi ri|) = W(0)
-      double w, gw;
-      kernel_wgw(tab, 0.0, hdet_iso<D>(hi), w, gw);
-      rho = f.m[i]*w;
+      const R hi = (R)f.hinv[i];
+      R w, gw;
+      kernel_wgw<R>(tab, (R)0, hdet_iso<D, R>(hi), w, gw);      // self term W(|Hi ri - Hi ri|) = W(0)
+      rho = f.m[i]*(double)w;
     }
+    double vmax2 = 0.0;       // pairwise limit, tracked as max of |v_ij|^2 / min(ns)^2
     const int nn = f.cnt[i];
     for (int k = 0; k < nn; ++k) {
       const uint32_t e = f.nbr[(size_t)k*f.ld + i];
@@ -221,27 +264,33 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
         double xj[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) xj[a] = f.x[a][j];
-        image_pos<D>(g, code, xj);
+        if (code) image_pos<D>(g, code, xj);
         const double hj = f.hinv[j];
-        double s = 0.0;
+        R s = (R)0;
+        if (sizeof(R) == sizeof(double)) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) { const double d = hj*xi[a] - hj*xj[a]; s += d*d; }
-        double w, gw;
-        kernel_wgw(tab, sqrt(s), hdet_iso<D>(hj), w, gw);
-        rho += f.m[j]*w;
+          for (int a = 0; a < D; ++a) { const double d = hj*xi[a] - hj*xj[a]; s += (R)(d*d); }
+        } else {
+#pragma unroll
+          for (int a = 0; a < D; ++a) { const R d = (R)hj*(R)(xi[a] - xj[a]); s += d*d; }
+        }
+        R w, gw;
+        kernel_wgw<R>(tab, sqrt(s), hdet_iso<D, R>((R)hj), w, gw);
+        rho += f.m[j]*(double)w;
       }
       if (DT) {
         double s = 0.0;
 #pragma unroll
         for (int a = 0; a < D; ++a) { const double d = vi[a] - refl_sign<D>(g, code, a)*f.v[a][j]; s += d*d; }
-        const double vmag = sqrt(s);
-        const double d = fmin(nsi, f.ns[j])*(1.0/fmax(2.220446049250313e-16, vmag));
-        dtmin = fmin(dtmin, d);
+        const double nsm = fmin(nsi, f.ns[j]);
+        vmax2 = fmax(vmax2, s/(nsm*nsm));
       }
     }
     if (DENS) f.rho[i] = rho; else rho = f.rho[i];
     if (DT) {
       const double tiny = 2.220446049250313e-16;
+      // min(ns_i, ns_j)/max(tiny, |v_ij|) minimised over the pairs
+      if (vmax2 > 0.0) dtmin = fmin(dtmin, 1.0/sqrt(vmax2));
       dtmin = fmin(dtmin, nsi/(f.cs[i] + tiny));
       const double csq = sqrt(f.maxQ[i]/rho);
       dtmin = fmin(dtmin, nsi/(csq + tiny));
@@ -257,45 +306,45 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
       if (ph.useVmagDt) dtmin = fmin(dtmin, nsi/(vmag + 1.0e-10));
     }
   }
-  if (DT) {
-    __shared__ double red[kBlock];
-    red[threadIdx.x] = dtmin;
-    __syncthreads();
-    for (int s = kBlock/2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) blockMin[blockIdx.x] = red[0];
-  }
+  if (DT) run_min(sc, r, dtmin, valid);
 }
 
-// Integrator::selectDt (Integrator/Integrator.cc:116-169) on the device.
-__global__ void k_select_dt(const double* blockMin, int nblocks, Scalars* sc, Phys ph, double dtMin, double dtMax,
-                            double maxTime, double dtGrowth, int useGravity) {
-  __shared__ double red[kBlock];
-  double m = 1.7976931348623157e308;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmin(m, blockMin[b]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = kBlock/2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+// Integrator::selectDt (Integrator/Integrator.cc:116-169), one thread per run.
+__global__ void k_select_dt(Scalars* sc, int nruns, Phys ph, double dtMin, double dtMax, double dtGrowth, int useGravity) {
+  const int r = blockIdx.x*blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  Scalars& s = sc[r];
+  if (!s.active) { s.dt = 0.0; s.hdt = 0.0; return; }
+  const double vote = ord_value(s.dtBits);
+  const double hydro = vote*ph.cfl;
+  s.hydroDt = hydro;
+  const double lo = fmin(dtMin, s.tEnd - s.t), hi = fmin(dtMax, s.tEnd - s.t);
+  double dt = hi;
+  if (hydro > 0.0 && hydro < dt) dt = hydro;
+  if (useGravity) {
+    const double gd = s.gravDt;
+    if (gd > 0.0 && gd < dt) dt = gd;
   }
-  if (threadIdx.x == 0) {
-    const double hydro = red[0]*ph.cfl;
-    sc->hydroDt = hydro;
-    const double lo = fmin(dtMin, maxTime - sc->t), hi = fmin(dtMax, maxTime - sc->t);
-    double dt = hi;
-    if (hydro > 0.0 && hydro < dt) dt = hydro;
-    if (useGravity) {
-      const double gd = sc->gravDt;
-      if (gd > 0.0 && gd < dt) dt = gd;
-    }
-    dt = fmin(dt, dtGrowth*sc->lastDt);
-    dt = fmin(hi, fmax(lo, dt));
-    sc->dt = dt;
-    sc->hdt = 0.5*dt;
-  }
+  dt = fmin(dt, dtGrowth*s.lastDt);
+  dt = fmin(hi, fmax(lo, dt));
+  s.dt = dt;
+  s.hdt = 0.5*dt;
+}
+
+__global__ void k_reset_dt(Scalars* sc, int nruns) {
+  const int r = blockIdx.x*blockDim.x + threadIdx.x;
+  if (r < nruns) sc[r].dtBits = 0xfff0000000000000ull;   // ord_bits(+inf)
+}
+
+// Time bookkeeping after a step; a run is done when it reaches its end time.
+__global__ void k_advance_time(Scalars* sc, int nruns) {
+  const int r = blockIdx.x*blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  Scalars& s = sc[r];
+  if (!s.active) return;
+  s.t = s.t + s.dt;
+  s.lastDt = s.dt;
+  s.cycle = s.cycle + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -308,7 +357,7 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_midstep(Fields f, Phys ph, const Scalars* sc, int n) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double hdt = sc->hdt;
+  const double hdt = sc[f.run[i]].hdt;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     f.xm[a][i] = f.x[a][i] + hdt*f.DxDt[a][i];
@@ -336,17 +385,19 @@ __global__ void __launch_bounds__(kBlock) k_eos(Fields f, Phys ph, int n) {
 
 // ---------------------------------------------------------------------------
 // computeSPHOmegaGradhCorrection (SPH/computeSPHOmegaGradhCorrection.cc:136-204), gather form.
+// runMask (optional): only nodes of runs with a non-zero entry are updated.
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, typename R>
 __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, double* const* xs,
-                                                  const double* hs, double* omega) {
+                                                  const double* hs, double* omega, const int* runMask) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (runMask && !runMask[f.run[i]]) return;
   double xi[D];
 #pragma unroll
   for (int a = 0; a < D; ++a) xi[a] = xs[a][i];
-  const double hi = hs[i];
-  const double hdeti = hdet_iso<D>(hi);
+  const R hi = (R)hs[i];
+  const R hdeti = hdet_iso<D, R>(hi);
   double wsum = 0.0, gsum = 0.0;
   const int nn = f.cnt[i];
   for (int k = 0; k < nn; ++k) {
@@ -356,24 +407,81 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
     double xj[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) xj[a] = xs[a][j];
-    image_pos<D>(g, code, xj);
-    double s = 0.0;
+    if (code) image_pos<D>(g, code, xj);
+    R s = (R)0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) { const double r = hi*(xi[a] - xj[a]); s += r*r; }
-    const double eta = sqrt(s);
-    double w, gw;
-    kernel_wgw(tab, eta, hdeti, w, gw);
-    wsum += w;
-    gsum += eta*gw;
+    for (int a = 0; a < D; ++a) { const R rr = hi*(R)(xi[a] - xj[a]); s += rr*rr; }
+    const R eta = sqrt(s);
+    R w, gw;
+    kernel_wgw<R>(tab, eta, hdeti, w, gw);
+    wsum += (double)w;
+    gsum += (double)(eta*gw);
   }
   double om;
   if (nn == 0) {
     om = 1.0;
   } else {
-    wsum += hdeti*tab.W0;
+    wsum += (double)hdeti*tab.W0;
     om = fmax(1.0e-30, -gsum/(double(D)*wsum));
   }
   omega[i] = om;
+}
+
+// Per-run extents for the next grid: min hinv and the bounding box (atomics on ordered bits).
+// ext layout per run: [hinv_min, xmin0, xmax0, xmin1, xmax1, xmin2, xmax2] as ordered bits.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_extent(Fields f, int n, unsigned long long* ext) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  const int r = valid ? f.run[i] : 0;
+  unsigned long long v[7];
+  v[0] = valid ? ord_bits(f.hinv[i]) : ~0ull;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const unsigned long long b = (valid && a < D) ? ord_bits(f.x[a][i]) : 0ull;
+    v[1 + 2*a] = (valid && a < D) ? b : ~0ull;
+    v[2 + 2*a] = b;
+  }
+  const int r0 = __shfl(r, 0);
+  const bool uniform = __all(!valid || r == r0);
+  if (uniform) {
+    // wavefront reduction first, one set of atomics per wavefront
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      unsigned long long x = v[q];
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long y = __shfl_xor(x, off);
+        x = (q == 0 || (q & 1)) ? (y < x ? y : x) : (y > x ? y : x);
+      }
+      v[q] = x;
+    }
+    if ((threadIdx.x & 63) == 0 && __any(valid)) {
+      unsigned long long* e = ext + (size_t)r0*7;
+      atomicMin(&e[0], v[0]);
+      for (int a = 0; a < D; ++a) { atomicMin(&e[1 + 2*a], v[1 + 2*a]); atomicMax(&e[2 + 2*a], v[2 + 2*a]); }
+    }
+  } else if (valid) {
+    unsigned long long* e = ext + (size_t)r*7;
+    atomicMin(&e[0], v[0]);
+    for (int a = 0; a < D; ++a) { atomicMin(&e[1 + 2*a], v[1 + 2*a]); atomicMax(&e[2 + 2*a], v[2 + 2*a]); }
+  }
+}
+
+__global__ void k_reset_extent(unsigned long long* ext, int nruns) {
+  const int k = blockIdx.x*blockDim.x + threadIdx.x;
+  if (k >= nruns*7) return;
+  const int q = k % 7;
+  ext[k] = (q == 0 || (q & 1)) ? ~0ull : 0ull;
+}
+
+// Non-finite guard on the state (NaN/Inf -> per-run status word).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_finite(Fields f, int n, Scalars* sc) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool ok = isfinite(f.rho[i]) && isfinite(f.eps[i]) && isfinite(f.hinv[i]);
+  for (int a = 0; a < D; ++a) ok = ok && isfinite(f.x[a][i]) && isfinite(f.v[a][i]);
+  if (!ok) atomicOr(&sc[f.run[i]].nonfinite, 1);
 }
 
 }  // namespace spx

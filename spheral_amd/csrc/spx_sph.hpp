@@ -6,32 +6,33 @@
 namespace spx {
 
 // Neighbour record at the midpoint state, image-transformed.
-template <int D>
+template <int D, typename R>
 struct Nbr {
-  double x[D], v[D];
-  double m, rho, cs, h, prho;
+  double x[D];           // positions stay double: differences are formed before rounding to R
+  R v[D];
+  R m, rho, cs, h, prho;
 };
 
-template <int D>
-__device__ __forceinline__ void load_mid(const Fields& f, const Geo& g, int j, int code, Nbr<D>& q) {
+template <int D, typename R>
+__device__ __forceinline__ void load_mid(const Fields& f, const Geo& g, int j, int code, Nbr<D, R>& q) {
 #pragma unroll
-  for (int a = 0; a < D; ++a) { q.x[a] = f.xm[a][j]; q.v[a] = refl_sign<D>(g, code, a)*f.vm[a][j]; }
+  for (int a = 0; a < D; ++a) { q.x[a] = f.xm[a][j]; q.v[a] = (R)(refl_sign<D>(g, code, a)*f.vm[a][j]); }
   if (code) image_pos<D>(g, code, q.x);
-  q.m = f.m[j];
-  q.rho = f.rhom[j];
-  q.cs = f.csm[j];
-  q.h = f.hinvm[j];
-  q.prho = f.prhom[j];
+  q.m = (R)f.m[j];
+  q.rho = (R)f.rhom[j];
+  q.cs = (R)f.csm[j];
+  q.h = (R)f.hinvm[j];
+  q.prho = (R)f.prhom[j];
 }
-template <int D>
-__device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D>& s) {
+template <int D, typename R>
+__device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D, R>& s) {
 #pragma unroll
-  for (int a = 0; a < D; ++a) { s.x[a] = f.xm[a][i]; s.v[a] = f.vm[a][i]; }
-  s.m = f.m[i];
-  s.rho = f.rhom[i];
-  s.cs = f.csm[i];
-  s.h = f.hinvm[i];
-  s.prho = f.prhom[i];
+  for (int a = 0; a < D; ++a) { s.x[a] = f.xm[a][i]; s.v[a] = (R)f.vm[a][i]; }
+  s.m = (R)f.m[i];
+  s.rho = (R)f.rhom[i];
+  s.cs = (R)f.csm[i];
+  s.h = (R)f.hinvm[i];
+  s.prho = (R)f.prhom[i];
 }
 
 // Pair quantities of SPH::evaluateDerivativesImpl (SPH/SPH.cc:362-430) and
@@ -41,19 +42,19 @@ __device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D>& s) {
 // equals its canonical value up to an exact sign flip, and the pairwise acceleration
 // dd is summed in the canonical order so both lanes produce the same bits:
 //   dd_canonical = aFirst ? dd : -dd.
-template <int D>
+template <int D, typename R>
 struct PairS {
-  double r[D], vso[D], gs[D], go[D], es[D], dd[D];
-  double Ws, Wo, Qs, workQs, etaMagS;
+  R r[D], vso[D], gs[D], go[D], es[D], dd[D];
+  R Ws, Wo, Qs, workQs, etaMagS;
 };
 
-template <int D>
-__device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, const Nbr<D>& S, const Nbr<D>& O,
-                                          bool aFirst, PairS<D>& e) {
-  double eo[D], e2s = 0.0, e2o = 0.0;
+template <int D, typename R>
+__device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, const Nbr<D, R>& S, const Nbr<D, R>& O,
+                                          bool aFirst, PairS<D, R>& e) {
+  R eo[D], e2s = (R)0, e2o = (R)0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
-    e.r[c] = S.x[c] - O.x[c];
+    e.r[c] = (R)(S.x[c] - O.x[c]);
     e.es[c] = S.h*e.r[c];
     eo[c] = O.h*e.r[c];
     e2s += e.es[c]*e.es[c];
@@ -61,13 +62,14 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
     e.vso[c] = S.v[c] - O.v[c];
   }
   e.etaMagS = sqrt(e2s);
-  const double etaMagO = sqrt(e2o);
-  const double is = 1.0/fmax(1.0e-30, e.etaMagS), io = 1.0/fmax(1.0e-30, etaMagO);   // safeInvVar
-  double gws, gwo;
-  kernel_wgw(tab, e.etaMagS, hdet_iso<D>(S.h), e.Ws, gws);
-  kernel_wgw(tab, etaMagO, hdet_iso<D>(O.h), e.Wo, gwo);
-  const double ss = gws*S.h, so = gwo*O.h;                   // (gW*H)*etaUnit
-  double mus = 0.0, muo = 0.0;
+  const R etaMagO = sqrt(e2o);
+  const R tiny = sizeof(R) == sizeof(double) ? (R)1.0e-30 : (R)1.0e-30f;
+  const R is = (R)1/fmax(tiny, e.etaMagS), io = (R)1/fmax(tiny, etaMagO);   // safeInvVar
+  R gws, gwo;
+  kernel_wgw<R>(tab, e.etaMagS, hdet_iso<D, R>(S.h), e.Ws, gws);
+  kernel_wgw<R>(tab, etaMagO, hdet_iso<D, R>(O.h), e.Wo, gwo);
+  const R ss = gws*S.h, so = gwo*O.h;                        // (gW*H)*etaUnit
+  R mus = (R)0, muo = (R)0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
     e.gs[c] = ss*(e.es[c]*is);
@@ -75,21 +77,22 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
     mus += e.vso[c]*e.es[c];
     muo += e.vso[c]*eo[c];
   }
-  mus = mus/(e2s + 1.0e-2);
-  muo = muo/(e2o + 1.0e-2);
-  const double mms = fmin(0.0, mus), mmo = fmin(0.0, muo);
-  const double es_ = -ph.Cl*S.cs*mms + ph.Cq*(mms*mms);
-  const double eo_ = -ph.Cl*O.cs*mmo + ph.Cq*(mmo*mmo);
-  const double QPs = es_/S.rho, QPo = eo_/O.rho;
+  mus = mus/(e2s + (R)1.0e-2);
+  muo = muo/(e2o + (R)1.0e-2);
+  const R mms = fmin((R)0, mus), mmo = fmin((R)0, muo);
+  const R Cl = (R)ph.Cl, Cq = (R)ph.Cq;
+  const R es_ = -Cl*S.cs*mms + Cq*(mms*mms);
+  const R eo_ = -Cl*O.cs*mmo + Cq*(mmo*mmo);
+  const R QPs = es_/S.rho, QPo = eo_/O.rho;
   e.Qs = S.rho*es_;
-  double wqs = 0.0;
+  R wqs = (R)0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
-    const double qs = 0.5*(QPs*e.gs[c]), qo = 0.5*(QPo*e.go[c]);
+    const R qs = (R)0.5*(QPs*e.gs[c]), qo = (R)0.5*(QPo*e.go[c]);
     wqs += e.vso[c]*qs;
-    const double p1 = aFirst ? S.prho : O.prho, p2 = aFirst ? O.prho : S.prho;
-    const double g1 = aFirst ? e.gs[c] : e.go[c], g2 = aFirst ? e.go[c] : e.gs[c];
-    const double q1 = aFirst ? qs : qo, q2 = aFirst ? qo : qs;
+    const R p1 = aFirst ? S.prho : O.prho, p2 = aFirst ? O.prho : S.prho;
+    const R g1 = aFirst ? e.gs[c] : e.go[c], g2 = aFirst ? e.go[c] : e.gs[c];
+    const R q1 = aFirst ? qs : qo, q2 = aFirst ? qo : qs;
     e.dd[c] = ((p1*g1 + p2*g2) + q1) + q2;
   }
   e.workQs = wqs;
@@ -101,13 +104,14 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
 // FULL also produces the outputs the integration does not consume (sum density,
 // normalisation, effective Q, density gradient, first moment, M).
 // ---------------------------------------------------------------------------
-template <int D, bool FULL>
+template <int D, bool FULL, typename R>
 __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t ld = (size_t)f.ld;
-  Nbr<D> S;
-  load_self<D>(f, i, S);
+  Nbr<D, R> S;
+  load_self<D, R>(f, i, S);
+  const double Sm = f.m[i], Srho = f.rhom[i], Sh = f.hinvm[i], Sprho = f.prhom[i];
 
   double dvdt[D], dvdx[D*D], Mc[D*D], depsdt = 0.0, maxq = 0.0, m0 = 0.0;
   double rhosum = 0.0, nrm = 0.0, effq = 0.0, grho[D], m1[D];
@@ -121,40 +125,41 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
     const uint32_t ent = f.nbr[(size_t)k*ld + i];
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
-    Nbr<D> O;
-    load_mid<D>(f, g, j, code, O);
-    PairS<D> e;
-    pair_eval<D>(tab, ph, S, O, code != 0 || i < j, e);
-    const double mo = O.m;
-    double vdg = 0.0;
+    Nbr<D, R> O;
+    load_mid<D, R>(f, g, j, code, O);
+    PairS<D, R> e;
+    pair_eval<D, R>(tab, ph, S, O, code != 0 || i < j, e);
+    const double mo = f.m[j];
+    R vdg = (R)0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      dvdt[c] -= mo*e.dd[c];
+      dvdt[c] -= mo*(double)e.dd[c];
       vdg += e.vso[c]*e.gs[c];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        dvdx[c*D + d] -= mo*(e.vso[c]*e.gs[d]);
-        Mc[c*D + d] -= mo*(e.r[c]*e.gs[d]);
+        dvdx[c*D + d] -= mo*(double)(e.vso[c]*e.gs[d]);
+        Mc[c*D + d] -= mo*(double)(e.r[c]*e.gs[d]);
       }
     }
-    depsdt += mo*(S.prho*vdg + e.workQs);
-    maxq = fmax(maxq, e.Qs);
-    const double wsph = kernel_wsph(tab, e.etaMagS);
-    m0 += wsph;
+    depsdt += mo*(double)(S.prho*vdg + e.workQs);
+    maxq = fmax(maxq, (double)e.Qs);
+    const R wsph = kernel_wsph<R>(tab, e.etaMagS);
+    m0 += (double)wsph;
     if (FULL) {
-      rhosum += mo*e.Ws;
-      nrm += S.m/S.rho*e.Ws;
-      effq += mo*e.Qs*e.Ws/O.rho;
+      const double orho = f.rhom[j];
+      rhosum += mo*(double)e.Ws;
+      nrm += Sm/Srho*(double)e.Ws;
+      effq += mo*(double)e.Qs*(double)e.Ws/orho;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
-        grho[c] += mo*(O.rho - S.rho)*e.gs[c];
-        m1[c] -= wsph*e.es[c];
+        grho[c] += mo*(orho - Srho)*(double)e.gs[c];
+        m1[c] -= (double)(wsph*e.es[c]);
       }
     }
   }
 
-  // ---- per-node finalize (SPH.cc:480-551)
-  const double hdet = hdet_iso<D>(S.h);
+  // ---- per-node finalize (SPH.cc:480-551), double precision
+  const double hdet = hdet_iso<D>(Sh);
   double det;
   if (D == 1) det = Mc[0];
   else if (D == 2) det = Mc[0]*Mc[3] - Mc[1]*Mc[2];
@@ -191,7 +196,7 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
     }
   } else {
 #pragma unroll
-    for (int a = 0; a < D*D; ++a) G[a] = rdiv<D>(dvdx[a], S.rho);
+    for (int a = 0; a < D*D; ++a) G[a] = rdiv<D>(dvdx[a], Srho);
   }
   double tr = 0.0;
 #pragma unroll
@@ -199,26 +204,26 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
 #pragma unroll
   for (int a = 0; a < D*D; ++a) f.DvDx[a*ld + i] = G[a];
 #pragma unroll
-  for (int a = 0; a < D; ++a) { f.DvDt[a][i] = dvdt[a]; f.DxDt[a][i] = S.v[a]; }
+  for (int a = 0; a < D; ++a) { f.DvDt[a][i] = dvdt[a]; f.DxDt[a][i] = f.vm[a][i]; }
   f.DepsDt[i] = depsdt;
-  f.DrhoDt[i] = -S.rho*tr;
+  f.DrhoDt[i] = -Srho*tr;
   f.maxQ[i] = maxq;
   // ---- SPHSmoothingScale finalize (SPHSmoothingScale.cc:228-245)
   m0 = rootnu<D>(fmax(0.0, m0));
   f.m0[i] = m0;
-  f.DHDt[i] = rdiv<D>(-S.h, double(D))*tr;
+  f.DHDt[i] = rdiv<D>(-Sh, double(D))*tr;
   const double cur = (fabs(m0) <= 1.0e-15*fmax(1.0, fabs(m0))) ? 0.5*ph.nperh : nperh_lookup(tab, m0);
   const double s = fmin(4.0, fmax(0.25, ph.nperh/(cur + 1.0e-30)));
   const double aa = s < 1.0 ? 0.4*(1.0 + s*s) : 0.4*(1.0 + 1.0/(s*s*s));
-  const double hi0 = 1.0/S.h;
+  const double hi0 = 1.0/Sh;
   const double hi1 = fmin(ph.hmax, fmax(ph.hmin, hi0*(1.0 - aa + aa*s)));
   f.Hideal[i] = 1.0/hi1;
   if (FULL) {
-    f.rhoSum[i] = rhosum + S.m*tab.W0*hdet;
-    f.norm[i] = nrm + S.m/S.rho*tab.W0*hdet;
+    f.rhoSum[i] = rhosum + Sm*tab.W0*hdet;
+    f.norm[i] = nrm + Sm/Srho*tab.W0*hdet;
     f.effQ[i] = effq;
 #pragma unroll
-    for (int c = 0; c < D; ++c) { f.gradRho[c][i] = rdiv<D>(grho[c], S.rho); f.m1[c][i] = m1[c]; }
+    for (int c = 0; c < D; ++c) { f.gradRho[c][i] = rdiv<D>(grho[c], Srho); f.m1[c][i] = m1[c]; }
 #pragma unroll
     for (int a = 0; a < D*D; ++a) f.Mcorr[a*ld + i] = Mc[a];
   }
@@ -232,50 +237,50 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
 // replace policies and the EOS.  v0 is read from f.v, the new velocity goes to
 // f.v1.  ENERGY_ONLY: the package-level policy call (eps only).
 // ---------------------------------------------------------------------------
-template <int D, bool ENERGY_ONLY>
+template <int D, bool ENERGY_ONLY, typename R>
 __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, DevTable tab, const Scalars* sc,
                                                         double multiplier, int n) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double dt = ENERGY_ONLY ? multiplier : sc->dt;
+  const double dt = ENERGY_ONLY ? multiplier : sc[f.run[i]].dt;
   const double hdt = 0.5*dt;
   const size_t ld = (size_t)f.ld;
-  Nbr<D> S;
-  load_self<D>(f, i, S);
-  double v12s[D];
+  Nbr<D, R> S;
+  load_self<D, R>(f, i, S);
+  R v12s[D];
 #pragma unroll
-  for (int a = 0; a < D; ++a) v12s[a] = f.v[a][i] + f.DvDt[a][i]*hdt;
-  const double de0s = f.DepsDt[i];
-  const double epsm = 2.220446049250313e-16;
+  for (int a = 0; a < D; ++a) v12s[a] = (R)(f.v[a][i] + f.DvDt[a][i]*hdt);
+  const R de0s = (R)f.DepsDt[i];
+  const R epsm = sizeof(R) == sizeof(double) ? (R)2.220446049250313e-16 : (R)1.1920929e-07f;
   double du = 0.0;
   const int nn = f.cnt[i];
   for (int k = 0; k < nn; ++k) {
     const uint32_t ent = f.nbr[(size_t)k*ld + i];
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
-    Nbr<D> O;
-    load_mid<D>(f, g, j, code, O);
+    Nbr<D, R> O;
+    load_mid<D, R>(f, g, j, code, O);
     const bool aFirst = code != 0 || i < j;
-    PairS<D> e;
-    pair_eval<D>(tab, ph, S, O, aFirst, e);
+    PairS<D, R> e;
+    pair_eval<D, R>(tab, ph, S, O, aFirst, e);
     // canonical: pacc_ab = (-m_b) dd_ab ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
-    const double mb = aFirst ? O.m : S.m, ma = aFirst ? S.m : O.m;
-    double v12o[D];
-    if (D >= 1) { const double sg = refl_sign<D>(g, code, 0); v12o[0] = sg*f.v[0][j] + (sg*f.DvDt[0][j])*hdt; }
-    if (D >= 2) { const double sg = refl_sign<D>(g, code, 1); v12o[1] = sg*f.v[1][j] + (sg*f.DvDt[1][j])*hdt; }
-    if (D >= 3) { const double sg = refl_sign<D>(g, code, 2); v12o[2] = sg*f.v[2][j] + (sg*f.DvDt[2][j])*hdt; }
-    double duab = 0.0;
+    const R mb = aFirst ? O.m : S.m, ma = aFirst ? S.m : O.m;
+    R v12o[D];
+    if (D >= 1) { const double sg = refl_sign<D>(g, code, 0); v12o[0] = (R)(sg*f.v[0][j] + (sg*f.DvDt[0][j])*hdt); }
+    if (D >= 2) { const double sg = refl_sign<D>(g, code, 1); v12o[1] = (R)(sg*f.v[1][j] + (sg*f.DvDt[1][j])*hdt); }
+    if (D >= 3) { const double sg = refl_sign<D>(g, code, 2); v12o[2] = (R)(sg*f.v[2][j] + (sg*f.DvDt[2][j])*hdt); }
+    R duab = (R)0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      const double va = aFirst ? v12s[c] : v12o[c], vb = aFirst ? v12o[c] : v12s[c];
-      const double ddc = aFirst ? e.dd[c] : -e.dd[c];
+      const R va = aFirst ? v12s[c] : v12o[c], vb = aFirst ? v12o[c] : v12s[c];
+      const R ddc = aFirst ? e.dd[c] : -e.dd[c];
       duab += (vb - va)*((-mb)*ddc);
     }
-    const double de0o = f.DepsDt[j];
-    const double sgn = duab < 0.0 ? -1.0 : 1.0;
-    const double wa = fmax(epsm, (aFirst ? de0s : de0o)*sgn), wb = fmax(epsm, (aFirst ? de0o : de0s)*sgn);
-    const double w = wa/(wa + wb);
-    du += aFirst ? w*duab : (1.0 - w)*duab*ma/mb;
+    const R de0o = (R)f.DepsDt[j];
+    const R sgn = duab < (R)0 ? (R)-1 : (R)1;
+    const R wa = fmax(epsm, (aFirst ? de0s : de0o)*sgn), wb = fmax(epsm, (aFirst ? de0o : de0s)*sgn);
+    const R w = wa/(wa + wb);
+    du += (double)(aFirst ? w*duab : ((R)1 - w)*duab*ma/mb);
   }
   const double eps1 = f.eps[i] + du*dt;
   f.eps[i] = eps1;
@@ -300,40 +305,6 @@ __global__ void __launch_bounds__(kBlock) k_prho(Fields f, int n) {
   if (i >= n) return;
   const double om = f.omegam[i], rho = f.rhom[i];
   f.prhom[i] = (om/(om*om + 1.0e-30))*f.Pm[i]/(rho*rho);
-}
-
-// Reductions for the next step's grid: min hinv and the bounding box (per block).
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_extent(Fields f, int n, double* part) {
-  __shared__ double red[kBlock];
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
-  double vals[1 + 2*3];
-  vals[0] = i < n ? f.hinv[i] : 1.7976931348623157e308;
-  for (int a = 0; a < D; ++a) {
-    const double x = i < n ? f.x[a][i] : 0.0;
-    vals[1 + 2*a] = i < n ? x : 1.7976931348623157e308;
-    vals[2 + 2*a] = i < n ? -x : 1.7976931348623157e308;
-  }
-  for (int q = 0; q < 1 + 2*D; ++q) {
-    red[threadIdx.x] = vals[q];
-    __syncthreads();
-    for (int s = kBlock/2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x*(1 + 2*D) + q] = red[0];
-    __syncthreads();
-  }
-}
-
-// Non-finite guard on the state (NaN/Inf -> status word).
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_finite(Fields f, int n, Scalars* sc) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  bool ok = isfinite(f.rho[i]) && isfinite(f.eps[i]) && isfinite(f.hinv[i]);
-  for (int a = 0; a < D; ++a) ok = ok && isfinite(f.x[a][i]) && isfinite(f.v[a][i]);
-  if (!ok) atomicOr(&sc->pad, 1);
 }
 
 }  // namespace spx

@@ -21,7 +21,13 @@ EXPORTS = (
     "spx_run_neighbors", "spx_run_sum_density", "spx_run_update_eos", "spx_run_post_state_update",
     "spx_run_evaluate_derivatives", "spx_run_dt", "spx_run_compatible_energy", "spx_run_conservation",
     "spx_run_kernel_times", "spx_run_num_pairs",
+    "spx_batch_create", "spx_batch_destroy", "spx_batch_add_run", "spx_batch_remove_run", "spx_batch_step",
+    "spx_batch_run_status", "spx_batch_get_field", "spx_batch_conservation", "spx_batch_usage",
+    "spx_batch_kernel_times",
 )
+
+RUN_ACTIVE = 1
+RUN_DONE = 2
 
 _lib = None
 
@@ -69,6 +75,16 @@ def lib():
             "spx_run_conservation": (C.c_int, [P, D]),
             "spx_run_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
             "spx_run_num_pairs": (C.c_int, [P, I64]),
+            "spx_batch_create": (C.c_int, [cfgp, C.c_int64, C.c_int32, C.c_int32, C.POINTER(P)]),
+            "spx_batch_destroy": (None, [P]),
+            "spx_batch_add_run": (C.c_int, [P, D, D, D, D, D, D, C.c_int64, C.c_double, I32]),
+            "spx_batch_remove_run": (C.c_int, [P, C.c_int32]),
+            "spx_batch_step": (C.c_int, [P, I32]),
+            "spx_batch_run_status": (C.c_int, [P, C.c_int32, I32, D, I64, D, I64]),
+            "spx_batch_get_field": (C.c_int, [P, C.c_int32, C.c_char_p, D, C.c_int64, I64]),
+            "spx_batch_conservation": (C.c_int, [P, C.c_int32, D]),
+            "spx_batch_usage": (C.c_int, [P, I64, I32]),
+            "spx_batch_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -214,6 +230,82 @@ class Run:
         k = C.c_int32()
         buf = C.create_string_buffer(1024)
         _check(lib().spx_run_kernel_times(self.h, _dp(ms), 32, C.byref(k), buf, 1024))
+        names = buf.value.decode().split(",") if k.value else []
+        return dict(zip(names, ms[:k.value].tolist()))
+
+
+class Batch:
+    """Many independent runs of one problem type stepped together (spx_batch)."""
+
+    def __init__(self, cfg: SpxConfig, capacity, max_runs, device=0):
+        self.cfg = cfg.copy()
+        self.dim = cfg.dim
+        self.sizes = {}
+        h = C.c_void_p()
+        _check(lib().spx_batch_create(C.byref(self.cfg), int(capacity), int(max_runs), int(device), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_run(self, state, t_end):
+        arrs = [np.ascontiguousarray(state[k], dtype=np.float64) for k in ("x", "v", "m", "rho", "eps", "H")]
+        n = len(arrs[2])
+        rid = C.c_int32()
+        _check(lib().spx_batch_add_run(self.h, *[_dp(a) for a in arrs], n, float(t_end), C.byref(rid)))
+        self.sizes[rid.value] = n
+        return rid.value
+
+    def remove_run(self, rid):
+        _check(lib().spx_batch_remove_run(self.h, int(rid)))
+        self.sizes.pop(rid, None)
+
+    def step(self):
+        k = C.c_int32()
+        _check(lib().spx_batch_step(self.h, C.byref(k)))
+        return k.value
+
+    def status(self, rid):
+        st, t, cyc, ldt, n = C.c_int32(), C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        _check(lib().spx_batch_run_status(self.h, int(rid), C.byref(st), C.byref(t), C.byref(cyc), C.byref(ldt),
+                                          C.byref(n)))
+        return {"state": st.value, "t": t.value, "cycle": cyc.value, "last_dt": ldt.value, "n": n.value}
+
+    def field(self, rid, name):
+        n = self.sizes[rid]
+        k = FIELD_KIND[name]
+        shape = (n,) if k == "s" else (n, self.dim) if k == "v" else (n, self.dim, self.dim)
+        out = np.zeros(shape, np.float64)
+        cnt = C.c_int64()
+        _check(lib().spx_batch_get_field(self.h, int(rid), name.encode(), _dp(out), out.size, C.byref(cnt)))
+        return out
+
+    def conservation(self, rid):
+        out = np.zeros(7)
+        _check(lib().spx_batch_conservation(self.h, int(rid), _dp(out)))
+        return out
+
+    def usage(self):
+        p, r = C.c_int64(), C.c_int32()
+        _check(lib().spx_batch_usage(self.h, C.byref(p), C.byref(r)))
+        return p.value, r.value
+
+    def enable_timing(self):
+        _check(lib().spx_batch_kernel_times(self.h, None, -1, None, None, 0))
+
+    def kernel_times(self):
+        ms = np.zeros(32)
+        k = C.c_int32()
+        buf = C.create_string_buffer(1024)
+        _check(lib().spx_batch_kernel_times(self.h, _dp(ms), 32, C.byref(k), buf, 1024))
         names = buf.value.decode().split(",") if k.value else []
         return dict(zip(names, ms[:k.value].tolist()))
 

@@ -98,7 +98,6 @@ def kelvin_helmholtz_2d(n_target=4096, seed=7, nperh=2.0, gamma=5.0 / 3.0, P0=2.
     n = x.shape[0]
     band = np.abs(x[:, 1] - 0.5) < 0.25
     rho = np.where(band, 2.0, 1.0)
-    vol = np.where(band, 0.5 * (1.0 / nyi) * dxi * 2.0 / 1.0, dx * dx)
     vol = np.where(band, dxi * (0.5 / nyi), dx * dx)
     m = rho * vol
     v = np.zeros((n, 2))

@@ -1,0 +1,115 @@
+"""Batched runs through the C ABI (spx_batch_*).
+
+A batch steps many independent runs together, each with its own dt, cell grid and
+end time.  The bar is bit-equality: a run evolved inside a batch -- next to runs of
+other sizes, runs that finish earlier, runs removed and admitted mid-flight -- gives
+exactly the state it gives alone as an spx_run.  Also: the mixed-precision mode
+against the FP64 oracle (density and energy L1 <= 1e-5)."""
+import numpy as np
+import pytest
+
+from spheral_amd import abi, problems
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("position", "velocity", "mass density", "specific thermal energy", "H")
+
+
+def alone(gpu, cfg, st, t_end):
+    r = gpu.make_run(cfg, st)
+    r.initialize()
+    k, t = r.advance(t_end, 100000)
+    return r, k, t
+
+
+def assert_same_as_alone(gpu, b, rid, cfg, st, t_end):
+    s = b.status(rid)
+    assert s["state"] == gpu.RUN_DONE
+    r, k, t = alone(gpu, cfg, st, t_end)
+    assert s["cycle"] == k and s["t"] == t and s["last_dt"] == r.time()[2]
+    for f in FIELDS:
+        assert np.array_equal(b.field(rid, f), r.field(f)), f
+    assert np.array_equal(b.conservation(rid), r.conservation())
+
+
+def run_to_completion(b, limit=20000):
+    steps = 0
+    while b.step() > 0:
+        steps += 1
+        assert steps < limit
+    return steps
+
+
+@pytest.mark.parametrize("precision", [abi.SPX_PREC_FP64, abi.SPX_PREC_MIXED], ids=["fp64", "mixed"])
+def test_batch_2d_runs_equal_alone(gpu, precision):
+    sizes, t_ends = [1500, 3000, 2200], [0.02, 0.05, 0.035]
+    print(f"[case] batch kh2d precision={precision} sizes={sizes}", flush=True)
+    cfg, _ = problems.kelvin_helmholtz_2d(sizes[0], precision=precision)
+    states = [problems.kelvin_helmholtz_2d(n, precision=precision)[1] for n in sizes]
+    b = gpu.Batch(cfg, capacity=sum(len(s["m"]) for s in states), max_runs=4)
+    ids = [b.add_run(s, te) for s, te in zip(states, t_ends)]
+    assert b.usage() == (sum(len(s["m"]) for s in states), 3)
+    run_to_completion(b)
+    for rid, st, te in zip(ids, states, t_ends):
+        assert_same_as_alone(gpu, b, rid, cfg, st, te)
+
+
+def test_batch_3d_runs_equal_alone(gpu):
+    print("[case] batch tg3d 16^3 + 12^3", flush=True)
+    cfg, s16 = problems.taylor_green_3d(16)
+    _, s12 = problems.taylor_green_3d(12)
+    b = gpu.Batch(cfg, capacity=16**3 + 12**3, max_runs=2)
+    a = b.add_run(s16, 0.004)
+    c = b.add_run(s12, 0.002)
+    run_to_completion(b)
+    assert_same_as_alone(gpu, b, a, cfg, s16, 0.004)
+    assert_same_as_alone(gpu, b, c, cfg, s12, 0.002)
+
+
+def test_batch_admission_removal_and_capacity(gpu):
+    print("[case] batch admission/removal", flush=True)
+    cfg, sa = problems.kelvin_helmholtz_2d(2000, precision=abi.SPX_PREC_FP64)
+    _, sb = problems.kelvin_helmholtz_2d(1200, precision=abi.SPX_PREC_FP64)
+    _, sc = problems.kelvin_helmholtz_2d(1100, precision=abi.SPX_PREC_FP64)
+    na, nb, nc = len(sa["m"]), len(sb["m"]), len(sc["m"])
+    assert nc <= nb
+    b = gpu.Batch(cfg, capacity=na + nb, max_runs=2)
+    a = b.add_run(sa, 0.04)
+    r2 = b.add_run(sb, 0.04)
+    with pytest.raises(gpu.SpxError) as e:           # no slot left
+        b.add_run(sc, 0.03)
+    assert e.value.code == abi.SPX_ERR_CAPACITY
+    for _ in range(3):
+        b.step()
+    b.remove_run(r2)
+    c = b.add_run(sc, 0.03)                            # reuses the slot after squeezing run r2 out
+    assert c == r2
+    assert b.usage() == (na + nc, 2)
+    with pytest.raises(gpu.SpxError):
+        b.status(5)
+    run_to_completion(b)
+    assert_same_as_alone(gpu, b, a, cfg, sa, 0.04)
+    assert_same_as_alone(gpu, b, c, cfg, sc, 0.03)
+
+
+def test_mixed_precision_accuracy_vs_fp64_oracle(gpu, oracle_mod):
+    """FP32 pair arithmetic on double-differenced offsets with double accumulation: density
+    and specific energy stay within L1 1e-5 (relative) of the FP64 oracle after 5 steps."""
+    print("[case] mixed tg3d 24^3", flush=True)
+    cfg, st = problems.taylor_green_3d(24)
+    cfg.precision = abi.SPX_PREC_MIXED
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    o = oracle_mod.OracleRun(cfg, len(st["m"]))
+    o.set_state(**st)
+    o.initialize()
+    for _ in range(5):
+        dg, do = g.step(), o.step()
+        assert abs(dg - do) <= 1e-4*do
+    for f in ("mass density", "specific thermal energy"):
+        a, r = g.field(f), o.field(f)
+        l1 = np.abs(a - r).sum()/np.abs(r).sum()
+        print(f"  {f}: L1 {l1:.3e}", flush=True)
+        assert l1 <= 1e-5, f
+    v, vo = g.field("velocity"), o.field("velocity")
+    assert np.abs(v - vo).max() <= 1e-4*np.abs(vo).max()

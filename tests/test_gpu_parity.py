@@ -178,7 +178,7 @@ def test_neighbor_symmetry_at_scale(gpu):
 def test_tree_gravity_matches_oracle(gpu, oracle_mod, n, opening):
     """Same octree, same acceptance decisions: accelerations and potential at roundoff."""
     print(f"[case] evrard n={n} opening={opening}", flush=True)
-    cfg, st = problems.evrard_3d(n)
+    cfg, st = problems.evrard_3d(n, precision=abi.SPX_PREC_FP64)
     cfg.opening = opening
     g, o = pair(gpu, oracle_mod, cfg, st)
     g.evaluate_derivatives()
