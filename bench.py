@@ -32,6 +32,19 @@ PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes over this workload, each scaled by the copy-kernel calibration recorded with it
+    (None when no such measurement is committed)."""
+    path = os.path.join(ROOT, "profiles", "traffic_taylor_green_128.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)["kernels"][kernel]
+        return rec["fetch_bytes"] + rec["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -65,6 +78,76 @@ def cpu_baseline(n1d, steps, precision):
             "seconds": dt}
 
 
+def ensemble(args):
+    """Kelvin-Helmholtz ensemble: args.runs independent 2-D periodic runs, N log-uniform in
+    [4096, 262144] (PCG64 seed args.seed), each to t = args.t_end.  Leg 1: all runs in one
+    mixed-precision batch (spx_batch), admitted at once.  Leg 2: the same runs one after
+    another as plain FP64 spx_runs.  runs/hr counts completed runs over wall time including
+    state upload; accuracy is the L1 distance of each batched mixed-precision result from
+    its sequential FP64 result (density, specific energy), relative to the FP64 L1 norm."""
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    targets = np.round(np.exp(rng.uniform(np.log(4096.0), np.log(262144.0), args.runs))).astype(int)
+    cfg_m, _ = problems.kelvin_helmholtz_2d(int(targets[0]), precision=abi.SPX_PREC_MIXED)
+    cfg_d, _ = problems.kelvin_helmholtz_2d(int(targets[0]), precision=abi.SPX_PREC_FP64)
+    states = [problems.kelvin_helmholtz_2d(int(n))[1] for n in targets]
+    sizes = np.array([len(s["m"]) for s in states])
+    total = int(sizes.sum())
+    print(f"[bench] ensemble: {args.runs} KH runs, {total} particles, t_end {args.t_end}", file=sys.stderr, flush=True)
+    # leg 1: one mixed-precision batch
+    t0 = time.perf_counter()
+    b = engine.Batch(cfg_m, capacity=total, max_runs=args.runs)
+    ids = [b.add_run(s, args.t_end) for s in states]
+    bsteps = 0
+    while b.step() > 0:
+        bsteps += 1
+    t_batch = time.perf_counter() - t0
+    cycles = np.array([b.status(r)["cycle"] for r in ids])
+    psteps = int((cycles*sizes).sum())
+    print(f"[bench] batched: {t_batch:.2f} s, {bsteps} batch steps", file=sys.stderr, flush=True)
+    line = {
+        "metric": "ensemble runs/hr",
+        "value": args.runs/t_batch*3600.0,
+        "unit": "runs/hr",
+        "n_gpus": 1,
+        "steps": bsteps,
+        "warmup": 0,
+        "ms_per_step": 1000.0*t_batch/max(bsteps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair / f64 accumulate",
+        "data": "synthetic",
+        "config": {"workload": f"kelvin_helmholtz_2d_ensemble_{args.runs}runs_N4096-262144_t{args.t_end}",
+                   "runs": args.runs, "particles": total, "seed": args.seed, "t_end": args.t_end,
+                   "kernel": "BSpline TableKernel(1000), nPerh 2", "boundary": "periodic unit square"},
+        "batched_seconds": t_batch,
+        "particle_steps": psteps,
+        "particle_steps_per_s": psteps/t_batch,
+    }
+    if not args.no_sequential:
+        t0 = time.perf_counter()
+        l1 = {"mass density": 0.0, "specific thermal energy": 0.0}
+        seq_time = 0.0
+        for rid, st in zip(ids, states):
+            ts = time.perf_counter()
+            r = engine.make_run(cfg_d, st)
+            r.initialize()
+            r.advance(args.t_end, 10**7)
+            ref = {f: r.field(f) for f in l1}
+            r.close()
+            seq_time += time.perf_counter() - ts
+            for f in l1:
+                got = b.field(rid, f)
+                l1[f] = max(l1[f], float(np.abs(got - ref[f]).sum()/np.abs(ref[f]).sum()))
+        line["sequential_fp64_seconds"] = seq_time
+        line["sequential_fp64_runs_per_hr"] = args.runs/seq_time*3600.0
+        line["speedup_vs_sequential_fp64"] = seq_time/t_batch
+        line["max_l1_vs_fp64"] = l1
+        print(f"[bench] sequential fp64: {seq_time:.2f} s", file=sys.stderr, flush=True)
+    b.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,18 +155,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n1d", type=int, default=128)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--reorder", type=int, default=1)
     ap.add_argument("--cpu-n1d", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["taylor_green", "ensemble"], default="taylor_green")
+    ap.add_argument("--runs", type=int, default=256, help="ensemble: number of Kelvin-Helmholtz runs")
+    ap.add_argument("--t-end", type=float, default=0.1, help="ensemble: end time of every run")
+    ap.add_argument("--seed", type=int, default=7, help="ensemble: seed of the run sizes")
+    ap.add_argument("--no-sequential", action="store_true", help="ensemble: skip the sequential FP64 leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), run with --gpus 1"}))
         sys.exit(2)
+    if args.workload == "ensemble":
+        ensemble(args)
+        return
     prec = abi.SPX_PREC_FP64 if args.precision == "fp64" else abi.SPX_PREC_MIXED
     cfg, st = problems.taylor_green_3d(args.n1d, precision=prec)
-    cfg.reorder_interval = args.reorder
     n = len(st["m"])
     print(f"[bench] building Taylor-Green {args.n1d}^3 = {n} particles", file=sys.stderr, flush=True)
     run = engine.make_run(cfg, st)
@@ -126,11 +215,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == abi.SPX_PREC_FP64 else "f32-pair/f64-accumulate",
+        "dtype": "f64" if prec == abi.SPX_PREC_FP64 else "f32 pair / f64 accumulate",
         "data": "synthetic",
         "config": {"workload": f"taylor_green_3d_{args.n1d}^3_wendlandC4_nperh2_cheapRK2", "particles": n,
                    "unique_pairs": npairs, "pairs_per_particle": ppn, "mean_neighbors": 2.0*ppn,
-                   "kernel": "WendlandC4 TableKernel(1000)", "boundary": "periodic", "reorder_interval": args.reorder,
+                   "kernel": "WendlandC4 TableKernel(1000)", "boundary": "periodic",
                    "t_end": t, "cycles": cycle},
         "roofline": {
             "kernel": "k_sph (SPH + SPHSmoothingScale pair pass)",
@@ -153,6 +242,9 @@ def main():
         "conservation": {"momentum": cons[1:4].tolist(), "energy": float(cons[4] + cons[5])},
     }
     line["step_roofline"]["frac"] = value/line["step_roofline"]["ceiling_particle_steps_per_s"]
+    # attainable HBM rate on this box: plain streaming copy kernel, 1 GiB read + 1 GiB write per launch
+    line["hbm_copy_gbps"] = engine.bandwidth_probe(1 << 30, 10)
+    line["roofline"]["traffic"] = measured_traffic(f"k_sph<3, false, {'double' if prec == abi.SPX_PREC_FP64 else 'float'}>")
     if not args.no_cpu:
         print("[bench] timing the CPU baseline (oracle) on a bounded sample", file=sys.stderr, flush=True)
         cb = cpu_baseline(args.cpu_n1d, args.cpu_steps, prec)

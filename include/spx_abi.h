@@ -206,6 +206,10 @@ int spx_run_kernel_times(spx_run* run, double* ms, int32_t max_entries,
                          int32_t* n_entries, char* names, int64_t names_cap);
 /* Unique pairs of the current connectivity (NodePairList size). */
 int spx_run_num_pairs(spx_run* run, int64_t* npairs);
+/* HBM copy probe (no counterpart in the reference: a measurement aid).  Copies `bytes`
+   between two device buffers `reps` times with a plain streaming kernel and returns the
+   achieved rate, read + write bytes per second, in GB/s. */
+int spx_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps);
 
 /* ---- batches: many independent runs of one problem type --------------------
    The reference advances one problem per process: SpheralController.advance

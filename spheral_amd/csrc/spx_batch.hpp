@@ -32,6 +32,9 @@ namespace spx {
 
 static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1)/kBlock); }
 
+// Particles per block of the extent reduction.
+constexpr int64_t kExtentTile = 8192;
+
 // A run has reached its end time (a clamped last step may land an ulp short).
 static inline bool reached_end(double t, double tEnd) { return t >= tEnd || tEnd - t <= 1.0e-15*std::fabs(tEnd); }
 
@@ -48,6 +51,7 @@ class Batch {
   struct RunInfo {
     int state = RUN_FREE;
     int64_t n = 0;
+    int64_t offset = 0;     // first slot of the run's contiguous particle segment
     bool needInit = false;
     bool stepOut = false;   // "new H", "mass zeroth moment" of this run's last evaluation are in place
     bool fullOut = false;   // the evaluate-only outputs (sum density, M, ...) are in place
@@ -85,6 +89,10 @@ class Batch {
   int* cellStart = nullptr;
   int* cellEnd = nullptr;
   int64_t cellsAlloc = 0;
+  int64_t nCells = 0;
+  int* dtiles = nullptr;      // (run, begin, end) particle tiles of the active runs
+  int* htiles = nullptr;
+  int64_t tilesAlloc = 0;
   int64_t* oidAlt = nullptr;
   int32_t* runAlt = nullptr;
   // pinned host mirrors
@@ -129,7 +137,8 @@ class Batch {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto e : evs) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
-    for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow, (void*)hptrs})
+    for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow, (void*)hptrs,
+                    (void*)htiles})
       if (p) (void)hipHostFree(p);
     grav.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -307,6 +316,9 @@ class Batch {
     hinit = halloc<int>(maxRuns);
     hOverflow = halloc<int>(1);
     hptrs = halloc<double*>(6);
+    tilesAlloc = cap/kExtentTile + maxRuns + 1;
+    dtiles = dalloc<int>(3*tilesAlloc);
+    htiles = halloc<int>(3*tilesAlloc);
     upload(dsc, hsc, sizeof(Scalars)*maxRuns);
     ensureNbrCapacity(D == 1 ? 16 : (D == 2 ? 32 : 64));
   }
@@ -314,16 +326,18 @@ class Batch {
   void ensureNbrCapacity(int c) {
     if (c <= f.cap) return;
     HIP_CHECK(hipStreamSynchronize(stream));
-    uint32_t* p = nullptr;
-    HIP_CHECK(hipMalloc((void**)&p, (size_t)c*ld*sizeof(uint32_t)));
-    if (f.nbr) {
-      auto it = std::find(allocs.begin(), allocs.end(), (void*)f.nbr);
+    for (void* old : {(void*)f.nbr, f.pdd}) {
+      if (!old) continue;
+      auto it = std::find(allocs.begin(), allocs.end(), old);
       if (it != allocs.end()) allocs.erase(it);
-      HIP_CHECK(hipFree(f.nbr));
+      HIP_CHECK(hipFree(old));
     }
-    allocs.push_back(p);
-    f.nbr = p;
+    f.nbr = nullptr;
+    f.pdd = nullptr;
+    f.nbr = dalloc<uint32_t>((size_t)c*ld);
+    f.pdd = mixed ? (void*)dalloc<float>((size_t)D*c*ld) : (void*)dalloc<double>((size_t)D*c*ld);
     f.cap = c;
+    npairsCache = -1;
   }
 
   template <typename F>
@@ -398,6 +412,7 @@ class Batch {
     RunInfo info;
     info.state = RUN_ACTIVE;
     info.n = n;
+    info.offset = o;
     info.needInit = true;
     runs[slot] = info;
     Scalars s;
@@ -448,11 +463,25 @@ class Batch {
   // Per-run cell grids from the current extents; returns the Morton bits per axis needed.
   int setupGrids() {
     hipLaunchKernelGGL(k_reset_extent, dim3(nblocks((int64_t)maxRuns*7)), dim3(kBlock), 0, stream, dext, maxRuns);
-    launchD([&](auto d) {
-      constexpr int DD = decltype(d)::value;
-      hipLaunchKernelGGL(k_extent<DD>, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, (int)nTotal, dext);
-    });
-    HIP_CHECK(hipGetLastError());
+    int64_t nt = 0;
+    for (int r = 0; r < maxRuns; ++r) {
+      if (runs[r].state != RUN_ACTIVE) continue;
+      const int64_t b = runs[r].offset, e = b + runs[r].n;
+      for (int64_t t = b; t < e; t += kExtentTile) {
+        htiles[3*nt] = r;
+        htiles[3*nt + 1] = (int)t;
+        htiles[3*nt + 2] = (int)std::min(e, t + kExtentTile);
+        ++nt;
+      }
+    }
+    if (nt > 0) {
+      HIP_CHECK(hipMemcpyAsync(dtiles, htiles, 3*nt*sizeof(int), hipMemcpyHostToDevice, stream));
+      launchD([&](auto d) {
+        constexpr int DD = decltype(d)::value;
+        hipLaunchKernelGGL(k_extent<DD>, dim3((unsigned)nt), dim3(kBlock), 0, stream, f, (const int*)dtiles, dext);
+      });
+      HIP_CHECK(hipGetLastError());
+    }
     download(hext, dext, sizeof(unsigned long long)*maxRuns*7);
     const int64_t maxAxis = D == 3 ? (1 << 15) : (D == 2 ? (1 << 20) : (1 << 26));
     int64_t base = 0;
@@ -523,6 +552,7 @@ class Batch {
       cellEnd = dalloc<int>(c);
       cellsAlloc = c;
     }
+    nCells = base;
     HIP_CHECK(hipMemcpyAsync(dgrids, hgrids, sizeof(RunGrid)*maxRuns, hipMemcpyHostToDevice, stream));
     if (base > 0) {
       HIP_CHECK(hipMemsetAsync(cellStart, 0, base*sizeof(int), stream));
@@ -571,12 +601,16 @@ class Batch {
     std::swap(f.DvDx, alt.DvDx);
     std::swap(f.oid, oidAlt);
     std::swap(f.run, runAlt);
+    // segments in key order: active runs by slot, then finished runs by slot
     nActive = 0;
     int64_t live = 0;
     for (int r = 0; r < maxRuns; ++r) {
       RunInfo& ri = runs[r];
-      if (ri.state == RUN_ACTIVE) nActive += ri.n;
-      if (ri.state == RUN_ACTIVE || ri.state == RUN_DONE) live += ri.n;
+      if (ri.state == RUN_ACTIVE) { ri.offset = live; live += ri.n; nActive += ri.n; }
+    }
+    for (int r = 0; r < maxRuns; ++r) {
+      RunInfo& ri = runs[r];
+      if (ri.state == RUN_DONE) { ri.offset = live; live += ri.n; }
       if (ri.state == RUN_REMOVED) ri = RunInfo();
       ri.stepOut = ri.fullOut = false;    // the per-step outputs were not carried through the reorder
     }
@@ -584,7 +618,8 @@ class Batch {
     npairsCache = -1;
   }
 
-  void buildConnectivity() {
+  // withDensityDt: follow the neighbour build with the step's sum density and dt vote.
+  void buildConnectivity(bool withDensityDt) {
     if (nTotal == 0) raise(SPX_ERR_STATE, "no state uploaded");
     const int bits = setupGrids();
     sortAndGather(bits);
@@ -596,14 +631,18 @@ class Batch {
       hipLaunchKernelGGL(k_cell_bounds<DD>, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t*)keysAlt, (int)nActive,
                          (const RunGrid*)dgrids, mortonBits, cellStart, cellEnd);
     });
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_nodescale<DD>, dim3(nb), dim3(kBlock), 0, stream, f, ph, (int)nActive);
+    });
     HIP_CHECK(hipGetLastError());
     bool ok = false;
     for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
       HIP_CHECK(hipMemsetAsync(dOverflow, 0, sizeof(int), stream));
       launchD([&](auto d) {
         constexpr int DD = decltype(d)::value;
-        hipLaunchKernelGGL(k_neighbors<DD>, dim3(nb), dim3(kBlock), 0, stream, f, geo, ph, (const RunGrid*)dgrids,
-                           (int)nActive, (const int*)cellStart, (const int*)cellEnd, (const uint32_t*)nullptr, dOverflow);
+        hipLaunchKernelGGL(k_neighbors<DD>, dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f, geo, ph,
+                           (const RunGrid*)dgrids, (int)nActive, (const int*)cellStart, (const int*)cellEnd, dOverflow);
       });
       HIP_CHECK(hipGetLastError());
       download(hOverflow, dOverflow, sizeof(int));
@@ -613,6 +652,7 @@ class Batch {
       ensureNbrCapacity(((need + need/4) + 7)/8*8);
     }
     if (!ok) raise(SPX_ERR_CAPACITY, "neighbour list capacity not reached");
+    if (withDensityDt) densityDt(true, true);
     for (int a = 0; a < 3; ++a) { hptrs[a] = f.xm[a]; hptrs[3 + a] = f.x[a]; }
     HIP_CHECK(hipMemcpyAsync(xsMid, hptrs, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(xsState, hptrs + 3, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
@@ -646,7 +686,7 @@ class Batch {
   }
   void omegaPass(bool mid, const int* runMask = nullptr) {
     if (!cfg.gradh_correction || nActive == 0) return;
-    const int nb = nblocks(nActive);
+    const int nb = (int)xcd_grid(nActive, kBlock);
     double* const* xs = mid ? xsMid : xsState;
     const double* hs = mid ? f.hinvm : f.hinv;
     double* om = mid ? f.omegam : f.omega;
@@ -657,7 +697,7 @@ class Batch {
   }
   void densityDt(bool dens, bool dtv) {
     if (nActive == 0) return;
-    const int nb = nblocks(nActive);
+    const int nb = (int)xcd_grid(nActive, kBlock);
     if (dtv) hipLaunchKernelGGL(k_reset_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
@@ -670,9 +710,10 @@ class Batch {
     });
   }
   void sphPass(bool full) {
-    const int nb = nblocks(nActive);
+    const int nb = (int)xcd_grid(nActive, kBlock);
     hipLaunchKernelGGL(k_prho, dim3(nb), dim3(kBlock), 0, stream, f, (int)nActive);
     HIP_CHECK(hipGetLastError());
+    mark("prho");
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (full) launchPrec(k_sph<DD, true, double>, k_sph<DD, true, float>, nb, f, geo, ph, dtab, (int)nActive);
@@ -721,18 +762,22 @@ class Batch {
   }
 
   // ---------------------------------------------------------------- lifecycle
-  // Start-up of runs admitted since the last step (the reinitializeProblem subset:
-  // EOS, grad-h Omega, TreeGravity dry run), on this step's connectivity.
-  void initNewRuns() {
+  // Start-up of runs admitted since the last step (the reinitializeProblem subset):
+  // EOS on the uploaded density before anything replaces it (prepare), then, on this
+  // step's connectivity, grad-h Omega and the TreeGravity dry run (finish).
+  bool prepareNewRuns() {
     bool any = false;
     for (int r = 0; r < maxRuns; ++r) {
       hinit[r] = runs[r].needInit && runs[r].state == RUN_ACTIVE ? 1 : 0;
       any |= hinit[r] != 0;
     }
-    if (!any) return;
+    if (!any) return false;
     HIP_CHECK(hipMemcpyAsync(dinit, hinit, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_eos_masked, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, ph, (const int*)dinit, (int)nActive);
+    hipLaunchKernelGGL(k_eos_masked, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, ph, (const int*)dinit, (int)nTotal);
     HIP_CHECK(hipGetLastError());
+    return true;
+  }
+  void finishNewRuns() {
     omegaPass(false, dinit);
     if (cfg.gravity) gravityPass(false);
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -748,8 +793,9 @@ class Batch {
     for (double* p : {f.DepsDt, f.DrhoDt, f.DHDt, f.maxQ}) z(p);
     for (int c = 0; c < D*D; ++c) z(f.DvDx + (size_t)c*ld);
     for (auto& r : runs) if (r.state == RUN_ACTIVE) r.needInit = true;
-    buildConnectivity();
-    initNewRuns();
+    const bool fresh = prepareNewRuns();
+    buildConnectivity(false);
+    if (fresh) finishNewRuns();
     initialized = true;
   }
 
@@ -762,19 +808,19 @@ class Batch {
       tnames.clear();
       mark("start");
     }
-    // Integrator::step -> boundary enforcement, connectivity
-    buildConnectivity();
+    const bool fresh = prepareNewRuns();
+    // Integrator::step -> boundary enforcement, connectivity; preStepInitialize
+    // (RigorousSumDensity) and the hydro dt vote ride on the neighbour pass
+    buildConnectivity(true);
     if (nActive == 0) return;
     mark("connectivity");
-    initNewRuns();
-    const int nb = nblocks(nActive);
-    // preStepInitialize (RigorousSumDensity) fused with the dt vote; TreeGravity's
-    // vote is the previous evaluation's
-    densityDt(true, true);
+    if (fresh) finishNewRuns();
+    const int nb = (int)xcd_grid(nActive, kBlock);
+    // Integrator::selectDt; TreeGravity's vote is the previous evaluation's
     hipLaunchKernelGGL(k_select_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns, ph, cfg.dt_min,
                        cfg.dt_max, cfg.dt_growth, cfg.gravity);
     HIP_CHECK(hipGetLastError());
-    mark("density_dt");
+    mark("select_dt");
     // half step with the previous derivatives
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
@@ -791,7 +837,7 @@ class Batch {
     // full step from the start-of-step state, compatible energy first
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      launchPrec(k_full_update<DD, false, double>, k_full_update<DD, false, float>, nb, f, geo, ph, dtab,
+      launchPrec(k_full_update<DD, false, double>, k_full_update<DD, false, float>, nb, f, geo, ph,
                  (const Scalars*)dsc, 0.0, (int)nActive);
     });
     for (int a = 0; a < D; ++a) {
@@ -859,7 +905,7 @@ class Batch {
     requireConn();
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      launchPrec(k_full_update<DD, true, double>, k_full_update<DD, true, float>, nblocks(nActive), f, geo, ph, dtab,
+      launchPrec(k_full_update<DD, true, double>, k_full_update<DD, true, float>, (int)xcd_grid(nActive, kBlock), f, geo, ph,
                  (const Scalars*)dsc, multiplier, (int)nActive);
     });
     eos();

@@ -17,6 +17,22 @@ constexpr int kImgShift = 26;
 constexpr uint32_t kIdxMask = (1u << kImgShift) - 1u;
 constexpr int kBlock = 256;
 
+// The command processor hands workgroup b to XCD b % 8, and each XCD has its own L2.
+// Pair kernels map blocks so that each XCD works through one contiguous range of
+// (Morton-ordered) particles and keeps its neighbours' data in its own L2.  Grids of
+// such kernels are launched as a multiple of kXcd blocks (xcd_grid); any other grid
+// keeps the identity mapping.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int nb = (int)gridDim.x;
+  if (nb % kXcd) return (int)blockIdx.x;
+  return ((int)blockIdx.x % kXcd)*(nb/kXcd) + (int)blockIdx.x/kXcd;
+}
+static inline unsigned xcd_grid(long long n, int block) {
+  const long long nb = (n + block - 1)/block;
+  return (unsigned)((nb + kXcd - 1)/kXcd*kXcd);
+}
+
 enum Bc { BC_NONE = 0, BC_PERIODIC = 1, BC_REFLECTING = 2 };
 
 // Life cycle of a run slot in a batch.
@@ -74,6 +90,10 @@ struct Fields {
   double *ns;        // nodeScale = 1/(lambda_max(H) nPerh) of the connectivity state
   double *pot;       // gravitational potential
   uint32_t* nbr;
+  // Pairwise acceleration term of each ELL entry (the reference's pairAccelerations,
+  // without the partner mass), in the pair arithmetic type: component c of entry k of
+  // slot i at pdd[(c*cap + k)*ld + i], from the slot's side of the canonical pair.
+  void* pdd;
   int32_t* cnt;
   int64_t* oid;      // (run slot << 40) | node index within the run
   int32_t* run;      // run slot of each particle

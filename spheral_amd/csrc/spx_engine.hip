@@ -17,6 +17,13 @@
 
 namespace spx {
 static thread_local std::string g_last_error;
+
+// Plain streaming copy, 16 bytes per lane per iteration: the reference point for the
+// achievable HBM bandwidth and for scaling the memory-traffic counters.
+__global__ void __launch_bounds__(kBlock) k_copy(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x*blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
 }  // namespace spx
 
 struct spx_run {
@@ -148,6 +155,48 @@ static void kernel_times(spx::Batch& B, double* ms, int32_t max_entries, int32_t
 #define RUN_STATE(run) \
   do { if (!(run)->haveState) spx::raise(SPX_ERR_STATE, "no state uploaded"); } while (0)
 
+int spx_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps) {
+  return guarded([&] {
+    if (bytes < 16 || reps < 1 || !gbps) spx::raise(SPX_ERR_ARG, "bad arguments");
+    check_device(device);
+    HIP_CHECK(hipSetDevice(device));
+    const int64_t n = bytes/16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (st) (void)hipStreamDestroy(st);
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+    };
+    try {
+      HIP_CHECK(hipMalloc(&a, n*16));
+      HIP_CHECK(hipMalloc(&b, n*16));
+      HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      HIP_CHECK(hipEventCreate(&e0));
+      HIP_CHECK(hipEventCreate(&e1));
+      HIP_CHECK(hipMemsetAsync(a, 0, n*16, st));
+      const unsigned grid = 256*32;
+      hipLaunchKernelGGL(spx::k_copy, dim3(grid), dim3(spx::kBlock), 0, st, (const double2*)a, (double2*)b, n);  // warm-up
+      HIP_CHECK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(spx::k_copy, dim3(grid), dim3(spx::kBlock), 0, st, (const double2*)a, (double2*)b, n);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipEventRecord(e1, st));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      *gbps = 2.0*(double)n*16.0*reps/(ms*1.0e-3)/1.0e9;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
 int spx_run_create(const spx_config* cfg, int64_t n, int32_t device, spx_run** out) {
   return guarded([&] {
     if (!cfg || !out) spx::raise(SPX_ERR_ARG, "null argument");
@@ -232,7 +281,7 @@ int spx_run_update_connectivity(spx_run* run) {
   RUN_OR_FAIL(run);
   return guarded([&] {
     RUN_STATE(run);
-    run->impl->buildConnectivity();
+    run->impl->buildConnectivity(false);
     HIP_CHECK(hipStreamSynchronize(run->impl->stream));
   });
 }

@@ -137,22 +137,43 @@ __global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, in
   if (p == n - 1 || keys[p + 1] != k) { unspread<D>(k, mortonBits, c); end[lin_cell<D>(grids[r], c)] = p + 1; }
 }
 
+// Running per-run minimum of a positive double: one atomic per wavefront when its lanes share a run.
+__device__ __forceinline__ void run_min(Scalars* sc, int r, double v, bool valid) {
+  const int r0 = __shfl(r, 0);
+  const bool uniform = __all(!valid || r == r0);
+  if (uniform) {
+    double m = valid ? v : 1.7976931348623157e308;
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+    // most waves do not lower the running minimum: read first, then update atomically
+    const unsigned long long b = ord_bits(m);
+    if ((threadIdx.x & 63) == 0 && __any(valid) && b < sc[r0].dtBits) atomicMin(&sc[r0].dtBits, b);
+  } else if (valid) {
+    const unsigned long long b = ord_bits(v);
+    if (b < sc[r].dtBits) atomicMin(&sc[r].dtBits, b);
+  }
+}
+
+// nodeScale = 1/(lambda_max(H) nPerh) of each node (GenericHydro::dt, Physics/GenericHydro.cc:142-150).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_nodescale(Fields f, Phys ph, int n) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) f.ns[i] = 1.0/max_eigen_iso<D>(f.hinv[i])/ph.nperh;
+}
+
 // ---------------------------------------------------------------------------
-// ConnectivityMap::computeConnectivity (Neighbor/ConnectivityMap.cc:905-947):
-// j is a neighbour of i iff |Hi rij|^2 <= ext^2 or |Hj rij|^2 <= ext^2, i != j
-// (an image of i itself counts, as the reference's ghost of i does).  The test is
-// evaluated with the reference's rounding (no contraction), so sets match exactly.
-// Also records nodeScale = 1/(lambda_max(H) nPerh) for the dt vote.
-// sorted: slot of the p-th particle in cell order, or nullptr when slots are sorted.
-// ---------------------------------------------------------------------------
+// ConnectivityMap::computeConnectivity (Neighbor/ConnectivityMap.cc:905-947): j is a
+// neighbour of i iff |Hi rij|^2 <= ext^2 or |Hj rij|^2 <= ext^2, i != j (an image of i
+// itself counts, as the reference's ghost of i does).  The test is evaluated with the
+// reference's rounding (no contraction), so the sets match exactly.  One lane per
+// particle walks the 3^D cells around its own; lanes of a wavefront mostly share a cell,
+// so candidate reads are wave-uniform.  Candidate order (cells, then slots) is the ELL
+// order.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, const RunGrid* grids, int n,
-                                                      const int* cstart, const int* cend, const uint32_t* sorted,
-                                                      int* overflow) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+                                                      const int* cstart, const int* cend, int* overflow) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int r = f.run[i];
-  const RunGrid gr = grids[r];
+  const RunGrid gr = grids[f.run[i]];
   double xi[D];
   int ci[3] = {0, 0, 0};
 #pragma unroll
@@ -161,9 +182,9 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
     ci[a] = cell_coord<D>(gr, a, xi[a]);
   }
   const double hi = f.hinv[i];
-  f.ns[i] = 1.0/max_eigen_iso<D>(hi)/ph.nperh;
-  int count = 0;
+  const size_t ld = (size_t)f.ld;
   const int cap = f.cap;
+  int count = 0;
   const int r1 = D >= 2 ? 1 : 0, r2 = D >= 3 ? 1 : 0;
   for (int dz = -r2; dz <= r2; ++dz)
     for (int dy = -r1; dy <= r1; ++dy)
@@ -189,9 +210,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
         }
         if (skip) continue;
         const int cell = lin_cell<D>(gr, cc);
-        const int p0 = cstart[cell], p1 = cend[cell];
-        for (int p = p0; p < p1; ++p) {
-          const int j = sorted ? (int)sorted[p] : p;
+        const int q1 = cend[cell];
+        for (int j = cstart[cell]; j < q1; ++j) {
           if (j == i && code == 0) continue;
           double xj[D];
 #pragma unroll
@@ -207,26 +227,13 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
             e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
           }
           if (e2i <= ph.ext2 || e2j <= ph.ext2) {
-            if (count < cap) f.nbr[(size_t)count*f.ld + i] = (uint32_t)j | ((uint32_t)code << kImgShift);
+            if (count < cap) f.nbr[(size_t)count*ld + i] = (uint32_t)j | ((uint32_t)code << kImgShift);
             ++count;
           }
         }
       }
   f.cnt[i] = count;
   if (count > cap) atomicMax(overflow, count);
-}
-
-// Running per-run minimum of a positive double: one atomic per wavefront when its lanes share a run.
-__device__ __forceinline__ void run_min(Scalars* sc, int r, double v, bool valid) {
-  const int r0 = __shfl(r, 0);
-  const bool uniform = __all(!valid || r == r0);
-  if (uniform) {
-    double m = valid ? v : 1.7976931348623157e308;
-    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0 && __any(valid)) atomicMin(&sc[r0].dtBits, ord_bits(m));
-  } else if (valid) {
-    atomicMin(&sc[r].dtBits, ord_bits(v));
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -237,7 +244,7 @@ __device__ __forceinline__ void run_min(Scalars* sc, int r, double v, bool valid
 // ---------------------------------------------------------------------------
 template <int D, bool DENS, bool DT, typename R>
 __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
   double dtmin = 1.7976931348623157e308;
   const bool valid = i < n;
   int r = 0;
@@ -390,7 +397,7 @@ __global__ void __launch_bounds__(kBlock) k_eos(Fields f, Phys ph, int n) {
 template <int D, typename R>
 __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, double* const* xs,
                                                   const double* hs, double* omega, const int* runMask) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (runMask && !runMask[f.run[i]]) return;
   double xi[D];
@@ -427,43 +434,48 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
   omega[i] = om;
 }
 
-// Per-run extents for the next grid: min hinv and the bounding box (atomics on ordered bits).
-// ext layout per run: [hinv_min, xmin0, xmax0, xmin1, xmax1, xmin2, xmax2] as ordered bits.
+// Per-run extents for the next grid: min hinv and the bounding box, as ordered bits.
+// One block per tile (run, begin, end) of a run's contiguous particle segment; a block
+// reduces its tile and adds one set of atomics.
+// ext layout per run: [hinv_min, xmin0, xmax0, xmin1, xmax1, xmin2, xmax2].
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_extent(Fields f, int n, unsigned long long* ext) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  const int r = valid ? f.run[i] : 0;
+__global__ void __launch_bounds__(kBlock) k_extent(Fields f, const int* tiles, unsigned long long* ext) {
+  __shared__ unsigned long long red[7][kBlock/64];
+  const int r = tiles[3*blockIdx.x], b = tiles[3*blockIdx.x + 1], e = tiles[3*blockIdx.x + 2];
   unsigned long long v[7];
-  v[0] = valid ? ord_bits(f.hinv[i]) : ~0ull;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const unsigned long long b = (valid && a < D) ? ord_bits(f.x[a][i]) : 0ull;
-    v[1 + 2*a] = (valid && a < D) ? b : ~0ull;
-    v[2 + 2*a] = b;
+  for (int q = 0; q < 7; ++q) v[q] = (q == 0 || (q & 1)) ? ~0ull : 0ull;
+  for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
+    const unsigned long long h = ord_bits(f.hinv[i]);
+    v[0] = h < v[0] ? h : v[0];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const unsigned long long x = ord_bits(f.x[a][i]);
+      v[1 + 2*a] = x < v[1 + 2*a] ? x : v[1 + 2*a];
+      v[2 + 2*a] = x > v[2 + 2*a] ? x : v[2 + 2*a];
+    }
   }
-  const int r0 = __shfl(r, 0);
-  const bool uniform = __all(!valid || r == r0);
-  if (uniform) {
-    // wavefront reduction first, one set of atomics per wavefront
 #pragma unroll
-    for (int q = 0; q < 7; ++q) {
-      unsigned long long x = v[q];
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long y = __shfl_xor(x, off);
+  for (int q = 0; q < 7; ++q) {
+    unsigned long long x = v[q];
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long y = __shfl_xor(x, o);
+      x = (q == 0 || (q & 1)) ? (y < x ? y : x) : (y > x ? y : x);
+    }
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long* out = ext + (size_t)r*7;
+    for (int q = 0; q < 1 + 2*D; ++q) {
+      unsigned long long x = red[q][0];
+      for (int w = 1; w < kBlock/64; ++w) {
+        const unsigned long long y = red[q][w];
         x = (q == 0 || (q & 1)) ? (y < x ? y : x) : (y > x ? y : x);
       }
-      v[q] = x;
+      if (q == 0 || (q & 1)) atomicMin(&out[q], x);
+      else atomicMax(&out[q], x);
     }
-    if ((threadIdx.x & 63) == 0 && __any(valid)) {
-      unsigned long long* e = ext + (size_t)r0*7;
-      atomicMin(&e[0], v[0]);
-      for (int a = 0; a < D; ++a) { atomicMin(&e[1 + 2*a], v[1 + 2*a]); atomicMax(&e[2 + 2*a], v[2 + 2*a]); }
-    }
-  } else if (valid) {
-    unsigned long long* e = ext + (size_t)r*7;
-    atomicMin(&e[0], v[0]);
-    for (int a = 0; a < D; ++a) { atomicMin(&e[1 + 2*a], v[1 + 2*a]); atomicMax(&e[2 + 2*a], v[2 + 2*a]); }
   }
 }
 

@@ -106,12 +106,13 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
 // ---------------------------------------------------------------------------
 template <int D, bool FULL, typename R>
 __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const size_t ld = (size_t)f.ld;
+  const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
+  R* pdd = (R*)f.pdd;
   Nbr<D, R> S;
   load_self<D, R>(f, i, S);
-  const double Sm = f.m[i], Srho = f.rhom[i], Sh = f.hinvm[i], Sprho = f.prhom[i];
+  const double Sm = f.m[i], Srho = f.rhom[i], Sh = f.hinvm[i];
 
   double dvdt[D], dvdx[D*D], Mc[D*D], depsdt = 0.0, maxq = 0.0, m0 = 0.0;
   double rhosum = 0.0, nrm = 0.0, effq = 0.0, grho[D], m1[D];
@@ -129,6 +130,8 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
     load_mid<D, R>(f, g, j, code, O);
     PairS<D, R> e;
     pair_eval<D, R>(tab, ph, S, O, code != 0 || i < j, e);
+#pragma unroll
+    for (int c = 0; c < D; ++c) pdd[((size_t)c*cap + k)*ld + i] = e.dd[c];
     const double mo = f.m[j];
     R vdg = (R)0;
 #pragma unroll
@@ -232,21 +235,21 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
 // ---------------------------------------------------------------------------
 // Full step from the start-of-step state with the midpoint derivatives
 // (CheapSynchronousRK2.cc:106-109): SpecificThermalEnergyPolicy::update
-// (Hydro/SpecificThermalEnergyPolicy.cc:79-144) with the pair accelerations
-// recomputed from the midpoint state in canonical order, then the increment and
-// replace policies and the EOS.  v0 is read from f.v, the new velocity goes to
-// f.v1.  ENERGY_ONLY: the package-level policy call (eps only).
+// (Hydro/SpecificThermalEnergyPolicy.cc:79-144) on the pair accelerations the SPH
+// pass stored per ELL entry (each already in canonical summation order), then the
+// increment and replace policies and the EOS.  v0 is read from f.v, the new velocity
+// goes to f.v1.  ENERGY_ONLY: the package-level policy call (eps only).
 // ---------------------------------------------------------------------------
 template <int D, bool ENERGY_ONLY, typename R>
-__global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, DevTable tab, const Scalars* sc,
-                                                        double multiplier, int n) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, const Scalars* sc, double multiplier,
+                                                        int n) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dt = ENERGY_ONLY ? multiplier : sc[f.run[i]].dt;
   const double hdt = 0.5*dt;
-  const size_t ld = (size_t)f.ld;
-  Nbr<D, R> S;
-  load_self<D, R>(f, i, S);
+  const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
+  const R* pdd = (const R*)f.pdd;
+  const R ms = (R)f.m[i];
   R v12s[D];
 #pragma unroll
   for (int a = 0; a < D; ++a) v12s[a] = (R)(f.v[a][i] + f.DvDt[a][i]*hdt);
@@ -258,13 +261,10 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
     const uint32_t ent = f.nbr[(size_t)k*ld + i];
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
-    Nbr<D, R> O;
-    load_mid<D, R>(f, g, j, code, O);
     const bool aFirst = code != 0 || i < j;
-    PairS<D, R> e;
-    pair_eval<D, R>(tab, ph, S, O, aFirst, e);
     // canonical: pacc_ab = (-m_b) dd_ab ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
-    const R mb = aFirst ? O.m : S.m, ma = aFirst ? S.m : O.m;
+    const R mo = (R)f.m[j];
+    const R mb = aFirst ? mo : ms, ma = aFirst ? ms : mo;
     R v12o[D];
     if (D >= 1) { const double sg = refl_sign<D>(g, code, 0); v12o[0] = (R)(sg*f.v[0][j] + (sg*f.DvDt[0][j])*hdt); }
     if (D >= 2) { const double sg = refl_sign<D>(g, code, 1); v12o[1] = (R)(sg*f.v[1][j] + (sg*f.DvDt[1][j])*hdt); }
@@ -272,8 +272,9 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
     R duab = (R)0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
+      const R dd = pdd[((size_t)c*cap + k)*ld + i];
       const R va = aFirst ? v12s[c] : v12o[c], vb = aFirst ? v12o[c] : v12s[c];
-      const R ddc = aFirst ? e.dd[c] : -e.dd[c];
+      const R ddc = aFirst ? dd : -dd;
       duab += (vb - va)*((-mb)*ddc);
     }
     const R de0o = (R)f.DepsDt[j];
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
 
 // Per-node pressure term Prho = safeInv(omega)*P/rho^2 of the midpoint state (SPH.cc:425-426).
 __global__ void __launch_bounds__(kBlock) k_prho(Fields f, int n) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double om = f.omegam[i], rho = f.rhom[i];
   f.prhom[i] = (om/(om*om + 1.0e-30))*f.Pm[i]/(rho*rho);

@@ -23,7 +23,7 @@ EXPORTS = (
     "spx_run_kernel_times", "spx_run_num_pairs",
     "spx_batch_create", "spx_batch_destroy", "spx_batch_add_run", "spx_batch_remove_run", "spx_batch_step",
     "spx_batch_run_status", "spx_batch_get_field", "spx_batch_conservation", "spx_batch_usage",
-    "spx_batch_kernel_times",
+    "spx_batch_kernel_times", "spx_bandwidth_probe",
 )
 
 RUN_ACTIVE = 1
@@ -85,6 +85,7 @@ def lib():
             "spx_batch_conservation": (C.c_int, [P, C.c_int32, D]),
             "spx_batch_usage": (C.c_int, [P, I64, I32]),
             "spx_batch_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
+            "spx_bandwidth_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -101,6 +102,13 @@ def _check(code):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def bandwidth_probe(nbytes=1 << 30, reps=10, device=0):
+    """Achieved HBM copy rate (read + write) of a plain streaming kernel, GB/s."""
+    g = C.c_double()
+    _check(lib().spx_bandwidth_probe(int(device), int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def device_count():
