@@ -1,0 +1,6 @@
+"""``from spheral_amd.Spheral1d import *``: the reference's 1-D Python names for the SPH +
+TreeGravity path, bound to the GPU engine (see spheral_amd.mirror)."""
+from .mirror import namespace as _namespace
+
+globals().update(_namespace(1))
+__all__ = sorted(k for k in globals() if not k.startswith("_"))

@@ -1,0 +1,121 @@
+"""The reference-named Python surface (spheral_amd.Spheral{1,2,3}d): a script written the way
+the reference's Sod test builds its problem maps onto the same engine configuration as the
+direct ABI binding, and unsupported options raise instead of silently differing."""
+import numpy as np
+import pytest
+
+from spheral_amd import Spheral1d as S1
+from spheral_amd import Spheral3d as S3
+from spheral_amd import abi, problems
+
+
+def sod_script(n=400, two_lists=True):
+    """Sod set-up in the style of tests/functional/Hydro/Sod/Sod-planar-1d.py (two NodeLists)."""
+    cfg_ref, st = problems.sod_1d(n)
+    eos = S1.GammaLawGas(1.4, 2.0)
+    WT = S1.TableKernel(S1.BSplineKernel(), 1000)
+    left = st["x"][:, 0] < 0.0
+    lists = []
+    parts = [left, ~left] if two_lists else [np.ones(n, bool)]
+    for k, sel in enumerate(parts):
+        nl = S1.makeFluidNodeList(f"nodes{k + 1}", eos, numInternal=int(sel.sum()), hmin=1e-20, hmax=1e20,
+                                  nPerh=1.51, kernelExtent=WT.kernelExtent, rhoMin=1e-10)
+        nl.positions()[...] = st["x"][sel]
+        nl.velocity()[...] = st["v"][sel]
+        nl.mass()[...] = st["m"][sel]
+        nl.massDensity()[...] = st["rho"][sel]
+        nl.specificThermalEnergy()[...] = st["eps"][sel]
+        nl.Hfield()[...] = st["H"][sel]
+        lists.append(nl)
+    db = S1.DataBase()
+    for nl in lists:
+        db.appendNodeList(nl)
+    q = S1.MonaghanGingoldViscosity(1.0, 1.0)
+    hydro = S1.SPH(dataBase=db, W=WT, Q=q, cfl=0.25, compatibleEnergyEvolution=True, gradhCorrection=True,
+                   XSPH=False, correctVelocityGradient=True)
+    for bc in (S1.ReflectingBoundary(S1.Plane(S1.Vector(-0.5), S1.Vector(1.0))),
+               S1.ReflectingBoundary(S1.Plane(S1.Vector(0.5), S1.Vector(-1.0)))):
+        hydro.appendBoundary(bc)
+    integ = S1.CheapSynchronousRK2Integrator(db)
+    integ.appendPhysicsPackage(hydro)
+    integ.lastDt = 1e-6
+    integ.dtMin = 1e-8
+    return cfg_ref, st, integ, lists
+
+
+FIELDS = [name for name, _ in abi.SpxConfig._fields_]
+
+
+def test_script_config_matches_problem_config():
+    cfg_ref, _, integ, _ = sod_script()
+    cfg = integ.config()
+    for name in FIELDS:
+        a, b = getattr(cfg, name), getattr(cfg_ref, name)
+        if hasattr(a, "__len__"):
+            assert list(a) == list(b), name
+        else:
+            assert a == b, name
+
+
+def test_nodelists_concatenate_in_order():
+    _, st, integ, lists = sod_script()
+    g = integ._gather_state()
+    for k in ("x", "v", "m", "rho", "eps", "H"):
+        assert np.array_equal(g[k], st[k]), k
+    assert [(a, b) for _, a, b in integ._slices] == [(0, lists[0].numInternalNodes), (lists[0].numInternalNodes, 400)]
+
+
+def test_reference_defaults_that_the_gpu_path_lacks_raise():
+    _, _, integ, lists = sod_script()
+    db = integ.dataBase
+    WT = S1.TableKernel(S1.BSplineKernel(), 1000)
+    q = S1.MonaghanGingoldViscosity(1.0, 1.0)
+    with pytest.raises(NotImplementedError):
+        S1.SPH(dataBase=db, W=WT, Q=q)                     # factory default XSPH=True
+    with pytest.raises(NotImplementedError):
+        S1.SPH(dataBase=db, W=WT, XSPH=False)              # default Q is LimitedMonaghanGingold
+    with pytest.raises(NotImplementedError):
+        S1.SPH(dataBase=db, W=WT, Q=q, XSPH=False, evolveTotalEnergy=True)
+    with pytest.raises(NotImplementedError):
+        S1.SPH(dataBase=db, W=WT, Q=q, XSPH=False, densityUpdate=S1.IntegrateDensity)
+    with pytest.raises(NotImplementedError):
+        S1.MonaghanGingoldViscosity(1.0, 1.0, linearInExpansion=True)
+    with pytest.raises(NotImplementedError):
+        S1.SpheralController(integ)                         # iterateInitialH=True default
+    with pytest.raises(NotImplementedError):
+        S1.Plane(S1.Vector(0.0), S1.Vector(0.5)).axis_side(1)
+
+
+def test_materials_must_agree_across_nodelists():
+    _, _, integ, lists = sod_script()
+    lists[1].eos = S1.GammaLawGas(5.0/3.0, 2.0)
+    with pytest.raises(NotImplementedError):
+        integ.config()
+
+
+def test_periodic_box_and_gravity_map_to_config():
+    cfg_ref, st = problems.taylor_green_3d(8)
+    WT = S3.TableKernel(S3.WendlandC4Kernel(), 1000)
+    nl = S3.makeFluidNodeList("fluid", S3.GammaLawGas(5.0/3.0, 2.0), numInternal=len(st["m"]), nPerh=2.0,
+                              kernelExtent=WT.kernelExtent)
+    db = S3.DataBase()
+    db.appendNodeList(nl)
+    hydro = S3.SPH(dataBase=db, W=WT, Q=S3.MonaghanGingoldViscosity(1.0, 1.0), XSPH=False)
+    for a in range(3):
+        lo = [0.0, 0.0, 0.0]
+        hi = [0.0, 0.0, 0.0]
+        hi[a] = 1.0
+        n = [0.0, 0.0, 0.0]
+        n[a] = 1.0
+        m = [-c for c in n]
+        hydro.appendBoundary(S3.PeriodicBoundary(S3.Plane(S3.Vector(*lo), S3.Vector(*n)),
+                                                 S3.Plane(S3.Vector(*hi), S3.Vector(*m))))
+    grav = S3.OctTreeGravity(G=1.0, softeningLength=0.01, opening=0.5, ftimestep=0.1)
+    integ = S3.CheapSynchronousRK2Integrator(db)
+    integ.appendPhysicsPackage(hydro)
+    integ.appendPhysicsPackage(grav)
+    cfg = integ.config()
+    assert list(cfg.bc_lo) == [abi.SPX_BC_PERIODIC]*3 and list(cfg.bc_hi) == [abi.SPX_BC_PERIODIC]*3
+    assert list(cfg.xmin) == [0.0]*3 and list(cfg.xmax) == [1.0]*3
+    assert cfg.kernel == abi.SPX_KERNEL_WENDLANDC4 and cfg.neighbor_extent == 1.0 and cfg.nperh == 2.0
+    assert cfg.gravity == 1 and cfg.G == 1.0 and cfg.softening == 0.01 and cfg.opening == 0.5
