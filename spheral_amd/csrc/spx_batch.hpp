@@ -490,10 +490,12 @@ class Batch {
       RunGrid& gr = hgrids[r];
       for (int a = 0; a < 3; ++a) { gr.origin[a] = 0.0; gr.inv_cell[a] = 0.0; gr.nc[a] = 1; }
       gr.cellBase = (int)base;
+      gr.hinvMin = 0.0;
       if (runs[r].state != RUN_ACTIVE) continue;
-      const double hmaxInv = ord_value(hext[r*7]);
-      if (!(hmaxInv > 0.0) || !std::isfinite(hmaxInv)) raise(SPX_ERR_NUMERIC, "non-positive or non-finite H on the device");
-      const double support = cfg.neighbor_extent/hmaxInv*(1.0 + 1.0e-9);
+      const double hinvMin = ord_value(hext[r*7]);
+      if (!(hinvMin > 0.0) || !std::isfinite(hinvMin)) raise(SPX_ERR_NUMERIC, "non-positive or non-finite H on the device");
+      gr.hinvMin = hinvMin;
+      const double support = cfg.neighbor_extent/hinvMin*(1.0 + 1.0e-9);
       int64_t total = 1;
       for (int a = 0; a < D; ++a) {
         const double lo = ord_value(hext[r*7 + 1 + 2*a]), hi = ord_value(hext[r*7 + 2 + 2*a]);
@@ -950,7 +952,7 @@ class Batch {
       const int64_t p = slotOf[i];
       std::vector<std::pair<int64_t, int32_t>> l;
       for (int k = 0; k < cnt[p]; ++k) {
-        const uint32_t e = ell[(size_t)k*ld + p];
+        const uint32_t e = ell[ell_at(k, (int)p, (size_t)ld)];
         l.push_back({oid[e & kIdxMask] & mask, (int32_t)(e >> kImgShift)});
       }
       std::sort(l.begin(), l.end());
@@ -971,7 +973,7 @@ class Batch {
     int64_t np = 0;
     for (int64_t p = 0; p < nActive; ++p)
       for (int k = 0; k < cnt[p]; ++k) {
-        const uint32_t e = ell[(size_t)k*ld + p];
+        const uint32_t e = ell[ell_at(k, (int)p, (size_t)ld)];
         if ((e >> kImgShift) != 0 || (int64_t)(e & kIdxMask) > p) ++np;
       }
     npairsCache = np;

@@ -3,9 +3,11 @@
 // Data layout in HBM: structure of arrays, one double array per scalar component,
 // particles in cell-Morton order (slot p holds original node oid[p]).  H is
 // isotropic (H = hinv * I, as SPH + IdealH keep it), so one double per node.
-// Neighbour lists are ELLPACK: entry k of slot i at nbr[k*ld + i] (ld >= n), a
-// uint32 holding the neighbour slot in the low 26 bits and a 6-bit image code
-// (2 bits per axis: 0 none, 1 beyond the low plane, 2 beyond the high plane).
+// Neighbour lists are a blocked ELLPACK: entries come in chunks of 8 per slot, chunk c of
+// slot i at nbr[(c*ld + i)*8 .. +8) (ld >= n), so a lane writes its list 32 bytes at a
+// time and a wavefront reading chunk c reads 2 KB contiguously.  Each entry is a uint32
+// holding the neighbour slot in the low 26 bits and a 6-bit image code (2 bits per axis:
+// 0 none, 1 beyond the low plane, 2 beyond the high plane).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,6 +18,12 @@ namespace spx {
 constexpr int kImgShift = 26;
 constexpr uint32_t kIdxMask = (1u << kImgShift) - 1u;
 constexpr int kBlock = 256;
+constexpr int kEllChunk = 8;
+
+// Position of neighbour entry k of slot i in the blocked ELLPACK array.
+__host__ __device__ __forceinline__ size_t ell_at(int k, int i, size_t ld) {
+  return (((size_t)(k >> 3)*ld + (size_t)i) << 3) + (size_t)(k & 7);
+}
 
 // The command processor hands workgroup b to XCD b % 8, and each XCD has its own L2.
 // Pair kernels map blocks so that each XCD works through one contiguous range of
@@ -63,6 +71,7 @@ struct Geo {
 struct RunGrid {
   double origin[3];
   double inv_cell[3];
+  double hinvMin;      // smallest h^-1 of the run's nodes: bounds every node's support radius
   int nc[3];
   int cellBase;
 };

@@ -160,6 +160,13 @@ __global__ void __launch_bounds__(kBlock) k_nodescale(Fields f, Phys ph, int n) 
   if (i < n) f.ns[i] = 1.0/max_eigen_iso<D>(f.hinv[i])/ph.nperh;
 }
 
+// Store the 8-entry chunk holding entry k of slot i (two 16-byte stores).
+__device__ __forceinline__ void flush_chunk(uint32_t* nbr, int k, int i, size_t ld, const uint32_t* c) {
+  uint4* dst = reinterpret_cast<uint4*>(nbr + ell_at(k & ~7, i, ld));
+  dst[0] = make_uint4(c[0], c[1], c[2], c[3]);
+  dst[1] = make_uint4(c[4], c[5], c[6], c[7]);
+}
+
 // ---------------------------------------------------------------------------
 // ConnectivityMap::computeConnectivity (Neighbor/ConnectivityMap.cc:905-947): j is a
 // neighbour of i iff |Hi rij|^2 <= ext^2 or |Hj rij|^2 <= ext^2, i != j (an image of i
@@ -182,9 +189,15 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
     ci[a] = cell_coord<D>(gr, a, xi[a]);
   }
   const double hi = f.hinv[i];
+  // Every neighbour lies within ext/min(hi, hj) <= ext/min(hi, hinvMin) of node i.  Farther
+  // candidates are rejected by an FP32 distance test first; its 1e-4 relative margin is far
+  // above the FP32 rounding of the test, so only the exact FP64 criterion decides membership.
+  const double hq = fmin(hi, gr.hinvMin);
+  const float reach2 = (float)(ph.ext2/(hq*hq)*(1.0 + 1.0e-4));
   const size_t ld = (size_t)f.ld;
   const int cap = f.cap;
   int count = 0;
+  uint32_t chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int r1 = D >= 2 ? 1 : 0, r2 = D >= 3 ? 1 : 0;
   for (int dz = -r2; dz <= r2; ++dz)
     for (int dy = -r1; dy <= r1; ++dy)
@@ -217,6 +230,10 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
 #pragma unroll
           for (int a = 0; a < D; ++a) xj[a] = f.x[a][j];
           if (code) image_pos<D>(g, code, xj);
+          float r2 = 0.0f;
+#pragma unroll
+          for (int a = 0; a < D; ++a) { const float d = (float)(xi[a] - xj[a]); r2 += d*d; }
+          if (r2 > reach2) continue;
           const double hj = f.hinv[j];
           double e2i = 0.0, e2j = 0.0;
 #pragma unroll
@@ -227,11 +244,17 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
             e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
           }
           if (e2i <= ph.ext2 || e2j <= ph.ext2) {
-            if (count < cap) f.nbr[(size_t)count*ld + i] = (uint32_t)j | ((uint32_t)code << kImgShift);
+            // write-combine 8 entries in registers, flush them as one 32-byte chunk
+            const uint32_t e = (uint32_t)j | ((uint32_t)code << kImgShift);
+            const int s = count & 7;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) chunk[u] = s == u ? e : chunk[u];
             ++count;
+            if (s == 7 && count <= cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
           }
         }
       }
+  if ((count & 7) && count <= cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
   f.cnt[i] = count;
   if (count > cap) atomicMax(overflow, count);
 }
@@ -264,7 +287,7 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
     double vmax2 = 0.0;       // pairwise limit, tracked as max of |v_ij|^2 / min(ns)^2
     const int nn = f.cnt[i];
     for (int k = 0; k < nn; ++k) {
-      const uint32_t e = f.nbr[(size_t)k*f.ld + i];
+      const uint32_t e = f.nbr[ell_at(k, i, (size_t)f.ld)];
       const int j = (int)(e & kIdxMask);
       const int code = (int)(e >> kImgShift);
       if (DENS) {
@@ -408,7 +431,7 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
   double wsum = 0.0, gsum = 0.0;
   const int nn = f.cnt[i];
   for (int k = 0; k < nn; ++k) {
-    const uint32_t e = f.nbr[(size_t)k*f.ld + i];
+    const uint32_t e = f.nbr[ell_at(k, i, (size_t)f.ld)];
     const int j = (int)(e & kIdxMask);
     const int code = (int)(e >> kImgShift);
     double xj[D];

@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
 
   const int nn = f.cnt[i];
   for (int k = 0; k < nn; ++k) {
-    const uint32_t ent = f.nbr[(size_t)k*ld + i];
+    const uint32_t ent = f.nbr[ell_at(k, i, ld)];
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
     Nbr<D, R> O;
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   double du = 0.0;
   const int nn = f.cnt[i];
   for (int k = 0; k < nn; ++k) {
-    const uint32_t ent = f.nbr[(size_t)k*ld + i];
+    const uint32_t ent = f.nbr[ell_at(k, i, ld)];
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
     const bool aFirst = code != 0 || i < j;
