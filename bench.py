@@ -148,6 +148,42 @@ def ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+def evrard(args):
+    """Evrard collapse with TreeGravity: args.particles particles to t = args.t_end (default
+    0.8) in mixed precision, then the same run in plain FP64; reports the mixed run's
+    particle-steps/s and its L1 distance from the FP64 run (density, specific energy)."""
+    t_end = args.t_end if args.t_end != 0.1 else 0.8
+    out = {}
+    for prec, name in ((abi.SPX_PREC_MIXED, "mixed"), (abi.SPX_PREC_FP64, "fp64")):
+        cfg, st = problems.evrard_3d(args.particles, precision=prec)
+        n = len(st["m"])
+        r = engine.make_run(cfg, st)
+        r.initialize()
+        t0 = time.perf_counter()
+        k, t = 0, 0.0
+        while t < t_end*(1.0 - 1e-15) and k < args.max_steps:
+            k1, t = r.advance(t_end, 50)
+            k += k1
+            print(f"[bench] evrard {name}: step {k} t {t:.4f}", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        out[name] = {"seconds": wall, "steps": k, "t": t, "particle_steps_per_s": n*k/wall,
+                     "fields": {f: r.field(f) for f in ("mass density", "specific thermal energy")}}
+        r.close()
+    m, d = out["mixed"], out["fp64"]
+    l1 = {f: float(np.abs(m["fields"][f] - d["fields"][f]).sum()/np.abs(d["fields"][f]).sum()) for f in m["fields"]}
+    line = {
+        "metric": "SPH + TreeGravity particle-steps/s",
+        "value": m["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": m["steps"], "warmup": 0,
+        "ms_per_step": 1000.0*m["seconds"]/max(m["steps"], 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 pair / f64 accumulate, f64 tree", "data": "synthetic",
+        "config": {"workload": f"evrard_3d_{args.particles}_to_t{t_end}", "particles": args.particles, "t_end": t_end,
+                   "opening": 0.5, "softening": 0.01, "kernel": "BSpline TableKernel(1000), nPerh 1.51"},
+        "t_reached": m["t"], "fp64_particle_steps_per_s": d["particle_steps_per_s"], "fp64_steps": d["steps"],
+        "l1_mixed_vs_fp64": l1,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,7 +194,9 @@ def main():
     ap.add_argument("--cpu-n1d", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["taylor_green", "ensemble"], default="taylor_green")
+    ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard"], default="taylor_green")
+    ap.add_argument("--particles", type=int, default=250_000, help="evrard: particle count")
+    ap.add_argument("--max-steps", type=int, default=100_000, help="evrard: step limit")
     ap.add_argument("--runs", type=int, default=256, help="ensemble: number of Kelvin-Helmholtz runs")
     ap.add_argument("--t-end", type=float, default=0.1, help="ensemble: end time of every run")
     ap.add_argument("--seed", type=int, default=7, help="ensemble: seed of the run sizes")
@@ -170,6 +208,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         ensemble(args)
+        return
+    if args.workload == "evrard":
+        evrard(args)
         return
     prec = abi.SPX_PREC_FP64 if args.precision == "fp64" else abi.SPX_PREC_MIXED
     cfg, st = problems.taylor_green_3d(args.n1d, precision=prec)
