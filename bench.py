@@ -184,6 +184,80 @@ def evrard(args):
     print(json.dumps(line), flush=True)
 
 
+SERVE_KINDS = ("sod2d", "sod3d", "gresho2d")
+SERVE_T_END = {"sod2d": 0.15, "sod3d": 0.15, "gresho2d": 1.0}
+
+
+def serve_problem(kind, n, precision):
+    if kind == "gresho2d":
+        return problems.gresho_2d(n, precision=precision)
+    return problems.sod_tube(2 if kind == "sod2d" else 3, n, precision=precision)
+
+
+def serve_trace(args):
+    """Served stream: args.requests requests with Poisson arrivals at args.rate/s (PCG64 seed 99),
+    kind uniform over {Sod 2-D, Sod 3-D, Gresho 2-D}, N log-uniform in [1000, 500000], fixed end
+    times (Sod 0.15, Gresho 1.0).  Leg 1: continuous batching in mixed precision (spheral_amd.serve,
+    one resident batch per kind, admission by particle budget and run slots).  Leg 2: the same trace
+    served one request at a time in arrival order as plain FP64 spx_runs (each request's service
+    time measured alone; queueing by the same arrival times)."""
+    from spheral_amd import serve
+    rng = np.random.Generator(np.random.PCG64(99))
+    R = args.requests
+    arrivals = np.cumsum(rng.exponential(1.0/args.rate, R))
+    kinds = [SERVE_KINDS[k] for k in rng.integers(0, 3, R)]
+    sizes = np.round(np.exp(rng.uniform(np.log(1000.0), np.log(500000.0), R))).astype(int)
+    reqs = [(k, serve_problem(k, int(n), abi.SPX_PREC_MIXED)[1], SERVE_T_END[k]) for k, n in zip(kinds, sizes)]
+    cfgs = {k: serve_problem(k, 4096, abi.SPX_PREC_MIXED)[0] for k in SERVE_KINDS}
+    print(f"[bench] serve: {R} requests over {arrivals[-1]:.2f} s, {sum(len(r[1]['m']) for r in reqs)} particles",
+          file=sys.stderr, flush=True)
+    srv = serve.Server({k: (cfgs[k], 6_000_000, 128) for k in SERVE_KINDS})
+    t0 = time.perf_counter()
+    handles = []
+    for (k, st, te), at in zip(reqs, arrivals):
+        wait = t0 + at - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        handles.append(srv.submit(k, st, te))
+    for i, h in enumerate(handles):
+        h.event.wait()
+        if h.error is not None:
+            raise h.error
+    makespan = time.perf_counter() - t0
+    srv.close()
+    tts = [h.time_to_solution for h in handles]
+    print(f"[bench] served: makespan {makespan:.2f} s, p50 {serve.percentile(tts, 50):.2f} s", file=sys.stderr, flush=True)
+    # leg 2: sequential FP64 in arrival order
+    cfgs64 = {k: serve_problem(k, 4096, abi.SPX_PREC_FP64)[0] for k in SERVE_KINDS}
+    service, l1 = [], {"mass density": 0.0, "specific thermal energy": 0.0}
+    for (k, st, te), h in zip(reqs, handles):
+        ts = time.perf_counter()
+        res, _ = serve.solve_alone(cfgs64[k], st, te)
+        service.append(time.perf_counter() - ts)
+        for f in l1:
+            l1[f] = max(l1[f], float(np.abs(h.result[f] - res[f]).sum()/np.abs(res[f]).sum()))
+        print(f"[bench] sequential {len(service)}/{R}", file=sys.stderr, flush=True)
+    end, tts_seq = 0.0, []
+    for at, sv in zip(arrivals, service):
+        end = max(end, at) + sv
+        tts_seq.append(end - at)
+    psteps = int(sum(h.steps*h.n for h in handles))
+    line = {
+        "metric": "served runs/hr and time-to-solution",
+        "value": R/makespan*3600.0, "unit": "runs/hr", "n_gpus": 1, "steps": R, "warmup": 0,
+        "ms_per_step": 1000.0*makespan/R, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair / f64 accumulate", "data": "synthetic",
+        "config": {"workload": f"serve_sod2d_sod3d_gresho2d_{R}req_poisson{args.rate}", "requests": R,
+                   "rate_per_s": args.rate, "seed": 99, "N_range": [1000, 500000], "t_end": SERVE_T_END},
+        "tts_p50_s": serve.percentile(tts, 50), "tts_p99_s": serve.percentile(tts, 99),
+        "makespan_s": makespan, "particle_steps_per_s": psteps/makespan,
+        "sequential_fp64_runs_per_hr": R/end*3600.0, "sequential_fp64_tts_p50_s": serve.percentile(tts_seq, 50),
+        "sequential_fp64_tts_p99_s": serve.percentile(tts_seq, 99), "speedup_runs_per_hr": end/makespan,
+        "max_l1_vs_fp64": l1,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +268,9 @@ def main():
     ap.add_argument("--cpu-n1d", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard"], default="taylor_green")
+    ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard", "serve"], default="taylor_green")
+    ap.add_argument("--requests", type=int, default=100, help="serve: number of requests in the trace")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--particles", type=int, default=250_000, help="evrard: particle count")
     ap.add_argument("--max-steps", type=int, default=100_000, help="evrard: step limit")
     ap.add_argument("--runs", type=int, default=256, help="ensemble: number of Kelvin-Helmholtz runs")
@@ -211,6 +287,9 @@ def main():
         return
     if args.workload == "evrard":
         evrard(args)
+        return
+    if args.workload == "serve":
+        serve_trace(args)
         return
     prec = abi.SPX_PREC_FP64 if args.precision == "fp64" else abi.SPX_PREC_MIXED
     cfg, st = problems.taylor_green_3d(args.n1d, precision=prec)

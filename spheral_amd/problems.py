@@ -155,3 +155,66 @@ def uniform_lattice(dim, n1d, nperh=2.0, kernel=abi.SPX_KERNEL_WENDLANDC4, rho0=
         abi.set_periodic(cfg, (0.0,) * 3, (1.0,) * 3)
     return cfg, dict(x=x, v=np.zeros((n, dim)), m=np.full(n, rho0 * dx ** dim), rho=np.full(n, rho0),
                      eps=np.full(n, P0 / ((gamma - 1.0) * rho0)), H=_iso_H(n, dim, 1.0 / (nperh * dx)))
+
+
+def _lattice(lo, hi, spacing):
+    """Cell-centred lattice filling the box [lo, hi) with the spacing rounded to fit."""
+    axes, vol = [], 1.0
+    for a, b in zip(lo, hi):
+        m = max(1, int(round((b - a) / spacing)))
+        d = (b - a) / m
+        axes.append(a + (np.arange(m) + 0.5) * d)
+        vol *= d
+    grids = np.meshgrid(*axes, indexing="ij")
+    return np.stack([g.ravel() for g in grids], 1), vol
+
+
+def sod_tube(dim, n_target, width=0.25, nperh=1.51, gamma=1.4, rho1=1.0, P1=1.0, rho2=0.125, P2=0.1,
+             kernel=abi.SPX_KERNEL_BSPLINE, precision=abi.SPX_PREC_MIXED):
+    """2-D or 3-D Sod tube: x in [-0.5, 0.5], cross-section [0, width]^(dim-1), reflecting walls on
+    every side; left rho1/P1, right rho2/P2 on lattices whose spacing ratio (rho1/rho2)^(1/dim) gives
+    (nearly) equal masses; about n_target particles."""
+    ratio = (rho1 / rho2) ** (1.0 / dim)
+    dx = (0.5 * width ** (dim - 1) * (1.0 + rho2 / rho1) / n_target) ** (1.0 / dim)
+    lo_t, hi_t = [0.0] * (dim - 1), [width] * (dim - 1)
+    xl, vl = _lattice([-0.5] + lo_t, [0.0] + hi_t, dx)
+    xr, vr = _lattice([0.0] + lo_t, [0.5] + hi_t, dx * ratio)
+    x = np.concatenate([xl, xr], 0)
+    n = x.shape[0]
+    left = np.arange(n) < len(xl)
+    rho = np.where(left, rho1, rho2)
+    vol = np.where(left, vl, vr)
+    P = np.where(left, P1, P2)
+    cfg = abi.default_config(dim, kernel=kernel, nperh=nperh, gamma=gamma, precision=precision)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    for a in range(dim):
+        cfg.bc_lo[a] = cfg.bc_hi[a] = abi.SPX_BC_REFLECTING
+        cfg.xmin[a], cfg.xmax[a] = (-0.5, 0.5) if a == 0 else (0.0, width)
+    cfg.last_dt = 1.0e-6
+    cfg.dt_min = 1.0e-8
+    return cfg, dict(x=x, v=np.zeros((n, dim)), m=rho * vol, rho=rho, eps=P / ((gamma - 1.0) * rho),
+                     H=_iso_H(n, dim, 1.0 / (nperh * vol ** (1.0 / dim))))
+
+
+def gresho_2d(n_target, nperh=1.51, gamma=5.0 / 3.0, kernel=abi.SPX_KERNEL_BSPLINE, precision=abi.SPX_PREC_MIXED):
+    """Gresho vortex on the periodic box [-0.5, 0.5]^2 (tests/functional/Hydro/GreshoVortex/
+    GreshoVortex.py:273-297): rho = 1, v_phi = 5r (r < 0.2), 2 - 5r (0.2 <= r < 0.4), 0 beyond,
+    pressure in radial balance (P = 5 + 12.5 r^2, 9 + 12.5 r^2 - 20 r + 4 ln 5r, 3 + 4 ln 2)."""
+    x, vol = _lattice([-0.5, -0.5], [0.5, 0.5], 1.0 / math.sqrt(n_target))
+    n = x.shape[0]
+    r = np.hypot(x[:, 0], x[:, 1])
+    r2 = r * r
+    inner, mid = r < 0.2, (r >= 0.2) & (r < 0.4)
+    vphi = np.where(inner, 5.0 * r, np.where(mid, 2.0 - 5.0 * r, 0.0))
+    P = np.where(inner, 5.0 + 12.5 * r2,
+                 np.where(mid, 9.0 + 12.5 * r2 - 20.0 * r + 4.0 * np.log(5.0 * np.maximum(r, 1e-300)),
+                          3.0 + 4.0 * math.log(2.0)))
+    rs = np.maximum(r, 1e-300)
+    v = np.stack([-vphi * x[:, 1] / rs, vphi * x[:, 0] / rs], 1)
+    cfg = abi.default_config(2, kernel=kernel, nperh=nperh, gamma=gamma, precision=precision)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    abi.set_periodic(cfg, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    return cfg, dict(x=x, v=v, m=np.full(n, vol), rho=np.ones(n), eps=P / (gamma - 1.0),
+                     H=_iso_H(n, 2, 1.0 / (nperh * math.sqrt(vol))))

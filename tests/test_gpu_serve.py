@@ -1,0 +1,34 @@
+"""Continuous batching: requests that arrive while others run are admitted into the resident
+batches, and every result equals the same request solved alone (bit-identical)."""
+import time
+
+import numpy as np
+import pytest
+
+from spheral_amd import abi, problems, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_alone(gpu):
+    print("[case] serve mixed trace", flush=True)
+    kinds = {"sod2d": lambda n: problems.sod_tube(2, n), "sod3d": lambda n: problems.sod_tube(3, n),
+             "gresho2d": lambda n: problems.gresho_2d(n)}
+    trace = [("sod2d", 900, 0.01), ("gresho2d", 1000, 0.004), ("sod3d", 1200, 0.006),
+             ("sod2d", 1500, 0.008), ("gresho2d", 700, 0.003), ("sod3d", 800, 0.004)]
+    cfgs = {k: f(1000)[0] for k, f in kinds.items()}
+    # small budgets so that admission has to wait for slots and particles to free up
+    srv = serve.Server({k: (cfgs[k], 2600, 2) for k in kinds})
+    handles = []
+    for k, n, te in trace:
+        handles.append((k, n, te, srv.submit(k, kinds[k](n)[1], te)))
+        time.sleep(0.02)
+    for *_, h in handles:
+        assert h.event.wait(120)
+        assert h.error is None
+    srv.close()
+    for k, n, te, h in handles:
+        ref, steps = serve.solve_alone(cfgs[k], kinds[k](n)[1], te)
+        assert h.steps == steps
+        for f in serve.RESULT_FIELDS:
+            assert np.array_equal(h.result[f], ref[f]), (k, n, f)
