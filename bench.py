@@ -258,6 +258,42 @@ def serve_trace(args):
     print(json.dumps(line), flush=True)
 
 
+def accuracy(args):
+    """Mixed-precision error next to the roundoff floor.  For each problem, three runs to the same
+    end time: plain FP64, FP64 with every initial position moved by one ulp, and mixed precision.
+    L1(perturbed FP64 - FP64) is what any roundoff-level difference (summation order, thread count,
+    compiler contraction) grows to over the run; L1(mixed - FP64) is read against it."""
+    cases = [
+        ("taylor_green_3d_32^3", lambda p: problems.taylor_green_3d(32, precision=p), 0.02),
+        ("kelvin_helmholtz_2d_16k", lambda p: problems.kelvin_helmholtz_2d(16384, precision=p), 0.1),
+        ("sod_tube_2d_20k", lambda p: problems.sod_tube(2, 20000, precision=p), 0.15),
+        ("gresho_2d_20k", lambda p: problems.gresho_2d(20000, precision=p), 1.0),
+        ("evrard_3d_20k", lambda p: problems.evrard_3d(20000, precision=p), 0.2),
+    ]
+    fields = ("mass density", "specific thermal energy")
+    rows = []
+    for name, make, t_end in cases:
+        out = {}
+        for label, prec, perturb in (("fp64", abi.SPX_PREC_FP64, False), ("fp64_ulp", abi.SPX_PREC_FP64, True),
+                                     ("mixed", abi.SPX_PREC_MIXED, False)):
+            cfg, st = make(prec)
+            if perturb:
+                st = dict(st)
+                st["x"] = np.nextafter(st["x"], np.inf)
+            r = engine.make_run(cfg, st)
+            r.initialize()
+            steps, t = r.advance(t_end, 10**7)
+            out[label] = {f: r.field(f) for f in fields}
+            out[label]["steps"] = steps
+            r.close()
+        ref = out["fp64"]
+        l1 = lambda a: {f: float(np.abs(a[f] - ref[f]).sum()/np.abs(ref[f]).sum()) for f in fields}
+        rows.append({"case": name, "t_end": t_end, "steps": ref["steps"], "l1_mixed_vs_fp64": l1(out["mixed"]),
+                     "l1_fp64_ulp_vs_fp64": l1(out["fp64_ulp"])})
+        print(f"[bench] accuracy {name}: {rows[-1]}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "mixed-precision L1 vs FP64 and the roundoff floor", "cases": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,7 +304,8 @@ def main():
     ap.add_argument("--cpu-n1d", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard", "serve"], default="taylor_green")
+    ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard", "serve", "accuracy"],
+                    default="taylor_green")
     ap.add_argument("--requests", type=int, default=100, help="serve: number of requests in the trace")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--particles", type=int, default=250_000, help="evrard: particle count")
@@ -290,6 +327,9 @@ def main():
         return
     if args.workload == "serve":
         serve_trace(args)
+        return
+    if args.workload == "accuracy":
+        accuracy(args)
         return
     prec = abi.SPX_PREC_FP64 if args.precision == "fp64" else abi.SPX_PREC_MIXED
     cfg, st = problems.taylor_green_3d(args.n1d, precision=prec)

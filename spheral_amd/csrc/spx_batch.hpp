@@ -260,7 +260,7 @@ class Batch {
     std::vector<double**> all;
     Fields& F = f;
     for (int a = 0; a < D; ++a)
-      for (double** p : {&F.x[a], &F.v[a], &F.xm[a], &F.vm[a], &F.v1[a], &F.DxDt[a], &F.DvDt[a], &F.DvDtG[a],
+      for (double** p : {&F.x[a], &F.v[a], &F.xm[a], &F.vm[a], &F.v1[a], &F.v12[a], &F.DxDt[a], &F.DvDt[a], &F.DvDtG[a],
                          &F.gradRho[a], &F.m1[a]})
         all.push_back(p);
     for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.rhom, &F.epsm, &F.hinvm, &F.Pm,
@@ -275,7 +275,8 @@ class Batch {
     F.DvDx = slab + off; off += (size_t)D*D*ld;
     F.Mcorr = slab + off;
     for (int a = D; a < 3; ++a)
-      F.x[a] = F.v[a] = F.xm[a] = F.vm[a] = F.v1[a] = F.DxDt[a] = F.DvDt[a] = F.DvDtG[a] = F.gradRho[a] = F.m1[a] = nullptr;
+      F.x[a] = F.v[a] = F.xm[a] = F.vm[a] = F.v1[a] = F.v12[a] = F.DxDt[a] = F.DvDt[a] = F.DvDtG[a] = F.gradRho[a] =
+          F.m1[a] = nullptr;
     alt = f;
     auto cl = carried(alt);
     const size_t altTotal = (cl.size() + (size_t)D*D)*(size_t)ld;
@@ -704,11 +705,14 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (dens && dtv)
-        launchPrec(k_density_dt<DD, true, true, double>, k_density_dt<DD, true, true, float>, nb, f, geo, ph, dtab, (int)nActive, dsc);
+        launchPrec(k_density_dt<DD, true, true, double, kGroup>, k_density_dt<DD, true, true, float, kGroup>, nb, f, geo, ph,
+                   dtab, (int)nActive, dsc);
       else if (dens)
-        launchPrec(k_density_dt<DD, true, false, double>, k_density_dt<DD, true, false, float>, nb, f, geo, ph, dtab, (int)nActive, dsc);
+        launchPrec(k_density_dt<DD, true, false, double, kGroup>, k_density_dt<DD, true, false, float, kGroup>, nb, f, geo,
+                   ph, dtab, (int)nActive, dsc);
       else
-        launchPrec(k_density_dt<DD, false, true, double>, k_density_dt<DD, false, true, float>, nb, f, geo, ph, dtab, (int)nActive, dsc);
+        launchPrec(k_density_dt<DD, false, true, double, kGroup>, k_density_dt<DD, false, true, float, kGroup>, nb, f, geo,
+                   ph, dtab, (int)nActive, dsc);
     });
   }
   void sphPass(bool full) {
@@ -839,7 +843,8 @@ class Batch {
     // full step from the start-of-step state, compatible energy first
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      launchPrec(k_full_update<DD, false, double>, k_full_update<DD, false, float>, nb, f, geo, ph,
+      hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)dsc, -1.0, (int)nActive);
+      launchPrec(k_full_update<DD, false, double, kGroup>, k_full_update<DD, false, float, kGroup>, nb, f, geo, ph,
                  (const Scalars*)dsc, 0.0, (int)nActive);
     });
     for (int a = 0; a < D; ++a) {
@@ -907,7 +912,9 @@ class Batch {
     requireConn();
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      launchPrec(k_full_update<DD, true, double>, k_full_update<DD, true, float>, (int)xcd_grid(nActive, kBlock), f, geo, ph,
+      hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)dsc,
+                         0.5*multiplier, (int)nActive);
+      launchPrec(k_full_update<DD, true, double, kGroup>, k_full_update<DD, true, float, kGroup>, (int)xcd_grid(nActive, kBlock), f, geo, ph,
                  (const Scalars*)dsc, multiplier, (int)nActive);
     });
     eos();

@@ -25,6 +25,32 @@ __host__ __device__ __forceinline__ size_t ell_at(int k, int i, size_t ld) {
   return (((size_t)(k >> 3)*ld + (size_t)i) << 3) + (size_t)(k & 7);
 }
 
+// Neighbour entries are consumed in groups of kGroup: one 16-byte load of the blocked list,
+// then every neighbour read of the group is issued before any is used, so a wavefront waits
+// on memory once per group instead of twice per neighbour.  Positions past the end of the list
+// (k0 + u >= nn) become the lane's own slot with no image, and are skipped by the caller.
+constexpr int kGroup = 4;
+template <int G>
+__device__ __forceinline__ void ell_group(const uint32_t* nbr, size_t ld, int k0, int i, int nn, int* js, int* codes) {
+  uint32_t e[G];
+  const uint32_t* p = nbr + ell_at(k0, i, ld);
+  if (G == 4) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    e[0] = q.x; e[1 % G] = q.y; e[2 % G] = q.z; e[3 % G] = q.w;
+  } else if (G == 2) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+    e[0] = q.x; e[1 % G] = q.y;
+  } else {
+    e[0] = p[0];
+  }
+#pragma unroll
+  for (int u = 0; u < G; ++u) {
+    const bool ok = k0 + u < nn;
+    js[u] = ok ? (int)(e[u] & kIdxMask) : i;
+    codes[u] = ok ? (int)(e[u] >> kImgShift) : 0;
+  }
+}
+
 // The command processor hands workgroup b to XCD b % 8, and each XCD has its own L2.
 // Pair kernels map blocks so that each XCD works through one contiguous range of
 // (Morton-ordered) particles and keeps its neighbours' data in its own L2.  Grids of
@@ -90,6 +116,8 @@ struct Fields {
   double *xm[3], *vm[3], *rhom, *epsm, *hinvm, *Pm, *csm, *omegam, *prhom;
   // end-of-step velocity (swapped with v after the full update)
   double *v1[3];
+  // half-step velocity v + a dt/2 of the compatible-energy partition
+  double *v12[3];
   // derivatives carried across steps
   double *DxDt[3], *DvDt[3], *DvDtG[3], *DepsDt, *DrhoDt, *DHDt, *DvDx, *maxQ;
   // derivatives consumed within a step

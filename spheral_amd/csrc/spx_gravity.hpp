@@ -180,48 +180,59 @@ __global__ void __launch_bounds__(kBlock) k_tree_gather(double* const* xs, const
 }
 
 // applyTreeForces for the point in tree slot p; results scattered to its hydro slot.
+// The wavefront walks the pre-order cells together: each lane keeps the next cell its own
+// walk would visit, the wavefront visits the smallest of them, and only the lanes waiting
+// on that cell act on it.  Every lane therefore takes exactly the decisions, in exactly the
+// order, of an independent stackless walk (same bits), while cell and leaf reads become
+// uniform across the wavefront instead of 64 scattered ones.
 __global__ void __launch_bounds__(kBlock) k_tree_walk(TreeView t, const uint32_t* perm, double soft2, double open2,
                                                       double soft, double G, const double* cs2, double* const* acc,
                                                       double* pot, double* blockMin) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  const bool valid = p < t.n;
   double dtv = 1.7976931348623157e308;
-  if (p < t.n) {
-    const double xi = t.tx[0][p], yi = t.tx[1][p], zi = t.tx[2][p];
-    double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
-    int c = 1;
-    while (c < t.ncells) {
-      const int lvl = t.level[c];
-      const double dx = t.cx[0][c] - xi, dy = t.cx[1][c] - yi, dz = t.cx[2][c] - zi;
-      double r2 = dx*dx + dy*dy + dz*dz;
-      if (r2 > cs2[lvl]/open2 + t.rc2[c]) {
-        const double mag = sqrt(r2);
-        const double inv = mag > 1.0e-50 ? 1.0/mag : 0.0;
-        const double nx = mag > 1.0e-50 ? dx*inv : 1.0, ny = dy*inv, nz = dz*inv;
-        const double rs = r2 + soft2;
-        const double Mc = t.M[c];
-        const double f = Mc*(1.0/rs);
-        ax += f*nx; ay += f*ny; az += f*nz;
-        phi -= Mc*(1.0/sqrt(rs));
-        c = t.skip[c];
-      } else if (t.first[c] == t.last[c] || lvl == kTreeBits) {
-        for (int q = t.first[c]; q <= t.last[c]; ++q) {
-          const double ex = t.tx[0][q] - xi, ey = t.tx[1][q] - yi, ez = t.tx[2][q] - zi;
-          const double q2 = ex*ex + ey*ey + ez*ez;
-          if (q2/soft2 > 1.0e-10) {
-            const double mag = sqrt(q2);
-            const double inv = 1.0/mag;
-            const double rs = q2 + soft2;
-            const double mq = t.tm[q];
-            const double f = mq*(1.0/rs);
-            ax += f*(ex*inv); ay += f*(ey*inv); az += f*(ez*inv);
-            phi -= mq*(1.0/sqrt(rs));
-          }
+  double xi = 0.0, yi = 0.0, zi = 0.0;
+  if (valid) { xi = t.tx[0][p]; yi = t.tx[1][p]; zi = t.tx[2][p]; }
+  double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
+  int next = valid ? 1 : t.ncells;
+  while (true) {
+    int c = next;
+    for (int off = 32; off > 0; off >>= 1) c = min(c, __shfl_xor(c, off));
+    if (c >= t.ncells) break;                     // every lane of the wavefront is done
+    if (next != c) continue;
+    const int lvl = t.level[c];
+    const double dx = t.cx[0][c] - xi, dy = t.cx[1][c] - yi, dz = t.cx[2][c] - zi;
+    const double r2 = dx*dx + dy*dy + dz*dz;
+    if (r2 > cs2[lvl]/open2 + t.rc2[c]) {
+      const double mag = sqrt(r2);
+      const double inv = mag > 1.0e-50 ? 1.0/mag : 0.0;
+      const double nx = mag > 1.0e-50 ? dx*inv : 1.0, ny = dy*inv, nz = dz*inv;
+      const double rs = r2 + soft2;
+      const double Mc = t.M[c];
+      const double f = Mc*(1.0/rs);
+      ax += f*nx; ay += f*ny; az += f*nz;
+      phi -= Mc*(1.0/sqrt(rs));
+      next = t.skip[c];
+    } else if (t.first[c] == t.last[c] || lvl == kTreeBits) {
+      for (int q = t.first[c]; q <= t.last[c]; ++q) {
+        const double ex = t.tx[0][q] - xi, ey = t.tx[1][q] - yi, ez = t.tx[2][q] - zi;
+        const double q2 = ex*ex + ey*ey + ez*ez;
+        if (q2/soft2 > 1.0e-10) {
+          const double mag = sqrt(q2);
+          const double inv = 1.0/mag;
+          const double rs = q2 + soft2;
+          const double mq = t.tm[q];
+          const double f = mq*(1.0/rs);
+          ax += f*(ex*inv); ay += f*(ey*inv); az += f*(ez*inv);
+          phi -= mq*(1.0/sqrt(rs));
         }
-        c = t.skip[c];
-      } else {
-        c = c + 1;
       }
+      next = t.skip[c];
+    } else {
+      next = c + 1;
     }
+  }
+  if (valid) {
     const double amag = sqrt(ax*ax + ay*ay + az*az);
     if (amag > 0.0) dtv = sqrt(soft/amag);
     const uint32_t s = perm[p];

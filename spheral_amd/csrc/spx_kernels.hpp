@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
 // vote (Physics/GenericHydro.cc:112-381): per-node limits on the new density and the
 // pairwise velocity-difference limit, min-reduced per run.
 // ---------------------------------------------------------------------------
-template <int D, bool DENS, bool DT, typename R>
+template <int D, bool DENS, bool DT, typename R, int G>
 __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   double dtmin = 1.7976931348623157e308;
@@ -286,34 +286,50 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
     }
     double vmax2 = 0.0;       // pairwise limit, tracked as max of |v_ij|^2 / min(ns)^2
     const int nn = f.cnt[i];
-    for (int k = 0; k < nn; ++k) {
-      const uint32_t e = f.nbr[ell_at(k, i, (size_t)f.ld)];
-      const int j = (int)(e & kIdxMask);
-      const int code = (int)(e >> kImgShift);
-      if (DENS) {
-        double xj[D];
+    for (int k0 = 0; k0 < nn; k0 += G) {
+      int js[G], codes[G];
+      ell_group<G>(f.nbr, (size_t)f.ld, k0, i, nn, js, codes);
+      double xj[G][D], vj[G][D], hj[G], mj[G], nsj[G];
 #pragma unroll
-        for (int a = 0; a < D; ++a) xj[a] = f.x[a][j];
-        if (code) image_pos<D>(g, code, xj);
-        const double hj = f.hinv[j];
-        R s = (R)0;
-        if (sizeof(R) == sizeof(double)) {
+      for (int u = 0; u < G; ++u) {
+        const int j = js[u];
+        if (DENS) {
 #pragma unroll
-          for (int a = 0; a < D; ++a) { const double d = hj*xi[a] - hj*xj[a]; s += (R)(d*d); }
-        } else {
-#pragma unroll
-          for (int a = 0; a < D; ++a) { const R d = (R)hj*(R)(xi[a] - xj[a]); s += d*d; }
+          for (int a = 0; a < D; ++a) xj[u][a] = f.x[a][j];
+          hj[u] = f.hinv[j];
+          mj[u] = f.m[j];
         }
-        R w, gw;
-        kernel_wgw<R>(tab, sqrt(s), hdet_iso<D, R>((R)hj), w, gw);
-        rho += f.m[j]*(double)w;
-      }
-      if (DT) {
-        double s = 0.0;
+        if (DT) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) { const double d = vi[a] - refl_sign<D>(g, code, a)*f.v[a][j]; s += d*d; }
-        const double nsm = fmin(nsi, f.ns[j]);
-        vmax2 = fmax(vmax2, s/(nsm*nsm));
+          for (int a = 0; a < D; ++a) vj[u][a] = f.v[a][j];
+          nsj[u] = f.ns[j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (k0 + u >= nn) continue;
+        const int code = codes[u];
+        if (DENS) {
+          if (code) image_pos<D>(g, code, xj[u]);
+          R s = (R)0;
+          if (sizeof(R) == sizeof(double)) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) { const double d = hj[u]*xi[a] - hj[u]*xj[u][a]; s += (R)(d*d); }
+          } else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) { const R d = (R)hj[u]*(R)(xi[a] - xj[u][a]); s += d*d; }
+          }
+          R w, gw;
+          kernel_wgw<R>(tab, sqrt(s), hdet_iso<D, R>((R)hj[u]), w, gw);
+          rho += mj[u]*(double)w;
+        }
+        if (DT) {
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < D; ++a) { const double d = vi[a] - refl_sign<D>(g, code, a)*vj[u][a]; s += d*d; }
+          const double nsm = fmin(nsi, nsj[u]);
+          vmax2 = fmax(vmax2, s/(nsm*nsm));
+        }
       }
     }
     if (DENS) f.rho[i] = rho; else rho = f.rho[i];
@@ -417,7 +433,7 @@ __global__ void __launch_bounds__(kBlock) k_eos(Fields f, Phys ph, int n) {
 // computeSPHOmegaGradhCorrection (SPH/computeSPHOmegaGradhCorrection.cc:136-204), gather form.
 // runMask (optional): only nodes of runs with a non-zero entry are updated.
 // ---------------------------------------------------------------------------
-template <int D, typename R>
+template <int D, typename R, int G = kGroup>
 __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, double* const* xs,
                                                   const double* hs, double* omega, const int* runMask) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
@@ -430,22 +446,28 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
   const R hdeti = hdet_iso<D, R>(hi);
   double wsum = 0.0, gsum = 0.0;
   const int nn = f.cnt[i];
-  for (int k = 0; k < nn; ++k) {
-    const uint32_t e = f.nbr[ell_at(k, i, (size_t)f.ld)];
-    const int j = (int)(e & kIdxMask);
-    const int code = (int)(e >> kImgShift);
-    double xj[D];
+  for (int k0 = 0; k0 < nn; k0 += G) {
+    int js[G], codes[G];
+    ell_group<G>(f.nbr, (size_t)f.ld, k0, i, nn, js, codes);
+    double xj[G][D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) xj[a] = xs[a][j];
-    if (code) image_pos<D>(g, code, xj);
-    R s = (R)0;
+    for (int u = 0; u < G; ++u)
 #pragma unroll
-    for (int a = 0; a < D; ++a) { const R rr = hi*(R)(xi[a] - xj[a]); s += rr*rr; }
-    const R eta = sqrt(s);
-    R w, gw;
-    kernel_wgw<R>(tab, eta, hdeti, w, gw);
-    wsum += (double)w;
-    gsum += (double)(eta*gw);
+      for (int a = 0; a < D; ++a) xj[u][a] = xs[a][js[u]];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (k0 + u < nn) {
+        if (codes[u]) image_pos<D>(g, codes[u], xj[u]);
+        R s = (R)0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) { const R rr = hi*(R)(xi[a] - xj[u][a]); s += rr*rr; }
+        const R eta = sqrt(s);
+        R w, gw;
+        kernel_wgw<R>(tab, eta, hdeti, w, gw);
+        wsum += (double)w;
+        gsum += (double)(eta*gw);
+      }
+    }
   }
   double om;
   if (nn == 0) {

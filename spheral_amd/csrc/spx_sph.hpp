@@ -8,26 +8,42 @@ namespace spx {
 // Neighbour record at the midpoint state, image-transformed.
 template <int D, typename R>
 struct Nbr {
-  double x[D];           // positions stay double: differences are formed before rounding to R
-  R v[D];
+  double x[D], v[D];     // positions and velocities stay double: differences are formed before rounding to R
   R m, rho, cs, h, prho;
 };
 
-template <int D, typename R>
-__device__ __forceinline__ void load_mid(const Fields& f, const Geo& g, int j, int code, Nbr<D, R>& q) {
+// A neighbour's midpoint record as loaded (no image transform yet), so the loads of the
+// next neighbour can be issued before the current pair is evaluated.
+template <int D>
+struct NbrRaw {
+  double x[D], v[D], m, rho, cs, h, prho;
+};
+template <int D>
+__device__ __forceinline__ void load_raw(const Fields& f, int j, NbrRaw<D>& q) {
 #pragma unroll
-  for (int a = 0; a < D; ++a) { q.x[a] = f.xm[a][j]; q.v[a] = (R)(refl_sign<D>(g, code, a)*f.vm[a][j]); }
+  for (int a = 0; a < D; ++a) { q.x[a] = f.xm[a][j]; q.v[a] = f.vm[a][j]; }
+  q.m = f.m[j];
+  q.rho = f.rhom[j];
+  q.cs = f.csm[j];
+  q.h = f.hinvm[j];
+  q.prho = f.prhom[j];
+}
+// Image-transformed neighbour record in the pair arithmetic type.
+template <int D, typename R>
+__device__ __forceinline__ void finish_mid(const Geo& g, int code, const NbrRaw<D>& r, Nbr<D, R>& q) {
+#pragma unroll
+  for (int a = 0; a < D; ++a) { q.x[a] = r.x[a]; q.v[a] = refl_sign<D>(g, code, a)*r.v[a]; }
   if (code) image_pos<D>(g, code, q.x);
-  q.m = (R)f.m[j];
-  q.rho = (R)f.rhom[j];
-  q.cs = (R)f.csm[j];
-  q.h = (R)f.hinvm[j];
-  q.prho = (R)f.prhom[j];
+  q.m = (R)r.m;
+  q.rho = (R)r.rho;
+  q.cs = (R)r.cs;
+  q.h = (R)r.h;
+  q.prho = (R)r.prho;
 }
 template <int D, typename R>
 __device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D, R>& s) {
 #pragma unroll
-  for (int a = 0; a < D; ++a) { s.x[a] = f.xm[a][i]; s.v[a] = (R)f.vm[a][i]; }
+  for (int a = 0; a < D; ++a) { s.x[a] = f.xm[a][i]; s.v[a] = f.vm[a][i]; }
   s.m = (R)f.m[i];
   s.rho = (R)f.rhom[i];
   s.cs = (R)f.csm[i];
@@ -59,7 +75,7 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
     eo[c] = O.h*e.r[c];
     e2s += e.es[c]*e.es[c];
     e2o += eo[c]*eo[c];
-    e.vso[c] = S.v[c] - O.v[c];
+    e.vso[c] = (R)(S.v[c] - O.v[c]);
   }
   e.etaMagS = sqrt(e2s);
   const R etaMagO = sqrt(e2o);
@@ -105,7 +121,7 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
 // normalisation, effective Q, density gradient, first moment, M).
 // ---------------------------------------------------------------------------
 template <int D, bool FULL, typename R>
-__global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n) {
+__global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
@@ -122,17 +138,40 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
   for (int a = 0; a < D*D; ++a) { dvdx[a] = 0.0; Mc[a] = 0.0; }
 
   const int nn = f.cnt[i];
+  // the ELL chunk (8 entries) and the next neighbour's record are fetched one pair ahead
+  uint4 c0 = make_uint4(0u, 0u, 0u, 0u), c1 = c0;
+  uint32_t en = 0;
+  NbrRaw<D> rn;
+  if (nn > 0) {
+    const uint4* cp = reinterpret_cast<const uint4*>(f.nbr + ell_at(0, i, ld));
+    c0 = cp[0];
+    c1 = cp[1];
+    en = c0.x;
+    load_raw<D>(f, (int)(en & kIdxMask), rn);
+  }
   for (int k = 0; k < nn; ++k) {
-    const uint32_t ent = f.nbr[ell_at(k, i, ld)];
+    const uint32_t ent = en;
+    const NbrRaw<D> rc = rn;
+    if (k + 1 < nn) {
+      const int k1 = k + 1, u = k1 & 7;
+      if (u == 0) {
+        const uint4* cp = reinterpret_cast<const uint4*>(f.nbr + ell_at(k1, i, ld));
+        c0 = cp[0];
+        c1 = cp[1];
+      }
+      en = u == 0 ? c0.x : u == 1 ? c0.y : u == 2 ? c0.z : u == 3 ? c0.w
+         : u == 4 ? c1.x : u == 5 ? c1.y : u == 6 ? c1.z : c1.w;
+      load_raw<D>(f, (int)(en & kIdxMask), rn);
+    }
     const int j = (int)(ent & kIdxMask);
     const int code = (int)(ent >> kImgShift);
     Nbr<D, R> O;
-    load_mid<D, R>(f, g, j, code, O);
+    finish_mid<D, R>(g, code, rc, O);
     PairS<D, R> e;
     pair_eval<D, R>(tab, ph, S, O, code != 0 || i < j, e);
 #pragma unroll
     for (int c = 0; c < D; ++c) pdd[((size_t)c*cap + k)*ld + i] = e.dd[c];
-    const double mo = f.m[j];
+    const double mo = rc.m;
     R vdg = (R)0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -149,7 +188,7 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
     const R wsph = kernel_wsph<R>(tab, e.etaMagS);
     m0 += (double)wsph;
     if (FULL) {
-      const double orho = f.rhom[j];
+      const double orho = rc.rho;
       rhosum += mo*(double)e.Ws;
       nrm += Sm/Srho*(double)e.Ws;
       effq += mo*(double)e.Qs*(double)e.Ws/orho;
@@ -240,48 +279,54 @@ __global__ void __launch_bounds__(kBlock) k_sph(Fields f, Geo g, Phys ph, DevTab
 // increment and replace policies and the EOS.  v0 is read from f.v, the new velocity
 // goes to f.v1.  ENERGY_ONLY: the package-level policy call (eps only).
 // ---------------------------------------------------------------------------
-template <int D, bool ENERGY_ONLY, typename R>
+template <int D, bool ENERGY_ONLY, typename R, int G>
 __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, const Scalars* sc, double multiplier,
                                                         int n) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dt = ENERGY_ONLY ? multiplier : sc[f.run[i]].dt;
-  const double hdt = 0.5*dt;
   const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
-  const R* pdd = (const R*)f.pdd;
-  const R ms = (R)f.m[i];
-  R v12s[D];
+  const R* pdd = (const R*)f.pdd;       // pair terms in the pair arithmetic; the partition is FP64
+  const double ms = f.m[i];
+  double v12s[D];
 #pragma unroll
-  for (int a = 0; a < D; ++a) v12s[a] = (R)(f.v[a][i] + f.DvDt[a][i]*hdt);
-  const R de0s = (R)f.DepsDt[i];
-  const R epsm = sizeof(R) == sizeof(double) ? (R)2.220446049250313e-16 : (R)1.1920929e-07f;
+  for (int a = 0; a < D; ++a) v12s[a] = f.v12[a][i];
+  const double de0s = f.DepsDt[i];
+  const double epsm = 2.220446049250313e-16;
   double du = 0.0;
   const int nn = f.cnt[i];
-  for (int k = 0; k < nn; ++k) {
-    const uint32_t ent = f.nbr[ell_at(k, i, ld)];
-    const int j = (int)(ent & kIdxMask);
-    const int code = (int)(ent >> kImgShift);
-    const bool aFirst = code != 0 || i < j;
-    // canonical: pacc_ab = (-m_b) dd_ab ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
-    const R mo = (R)f.m[j];
-    const R mb = aFirst ? mo : ms, ma = aFirst ? ms : mo;
-    R v12o[D];
-    if (D >= 1) { const double sg = refl_sign<D>(g, code, 0); v12o[0] = (R)(sg*f.v[0][j] + (sg*f.DvDt[0][j])*hdt); }
-    if (D >= 2) { const double sg = refl_sign<D>(g, code, 1); v12o[1] = (R)(sg*f.v[1][j] + (sg*f.DvDt[1][j])*hdt); }
-    if (D >= 3) { const double sg = refl_sign<D>(g, code, 2); v12o[2] = (R)(sg*f.v[2][j] + (sg*f.DvDt[2][j])*hdt); }
-    R duab = (R)0;
+  for (int k0 = 0; k0 < nn; k0 += G) {
+    int js[G], codes[G];
+    ell_group<G>(f.nbr, ld, k0, i, nn, js, codes);
+    double vo[G][D], mo[G], deo[G], dd[G][D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      const R dd = pdd[((size_t)c*cap + k)*ld + i];
-      const R va = aFirst ? v12s[c] : v12o[c], vb = aFirst ? v12o[c] : v12s[c];
-      const R ddc = aFirst ? dd : -dd;
-      duab += (vb - va)*((-mb)*ddc);
+    for (int u = 0; u < G; ++u) {
+      const int j = js[u];
+#pragma unroll
+      for (int a = 0; a < D; ++a) { vo[u][a] = f.v12[a][j]; dd[u][a] = (double)pdd[((size_t)a*cap + k0 + u)*ld + i]; }
+      mo[u] = f.m[j];
+      deo[u] = f.DepsDt[j];
     }
-    const R de0o = (R)f.DepsDt[j];
-    const R sgn = duab < (R)0 ? (R)-1 : (R)1;
-    const R wa = fmax(epsm, (aFirst ? de0s : de0o)*sgn), wb = fmax(epsm, (aFirst ? de0o : de0s)*sgn);
-    const R w = wa/(wa + wb);
-    du += (double)(aFirst ? w*duab : ((R)1 - w)*duab*ma/mb);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (k0 + u >= nn) continue;
+      const int j = js[u], code = codes[u];
+      const bool aFirst = code != 0 || i < j;
+      // canonical: pacc_ab = (-m_b) dd_ab ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
+      const double mb = aFirst ? mo[u] : ms, ma = aFirst ? ms : mo[u];
+      double duab = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const double v12o = refl_sign<D>(g, code, c)*vo[u][c];
+        const double va = aFirst ? v12s[c] : v12o, vb = aFirst ? v12o : v12s[c];
+        const double ddc = aFirst ? dd[u][c] : -dd[u][c];
+        duab += (vb - va)*((-mb)*ddc);
+      }
+      const double sgn = duab < 0.0 ? -1.0 : 1.0;
+      const double wa = fmax(epsm, (aFirst ? de0s : deo[u])*sgn), wb = fmax(epsm, (aFirst ? deo[u] : de0s)*sgn);
+      const double w = wa/(wa + wb);
+      du += aFirst ? w*duab : (1.0 - w)*duab*ma/mb;
+    }
   }
   const double eps1 = f.eps[i] + du*dt;
   f.eps[i] = eps1;
@@ -298,6 +343,17 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   f.hinv[i] = fmin(ph.hminInv, fmax(ph.hmaxInv, f.Hideal[i]));
   f.P[i] = ph.g1*rho1*eps1;
   f.cs[i] = sqrt(fmax(0.0, ph.gamma*ph.g1*eps1));
+}
+
+// Half-step velocity of the compatible-energy partition, v^{1/2} = v + a dt/2 with the
+// midpoint hydro acceleration (SpecificThermalEnergyPolicy.cc:97-99).  hdt < 0: the run's dt/2.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_v12(Fields f, const Scalars* sc, double hdtFixed, int n) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hdt = hdtFixed < 0.0 ? 0.5*sc[f.run[i]].dt : hdtFixed;
+#pragma unroll
+  for (int a = 0; a < D; ++a) f.v12[a][i] = f.v[a][i] + f.DvDt[a][i]*hdt;
 }
 
 // Per-node pressure term Prho = safeInv(omega)*P/rho^2 of the midpoint state (SPH.cc:425-426).
