@@ -206,6 +206,12 @@ int spx_run_kernel_times(spx_run* run, double* ms, int32_t max_entries,
                          int32_t* n_entries, char* names, int64_t names_cap);
 /* Unique pairs of the current connectivity (NodePairList size). */
 int spx_run_num_pairs(spx_run* run, int64_t* npairs);
+/* Candidate-list builds of the run so far (no counterpart in the reference, which rebuilds
+   its ConnectivityMap from scratch every step: a measurement aid).  The neighbour sets are
+   the reference's every step either way; a build happens at start-up, after
+   spx_run_update_connectivity, and when the run's nodes have moved or grown enough that
+   the candidate list could miss a pair. */
+int spx_run_neighbor_builds(spx_run* run, int64_t* builds);
 /* HBM copy probe (no counterpart in the reference: a measurement aid).  Copies `bytes`
    between two device buffers `reps` times with a plain streaming kernel and returns the
    achieved rate, read + write bytes per second, in GB/s. */
@@ -251,6 +257,8 @@ int spx_batch_get_field(spx_batch* batch, int32_t run, const char* name, double*
 int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7);
 /* Occupancy for admission control: particles resident and run slots in use. */
 int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs);
+/* As spx_run_neighbor_builds for one run of a batch. */
+int spx_batch_neighbor_builds(spx_batch* batch, int32_t run, int64_t* builds);
 /* As spx_run_kernel_times for the batch's last step (max_entries < 0 enables timing). */
 int spx_batch_kernel_times(spx_batch* batch, double* ms, int32_t max_entries,
                            int32_t* n_entries, char* names, int64_t names_cap);

@@ -11,6 +11,12 @@
 //
 // Particle order in the arrays: the active runs first, sorted by (run, cell Morton
 // code), then the finished runs, then removed runs until they are squeezed out.
+//
+// Neighbour lists are filtered every step from per-run candidate lists (k_candidates,
+// k_neighbor_filter).  A run re-sorts and rebuilds its candidates only when its own nodes
+// have moved or grown enough that the list might miss a pair (k_skin_check), so the
+// rebuild schedule of a run, and with it every bit of its results, does not depend on
+// the other runs of the batch.
 #pragma once
 
 #include <rocprim/rocprim.hpp>
@@ -35,6 +41,26 @@ static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1)/kBlock); }
 // Particles per block of the extent reduction.
 constexpr int64_t kExtentTile = 8192;
 
+// Widening of the candidate criterion (ext (1 + skin)) of a run's first candidate list,
+// and the range the per-run choice stays in.
+constexpr double kSkinInit = 0.1, kSkinMin = 0.03, kSkinMax = 0.5;
+
+// Skin that minimises the modelled per-step cost of a run whose nodes use up the skin at
+// `rate` per step: filtering (1 + s)^D candidates every step, plus a sort and candidate
+// search (fixed part + (1 + s)^D part, about 1 and 5 filters' worth) every s/rate steps.
+static inline double choose_skin(double rate, int D) {
+  double best = kSkinInit, bestCost = 1e300;
+  for (double s = kSkinMin; s <= kSkinMax + 1e-12; s += 0.01) {
+    const double vol = std::pow(1.0 + s, D);
+    const double cost = vol + (1.0 + 5.0*vol)*rate/s;
+    if (cost < bestCost) { bestCost = cost; best = s; }
+  }
+  return best;
+}
+
+// Value of an order-preserving bit key written by a max-reduction (0: nothing measured).
+static inline double skin_value(unsigned long long b) { return b == 0ull ? 0.0 : ord_value(b); }
+
 // A run has reached its end time (a clamped last step may land an ulp short).
 static inline bool reached_end(double t, double tEnd) { return t >= tEnd || tEnd - t <= 1.0e-15*std::fabs(tEnd); }
 
@@ -55,6 +81,10 @@ class Batch {
     bool needInit = false;
     bool stepOut = false;   // "new H", "mass zeroth moment" of this run's last evaluation are in place
     bool fullOut = false;   // the evaluate-only outputs (sum density, M, ...) are in place
+    bool candValid = false; // the run's candidate list covers its nodes (see k_skin_check)
+    double skin = kSkinInit;
+    long long builtCycle = 0;   // the run's step count when its candidate list was built
+    int64_t builds = 0;         // candidate-list builds of the run
   };
 
   spx_config cfg;
@@ -80,6 +110,11 @@ class Batch {
   int* dstate = nullptr;
   int* dinit = nullptr;
   int* dOverflow = nullptr;
+  int* dbuild = nullptr;      // per run: 1 if it builds its candidate list this step
+  int* ddelta = nullptr;      // per run: shift of its segment in a reorder (k_remap_candidates)
+  uint32_t* candAlt = nullptr;
+  int32_t* ccntAlt = nullptr;
+  int candAltCap = 0;
   double** xsMid = nullptr;
   double** xsState = nullptr;
   uint64_t *keys = nullptr, *keysAlt = nullptr;
@@ -102,12 +137,16 @@ class Batch {
   int* hstate = nullptr;
   int* hinit = nullptr;
   int* hOverflow = nullptr;
+  int* hbuild = nullptr;
+  int* hdelta = nullptr;
   double** hptrs = nullptr;
   int ld = 0;
   int mortonBits = 0, runBits = 0;
   int64_t nActive = 0;    // particles of active runs (front of the arrays after a sort)
   int64_t nTotal = 0;     // particles resident (active, done, removed-but-not-yet-dropped)
   bool initialized = false, connValid = false, wrapPending = false;
+  bool orderDirty = false;  // a run was added, finished or removed since the last reorder
+  int64_t candBuilds = 0;   // candidate-list builds, summed over runs
   int64_t npairsCache = -1;
   bool mixed = false;
   // per-class kernel time of the last step (ms)
@@ -138,7 +177,7 @@ class Batch {
     for (auto e : evs) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow, (void*)hptrs,
-                    (void*)htiles})
+                    (void*)htiles, (void*)hbuild, (void*)hdelta})
       if (p) (void)hipHostFree(p);
     grav.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -253,6 +292,8 @@ class Batch {
     for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.pot})
       v.push_back(p);
     for (int a = 0; a < D; ++a) { v.push_back(&F.DxDt[a]); v.push_back(&F.DvDt[a]); v.push_back(&F.DvDtG[a]); }
+    for (int a = 0; a < D; ++a) v.push_back(&F.xb[a]);
+    v.push_back(&F.hb);
     return v;
   }
 
@@ -261,11 +302,11 @@ class Batch {
     Fields& F = f;
     for (int a = 0; a < D; ++a)
       for (double** p : {&F.x[a], &F.v[a], &F.xm[a], &F.vm[a], &F.v1[a], &F.v12[a], &F.DxDt[a], &F.DvDt[a], &F.DvDtG[a],
-                         &F.gradRho[a], &F.m1[a]})
+                         &F.gradRho[a], &F.m1[a], &F.xb[a]})
         all.push_back(p);
     for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.rhom, &F.epsm, &F.hinvm, &F.Pm,
                        &F.csm, &F.omegam, &F.prhom, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.Hideal, &F.m0,
-                       &F.rhoSum, &F.norm, &F.effQ, &F.ns, &F.pot})
+                       &F.rhoSum, &F.norm, &F.effQ, &F.ns, &F.pot, &F.hb})
       all.push_back(p);
     const size_t total = (all.size() + 2*(size_t)D*D)*(size_t)ld;   // + DvDx, Mcorr
     double* slab = dalloc<double>(total);
@@ -276,7 +317,7 @@ class Batch {
     F.Mcorr = slab + off;
     for (int a = D; a < 3; ++a)
       F.x[a] = F.v[a] = F.xm[a] = F.vm[a] = F.v1[a] = F.v12[a] = F.DxDt[a] = F.DvDt[a] = F.DvDtG[a] = F.gradRho[a] =
-          F.m1[a] = nullptr;
+          F.m1[a] = F.xb[a] = nullptr;
     alt = f;
     auto cl = carried(alt);
     const size_t altTotal = (cl.size() + (size_t)D*D)*(size_t)ld;
@@ -285,6 +326,7 @@ class Batch {
     off = 0;
     for (double** p : cl) { *p = slabAlt + off; off += ld; }
     alt.DvDx = slabAlt + off;
+    f.pk = dalloc<double>((size_t)8*ld);
     f.oid = dalloc<int64_t>(ld);
     oidAlt = dalloc<int64_t>(ld);
     f.run = dalloc<int32_t>(ld);
@@ -294,6 +336,10 @@ class Batch {
     f.ld = ld;
     f.cap = 0;
     f.nbr = nullptr;
+    f.ccap = 0;
+    f.cand = nullptr;
+    f.ccnt = dalloc<int32_t>(ld);
+    HIP_CHECK(hipMemsetAsync(f.ccnt, 0, ld*sizeof(int32_t), stream));
     keys = dalloc<uint64_t>(ld);
     keysAlt = dalloc<uint64_t>(ld);
     vals = dalloc<uint32_t>(ld);
@@ -308,6 +354,8 @@ class Batch {
     dstate = dalloc<int>(maxRuns);
     dinit = dalloc<int>(maxRuns);
     dOverflow = dalloc<int>(1);
+    dbuild = dalloc<int>(maxRuns);
+    ddelta = dalloc<int>(maxRuns);
     xsMid = dalloc<double*>(3);
     xsState = dalloc<double*>(3);
     hgrids = halloc<RunGrid>(maxRuns);
@@ -316,12 +364,46 @@ class Batch {
     hstate = halloc<int>(maxRuns);
     hinit = halloc<int>(maxRuns);
     hOverflow = halloc<int>(1);
+    hbuild = halloc<int>(maxRuns);
+    hdelta = halloc<int>(maxRuns);
     hptrs = halloc<double*>(6);
     tilesAlloc = cap/kExtentTile + maxRuns + 1;
     dtiles = dalloc<int>(3*tilesAlloc);
     htiles = halloc<int>(3*tilesAlloc);
     upload(dsc, hsc, sizeof(Scalars)*maxRuns);
     ensureNbrCapacity(D == 1 ? 16 : (D == 2 ? 32 : 64));
+    ensureCandCapacity(D == 1 ? 16 : (D == 2 ? 48 : 96));
+  }
+
+  void dfree(void* p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    HIP_CHECK(hipFree(p));
+  }
+
+  // Grows the candidate lists to c entries per node, keeping every run's entries (the
+  // blocked layout does not depend on the capacity).
+  void ensureCandCapacity(int c) {
+    if (c <= f.ccap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    uint32_t* p = dalloc<uint32_t>((size_t)c*ld);
+    if (f.cand) HIP_CHECK(hipMemcpyAsync(p, f.cand, (size_t)f.ccap*ld*sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    dfree(f.cand);
+    dfree(candAlt);
+    f.cand = p;
+    f.ccap = c;
+    candAlt = nullptr;
+    candAltCap = 0;
+  }
+  void ensureCandAlt() {
+    if (candAlt && candAltCap == f.ccap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    dfree(candAlt);
+    candAlt = dalloc<uint32_t>((size_t)f.ccap*ld);
+    candAltCap = f.ccap;
+    if (!ccntAlt) ccntAlt = dalloc<int32_t>(ld);
   }
 
   void ensureNbrCapacity(int c) {
@@ -426,12 +508,15 @@ class Batch {
     nTotal += n;
     if (!retire) nActive = nTotal;     // a single run is always active, in upload order until the first sort
     connValid = false;
+    orderDirty = true;
     if (retire && tEnd <= 0.0) markDone(slot);
     return slot;
   }
 
   void markDone(int slot) {
     runs[slot].state = RUN_DONE;
+    runs[slot].candValid = false;
+    orderDirty = true;
     hsc[slot].active = 0;
     upload(&dsc[slot].active, &hsc[slot].active, sizeof(int));
   }
@@ -439,6 +524,7 @@ class Batch {
   void removeRun(int slot) {
     checkSlot(slot);
     runs[slot].state = RUN_REMOVED;
+    orderDirty = true;
     hsc[slot].active = 0;
     upload(&dsc[slot].active, &hsc[slot].active, sizeof(int));
   }
@@ -456,17 +542,21 @@ class Batch {
       g.cellBase = 0;
     }
     HIP_CHECK(hipMemcpyAsync(dgrids, hgrids, sizeof(RunGrid)*maxRuns, hipMemcpyHostToDevice, stream));
+    for (int r = 0; r < maxRuns; ++r) hbuild[r] = 0;
+    HIP_CHECK(hipMemcpyAsync(dbuild, hbuild, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
     sortAndGather(0);
+    orderDirty = false;
     connValid = false;
   }
 
   // ---------------------------------------------------------------- connectivity
-  // Per-run cell grids from the current extents; returns the Morton bits per axis needed.
+  // Cell grids of the runs that build candidate lists (hbuild), from their current extents;
+  // returns the Morton bits per axis needed.
   int setupGrids() {
     hipLaunchKernelGGL(k_reset_extent, dim3(nblocks((int64_t)maxRuns*7)), dim3(kBlock), 0, stream, dext, maxRuns);
     int64_t nt = 0;
     for (int r = 0; r < maxRuns; ++r) {
-      if (runs[r].state != RUN_ACTIVE) continue;
+      if (runs[r].state != RUN_ACTIVE || !hbuild[r]) continue;
       const int64_t b = runs[r].offset, e = b + runs[r].n;
       for (int64_t t = b; t < e; t += kExtentTile) {
         htiles[3*nt] = r;
@@ -492,11 +582,14 @@ class Batch {
       for (int a = 0; a < 3; ++a) { gr.origin[a] = 0.0; gr.inv_cell[a] = 0.0; gr.nc[a] = 1; }
       gr.cellBase = (int)base;
       gr.hinvMin = 0.0;
-      if (runs[r].state != RUN_ACTIVE) continue;
+      gr.ext2c = 0.0;
+      if (runs[r].state != RUN_ACTIVE || !hbuild[r]) continue;
       const double hinvMin = ord_value(hext[r*7]);
       if (!(hinvMin > 0.0) || !std::isfinite(hinvMin)) raise(SPX_ERR_NUMERIC, "non-positive or non-finite H on the device");
       gr.hinvMin = hinvMin;
-      const double support = cfg.neighbor_extent/hinvMin*(1.0 + 1.0e-9);
+      const double extc = cfg.neighbor_extent*(1.0 + runs[r].skin);
+      gr.ext2c = extc*extc;
+      const double support = extc/hinvMin*(1.0 + 1.0e-9);
       int64_t total = 1;
       for (int a = 0; a < D; ++a) {
         const double lo = ord_value(hext[r*7 + 1 + 2*a]), hi = ord_value(hext[r*7 + 2 + 2*a]);
@@ -507,6 +600,11 @@ class Batch {
         double cell, origin;
         if (blo == BC_PERIODIC) {
           nc = (int64_t)std::floor(L/support);
+          if (nc < 3 && runs[r].skin > 0.0) {
+            // too small a box for the widened radius: fall back to the plain one
+            runs[r].skin = 0.0;
+            return setupGrids();
+          }
           if (nc < 3) raise(SPX_ERR_UNSUPPORTED, "periodic box must span at least 3 support radii on every axis");
           nc = std::min<int64_t>(nc, maxAxis);
           cell = L/nc;
@@ -564,7 +662,8 @@ class Batch {
     return bits;
   }
 
-  // Sort by [rank | run | Morton] and gather the carried arrays; drops removed runs.
+  // Sort by [rank | run | Morton] and gather the carried arrays; drops removed runs.  Runs
+  // that keep their candidate lists (hbuild 0) keep their order; their lists follow them.
   void sortAndGather(int bitsPerAxis) {
     mortonBits = D*bitsPerAxis;
     const int endBit = mortonBits + runBits + 2;
@@ -575,7 +674,7 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       hipLaunchKernelGGL(k_keys<DD>, dim3(nb), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)dgrids, (const int*)dstate,
-                         (int)nTotal, wrapPending ? 1 : 0, 1, mortonBits, runBits, keys, vals);
+                         (const int*)dbuild, (int)nTotal, wrapPending ? 1 : 0, 1, mortonBits, runBits, keys, vals);
     });
     HIP_CHECK(hipGetLastError());
     wrapPending = false;
@@ -605,6 +704,8 @@ class Batch {
     std::swap(f.oid, oidAlt);
     std::swap(f.run, runAlt);
     // segments in key order: active runs by slot, then finished runs by slot
+    std::vector<int64_t> oldOffset(maxRuns);
+    for (int r = 0; r < maxRuns; ++r) oldOffset[r] = runs[r].offset;
     nActive = 0;
     int64_t live = 0;
     for (int r = 0; r < maxRuns; ++r) {
@@ -619,33 +720,109 @@ class Batch {
     }
     nTotal = live;
     npairsCache = -1;
+    // candidate lists of the runs that keep them move with their segments
+    bool moved = false;
+    for (int r = 0; r < maxRuns; ++r) {
+      hdelta[r] = kNoRemap;
+      if (runs[r].state == RUN_ACTIVE && !hbuild[r] && runs[r].candValid) {
+        hdelta[r] = (int)(runs[r].offset - oldOffset[r]);
+        moved |= hdelta[r] != 0;
+      }
+    }
+    if (moved && nActive > 0) {
+      ensureCandAlt();
+      HIP_CHECK(hipMemcpyAsync(ddelta, hdelta, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(k_remap_candidates, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, (const uint32_t*)f.cand,
+                         (const int32_t*)f.ccnt, candAlt, ccntAlt, (const int32_t*)f.run, (const int*)ddelta, (size_t)ld,
+                         (int)nActive);
+      HIP_CHECK(hipGetLastError());
+      std::swap(f.cand, candAlt);
+      std::swap(f.ccnt, ccntAlt);
+    }
   }
 
-  // withDensityDt: follow the neighbour build with the step's sum density and dt vote.
-  void buildConnectivity(bool withDensityDt) {
+  // The candidate list of run r no longer provably holds every neighbour pair (or the
+  // measurement is not finite).
+  bool skinTripped(int r) const {
+    const double d = skin_value(hsc[r].skinD), g = skin_value(hsc[r].skinG);
+    return !(g + 2.0*d <= (1.0 + runs[r].skin)*(1.0 - 1.0e-9));
+  }
+
+  // ConnectivityMap::computeConnectivity for every active run: runs whose candidate list is
+  // missing or stale (or all, with force) re-sort and rebuild it; then every active node's
+  // neighbour list is filtered from its candidates.  withDensityDt: follow with the step's
+  // sum density and dt vote.
+  void buildConnectivity(bool withDensityDt, bool force = false) {
     if (nTotal == 0) raise(SPX_ERR_STATE, "no state uploaded");
-    const int bits = setupGrids();
-    sortAndGather(bits);
+    bool any = false;
+    for (int r = 0; r < maxRuns; ++r) {
+      RunInfo& ri = runs[r];
+      hbuild[r] = 0;
+      if (ri.state != RUN_ACTIVE) continue;
+      const bool tripped = ri.candValid && skinTripped(r);
+      if (!(force || !ri.candValid || tripped)) continue;
+      if (tripped && ri.skin > 0.0) {
+        // the skin was used up in (cycle - builtCycle) steps: re-choose it from that rate
+        const long long k = std::max<long long>(1, hsc[r].cycle - ri.builtCycle);
+        const double used = skin_value(hsc[r].skinG) + 2.0*skin_value(hsc[r].skinD) - 1.0;
+        if (std::isfinite(used)) ri.skin = choose_skin(std::max(used, 0.0)/(double)k, D);
+      }
+      hbuild[r] = 1;
+      any = true;
+    }
+    if (any || orderDirty) {
+      HIP_CHECK(hipMemcpyAsync(dbuild, hbuild, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
+      const int bits = any ? setupGrids() : 0;
+      sortAndGather(bits);
+      orderDirty = false;
+    } else {
+      applyPendingBoundaries();
+    }
     connValid = false;
     if (nActive == 0) return;
     const int nb = nblocks(nActive);
-    launchD([&](auto d) {
-      constexpr int DD = decltype(d)::value;
-      hipLaunchKernelGGL(k_cell_bounds<DD>, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t*)keysAlt, (int)nActive,
-                         (const RunGrid*)dgrids, mortonBits, cellStart, cellEnd);
-    });
-    launchD([&](auto d) {
-      constexpr int DD = decltype(d)::value;
-      hipLaunchKernelGGL(k_nodescale<DD>, dim3(nb), dim3(kBlock), 0, stream, f, ph, (int)nActive);
-    });
-    HIP_CHECK(hipGetLastError());
+    if (any) {
+      launchD([&](auto d) {
+        constexpr int DD = decltype(d)::value;
+        hipLaunchKernelGGL(k_cell_bounds<DD>, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t*)keysAlt, (int)nActive,
+                           (const RunGrid*)dgrids, (const int*)dbuild, mortonBits, cellStart, cellEnd);
+      });
+      HIP_CHECK(hipGetLastError());
+      bool ok = false;
+      for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
+        HIP_CHECK(hipMemsetAsync(dOverflow, 0, sizeof(int), stream));
+        launchD([&](auto d) {
+          constexpr int DD = decltype(d)::value;
+          hipLaunchKernelGGL(k_candidates<DD>, dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f, geo,
+                             (const RunGrid*)dgrids, (const int*)dbuild, (int)nActive, (const int*)cellStart,
+                             (const int*)cellEnd, dOverflow);
+        });
+        HIP_CHECK(hipGetLastError());
+        download(hOverflow, dOverflow, sizeof(int));
+        if (*hOverflow == 0) { ok = true; break; }
+        const int need = *hOverflow;
+        if (need > 8192) raise(SPX_ERR_CAPACITY, "more than 8192 neighbour candidates for one node");
+        ensureCandCapacity(((need + need/4) + 7)/8*8);
+      }
+      if (!ok) raise(SPX_ERR_CAPACITY, "candidate list capacity not reached");
+      for (int r = 0; r < maxRuns; ++r) {
+        if (!hbuild[r]) continue;
+        RunInfo& ri = runs[r];
+        ri.candValid = true;
+        ri.builtCycle = hsc[r].cycle;
+        ++ri.builds;
+        ++candBuilds;
+        hsc[r].skinD = hsc[r].skinG = 0ull;
+      }
+    }
+    pack();
     bool ok = false;
     for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
       HIP_CHECK(hipMemsetAsync(dOverflow, 0, sizeof(int), stream));
       launchD([&](auto d) {
         constexpr int DD = decltype(d)::value;
-        hipLaunchKernelGGL(k_neighbors<DD>, dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f, geo, ph,
-                           (const RunGrid*)dgrids, (int)nActive, (const int*)cellStart, (const int*)cellEnd, dOverflow);
+        hipLaunchKernelGGL((k_neighbor_filter<DD, kGroup>), dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f,
+                           geo, ph, (int)nActive, dOverflow);
       });
       HIP_CHECK(hipGetLastError());
       download(hOverflow, dOverflow, sizeof(int));
@@ -668,7 +845,7 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       hipLaunchKernelGGL(k_keys<DD>, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)nullptr,
-                         (const int*)nullptr, (int)nTotal, 1, 0, 0, 0, keys, vals);
+                         (const int*)nullptr, (const int*)nullptr, (int)nTotal, 1, 0, 0, 0, keys, vals);
     });
     HIP_CHECK(hipGetLastError());
     wrapPending = false;
@@ -698,6 +875,17 @@ class Batch {
       launchPrec(k_omega<DD, double>, k_omega<DD, float>, nb, f, geo, dtab, (int)nActive, xs, hs, om, runMask);
     });
   }
+  // Packed neighbour records and nodeScale of the current state (active runs).
+  void pack() {
+    if (nActive == 0) return;
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_pack<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, ph, (int)nActive);
+    });
+    HIP_CHECK(hipGetLastError());
+  }
+  // Sum density and/or dt vote over the current neighbour lists; reads the packed records,
+  // which must describe the current state (pack()).
   void densityDt(bool dens, bool dtv) {
     if (nActive == 0) return;
     const int nb = (int)xcd_grid(nActive, kBlock);
@@ -857,6 +1045,12 @@ class Batch {
     mark("energy_update");
     omegaPass(false);
     mark("omega");
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_skin_check<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, geo, cfg.neighbor_extent,
+                         (int)nActive, dsc);
+    });
+    HIP_CHECK(hipGetLastError());
     wrapPending = true;
     hipLaunchKernelGGL(k_advance_time, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
     HIP_CHECK(hipGetLastError());
@@ -895,11 +1089,13 @@ class Batch {
   }
   void sumDensity() {
     requireConn();
+    pack();
     densityDt(true, false);
     HIP_CHECK(hipStreamSynchronize(stream));
   }
   void hydroDtVote(int slot, double* hydro, double* gravity) {
     requireConn();
+    pack();
     densityDt(false, true);
     hipLaunchKernelGGL(k_select_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns, ph, 0.0,
                        1.7976931348623157e308, 1.7976931348623157e308, cfg.gravity);

@@ -98,6 +98,7 @@ struct RunGrid {
   double origin[3];
   double inv_cell[3];
   double hinvMin;      // smallest h^-1 of the run's nodes: bounds every node's support radius
+  double ext2c;        // candidate-list criterion: ext^2 (1 + skin)^2
   int nc[3];
   int cellBase;
 };
@@ -125,8 +126,16 @@ struct Fields {
   // evaluateDerivatives outputs not consumed by the integration
   double *rhoSum, *norm, *effQ, *gradRho[3], *m1[3], *Mcorr;
   double *ns;        // nodeScale = 1/(lambda_max(H) nPerh) of the connectivity state
+  // Packed record of each node for the connectivity-time pair passes: x0 x1 x2 h^-1 v0 v1 v2 m
+  // (64 bytes, unused axes 0), so a neighbour read is four 16-byte loads from one line.
+  double *pk;
   double *pot;       // gravitational potential
   uint32_t* nbr;
+  // Candidate lists (same blocked layout as nbr): every j with |Hi rij| or |Hj rij| within
+  // ext (1 + skin) when the run's list was built, from positions xb and h^-1 hb of that time.
+  uint32_t* cand;
+  int32_t* ccnt;
+  double *xb[3], *hb;
   // Pairwise acceleration term of each ELL entry (the reference's pairAccelerations,
   // without the partner mass), in the pair arithmetic type: component c of entry k of
   // slot i at pdd[(c*cap + k)*ld + i], from the slot's side of the canonical pair.
@@ -134,7 +143,7 @@ struct Fields {
   int32_t* cnt;
   int64_t* oid;      // (run slot << 40) | node index within the run
   int32_t* run;      // run slot of each particle
-  int ld, cap;
+  int ld, cap, ccap;
 };
 
 // Device scalars of one run.
@@ -143,6 +152,9 @@ struct Scalars {
   double hydroDt, gravDt, dtMinAcc, tEnd;
   long long cycle;
   unsigned long long dtBits;   // running min of the hydro dt vote (order-preserving bits)
+  // since the run's candidate list was built (order-preserving bits, 0 = nothing measured):
+  // max over nodes of |x - xb| hb/ext, and of hb/h^-1 (growth of the support radius)
+  unsigned long long skinD, skinG;
   int active, overflow;
   int nonfinite, pad;
 };

@@ -281,7 +281,7 @@ int spx_run_update_connectivity(spx_run* run) {
   RUN_OR_FAIL(run);
   return guarded([&] {
     RUN_STATE(run);
-    run->impl->buildConnectivity(false);
+    run->impl->buildConnectivity(false, true);
     HIP_CHECK(hipStreamSynchronize(run->impl->stream));
   });
 }
@@ -374,6 +374,15 @@ int spx_run_num_pairs(spx_run* run, int64_t* npairs) {
   });
 }
 
+int spx_run_neighbor_builds(spx_run* run, int64_t* builds) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    if (!builds) spx::raise(SPX_ERR_ARG, "null argument");
+    RUN_STATE(run);
+    *builds = run->impl->runs[0].builds;
+  });
+}
+
 // ---------------------------------------------------------------- batches
 #define BATCH_OR_FAIL(b)                                                           \
   do {                                                                             \
@@ -454,6 +463,15 @@ int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs) {
     for (auto& r : B.runs) k += r.state != spx::RUN_FREE;
     if (particles) *particles = B.nTotal;
     if (runs) *runs = k;
+  });
+}
+
+int spx_batch_neighbor_builds(spx_batch* batch, int32_t run, int64_t* builds) {
+  BATCH_OR_FAIL(batch);
+  return guarded([&] {
+    if (!builds) spx::raise(SPX_ERR_ARG, "null argument");
+    batch->impl->checkSlot(run);
+    *builds = batch->impl->runs[run].builds;
   });
 }
 

@@ -48,12 +48,14 @@ __device__ __forceinline__ int cell_coord(const RunGrid& gr, int a, double x) {
 // opposite plane, reflecting violators mirrored with their velocity
 // (Boundary/PlanarBoundary.cc:153-200, ReflectingBoundary.cc:465-473).  Keys order
 // the particles of active runs by (run, cell Morton code), then the finished runs
-// (kept whole, behind the active ones), then removed runs (largest key).
+// (kept whole, behind the active ones), then removed runs (largest key).  Active runs
+// that keep their candidate list (build[run] == 0) get no Morton bits: the sort is
+// stable, so they keep their order.
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, const RunGrid* grids, const int* rstate, int n,
-                                                 int enforce, int withKeys, int mortonBits, int runBits,
-                                                 uint64_t* keys, uint32_t* vals) {
+__global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, const RunGrid* grids, const int* rstate,
+                                                 const int* build, int n, int enforce, int withKeys, int mortonBits,
+                                                 int runBits, uint64_t* keys, uint32_t* vals) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int r = f.run[i];
@@ -79,7 +81,7 @@ __global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, const RunGrid*
       }
       f.x[a][i] = xa;
     }
-    if (withKeys) key |= spread_bits((uint64_t)cell_coord<D>(grids[r], a, xa), D) << a;
+    if (withKeys && build[r]) key |= spread_bits((uint64_t)cell_coord<D>(grids[r], a, xa), D) << a;
   }
   if (withKeys) {
     // layout: [rank | run | Morton], rank 0 active, 1 done; removed runs take the maximum key
@@ -124,14 +126,16 @@ __device__ __forceinline__ void unspread(uint64_t key, int mortonBits, int* c) {
   }
 }
 
-// Cell ranges of the sorted keys of the active runs: start[cell], end[cell] (both 0 when empty).
+// Cell ranges of the sorted keys of the runs that build candidate lists: start[cell],
+// end[cell] (both 0 when empty).
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, int n, const RunGrid* grids, int mortonBits,
-                                                        int* start, int* end) {
+__global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, int n, const RunGrid* grids, const int* build,
+                                                        int mortonBits, int* start, int* end) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint64_t k = keys[p];
   const int r = (int)(k >> mortonBits);          // active keys carry rank 0 above the run field
+  if (!build[r]) return;
   int c[3];
   if (p == 0 || keys[p - 1] != k) { unspread<D>(k, mortonBits, c); start[lin_cell<D>(grids[r], c)] = p; }
   if (p == n - 1 || keys[p + 1] != k) { unspread<D>(k, mortonBits, c); end[lin_cell<D>(grids[r], c)] = p + 1; }
@@ -153,11 +157,40 @@ __device__ __forceinline__ void run_min(Scalars* sc, int r, double v, bool valid
   }
 }
 
-// nodeScale = 1/(lambda_max(H) nPerh) of each node (GenericHydro::dt, Physics/GenericHydro.cc:142-150).
+// Packed neighbour records (Fields::pk) of the current state, and nodeScale =
+// 1/(lambda_max(H) nPerh) of each node (GenericHydro::dt, Physics/GenericHydro.cc:142-150).
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_nodescale(Fields f, Phys ph, int n) {
+__global__ void __launch_bounds__(kBlock) k_pack(Fields f, Phys ph, int n) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
-  if (i < n) f.ns[i] = 1.0/max_eigen_iso<D>(f.hinv[i])/ph.nperh;
+  if (i >= n) return;
+  const double h = f.hinv[i];
+  double2* P = reinterpret_cast<double2*>(f.pk) + 4*(size_t)i;
+  P[0] = make_double2(f.x[0][i], D >= 2 ? f.x[1 % D][i] : 0.0);
+  P[1] = make_double2(D >= 3 ? f.x[2 % D][i] : 0.0, h);
+  P[2] = make_double2(f.v[0][i], D >= 2 ? f.v[1 % D][i] : 0.0);
+  P[3] = make_double2(D >= 3 ? f.v[2 % D][i] : 0.0, f.m[i]);
+  f.ns[i] = 1.0/max_eigen_iso<D>(h)/ph.nperh;
+}
+
+// Position and h^-1 of node j from its packed record.
+template <int D>
+__device__ __forceinline__ void load_xh(const double* pk, int j, double* x, double& h) {
+  const double2* P = reinterpret_cast<const double2*>(pk) + 4*(size_t)j;
+  const double2 a = P[0], b = P[1];
+  x[0] = a.x;
+  if (D >= 2) x[1 % D] = a.y;
+  if (D >= 3) x[2 % D] = b.x;
+  h = b.y;
+}
+// Velocity and mass of node j from its packed record.
+template <int D>
+__device__ __forceinline__ void load_vm(const double* pk, int j, double* v, double& m) {
+  const double2* P = reinterpret_cast<const double2*>(pk) + 4*(size_t)j;
+  const double2 a = P[2], b = P[3];
+  v[0] = a.x;
+  if (D >= 2) v[1 % D] = a.y;
+  if (D >= 3) v[2 % D] = b.x;
+  m = b.y;
 }
 
 // Store the 8-entry chunk holding entry k of slot i (two 16-byte stores).
@@ -170,32 +203,47 @@ __device__ __forceinline__ void flush_chunk(uint32_t* nbr, int k, int i, size_t 
 // ---------------------------------------------------------------------------
 // ConnectivityMap::computeConnectivity (Neighbor/ConnectivityMap.cc:905-947): j is a
 // neighbour of i iff |Hi rij|^2 <= ext^2 or |Hj rij|^2 <= ext^2, i != j (an image of i
-// itself counts, as the reference's ghost of i does).  The test is evaluated with the
-// reference's rounding (no contraction), so the sets match exactly.  One lane per
-// particle walks the 3^D cells around its own; lanes of a wavefront mostly share a cell,
-// so candidate reads are wave-uniform.  Candidate order (cells, then slots) is the ELL
-// order.
+// itself counts, as the reference's ghost of i does), evaluated with the reference's
+// rounding (no contraction) so the sets match exactly.
+//
+// Two stages.  k_candidates builds, for the runs whose list is stale, a candidate list
+// with the criterion widened to ext (1 + skin): one lane per particle walks the 3^D cells
+// (of side >= the widened support) around its own; lanes of a wavefront mostly share a
+// cell, so candidate reads are wave-uniform.  It records the positions and h^-1 the list
+// was built from.  k_neighbor_filter applies the exact criterion to the candidates every
+// step.  The candidates stay complete while no pair can have closed in from beyond the
+// widened radius: with D = max |x - xb| hb/ext and G = max hb/h^-1 over the run's nodes,
+// |rij(now)| <= max(Si, Sj)(now) implies |rij(built)| <= max(Si, Sj)(built) (G + 2D), so a
+// list is valid while G + 2D <= 1 + skin (k_skin_check measures both after every step).
+// Reflecting images are isometries of the node, and periodic images are re-chosen as the
+// nearest copy each step (the box spans >= 3 widened supports), so both obey the bound.
+// ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, const RunGrid* grids, int n,
-                                                      const int* cstart, const int* cend, int* overflow) {
+__global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const RunGrid* grids, const int* build, int n,
+                                                       const int* cstart, const int* cend, int* overflow) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const RunGrid gr = grids[f.run[i]];
+  const int r = f.run[i];
+  if (!build[r]) return;
+  const RunGrid gr = grids[r];
   double xi[D];
   int ci[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     xi[a] = f.x[a][i];
     ci[a] = cell_coord<D>(gr, a, xi[a]);
+    f.xb[a][i] = xi[a];
   }
   const double hi = f.hinv[i];
-  // Every neighbour lies within ext/min(hi, hj) <= ext/min(hi, hinvMin) of node i.  Farther
+  f.hb[i] = hi;
+  const double ext2c = gr.ext2c;
+  // Every candidate lies within extc/min(hi, hj) <= extc/min(hi, hinvMin) of node i.  Farther
   // candidates are rejected by an FP32 distance test first; its 1e-4 relative margin is far
-  // above the FP32 rounding of the test, so only the exact FP64 criterion decides membership.
+  // above the FP32 rounding of the test, so only the FP64 criterion decides membership.
   const double hq = fmin(hi, gr.hinvMin);
-  const float reach2 = (float)(ph.ext2/(hq*hq)*(1.0 + 1.0e-4));
+  const float reach2 = (float)(ext2c/(hq*hq)*(1.0 + 1.0e-4));
   const size_t ld = (size_t)f.ld;
-  const int cap = f.cap;
+  const int cap = f.ccap;
   int count = 0;
   uint32_t chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int r1 = D >= 2 ? 1 : 0, r2 = D >= 3 ? 1 : 0;
@@ -243,20 +291,157 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(Fields f, Geo g, Phys ph, 
             e2i = a == 0 ? dmul(ei, ei) : dadd(e2i, dmul(ei, ei));
             e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
           }
-          if (e2i <= ph.ext2 || e2j <= ph.ext2) {
+          if (e2i <= ext2c || e2j <= ext2c) {
             // write-combine 8 entries in registers, flush them as one 32-byte chunk
             const uint32_t e = (uint32_t)j | ((uint32_t)code << kImgShift);
             const int s = count & 7;
 #pragma unroll
             for (int u = 0; u < 8; ++u) chunk[u] = s == u ? e : chunk[u];
             ++count;
-            if (s == 7 && count <= cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
+            if (s == 7 && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
           }
         }
       }
+  if ((count & 7) && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
+  f.ccnt[i] = count;
+  if (count > cap) atomicMax(overflow, count);
+}
+
+// The exact criterion over each active node's candidates; writes the ELL neighbour list in
+// candidate order.  Periodic image codes are re-chosen from the current positions (nearest
+// copy); reflecting codes are kept.
+template <int D, int G>
+__global__ void __launch_bounds__(kBlock) k_neighbor_filter(Fields f, Geo g, Phys ph, int n, int* overflow) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xi[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) xi[a] = f.x[a][i];
+  const double hi = f.hinv[i];
+  const size_t ld = (size_t)f.ld;
+  const int cap = f.cap;
+  const int nc = f.ccnt[i];
+  int count = 0;
+  uint32_t chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k0 = 0; k0 < nc; k0 += G) {
+    int js[G], codes[G];
+    ell_group<G>(f.cand, ld, k0, i, nc, js, codes);
+    double xj[G][D], hj[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) load_xh<D>(f.pk, js[u], xj[u], hj[u]);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (k0 + u >= nc) continue;
+      int code = codes[u];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        if ((g.permask >> a) & 1) {
+          const double half = 0.5*(g.xmax[a] - g.xmin[a]);
+          const double d = xj[u][a] - xi[a];
+          const int c = d > half ? 1 : (d < -half ? 2 : 0);
+          code = (code & ~(3 << (2*a))) | (c << (2*a));
+        }
+      }
+      if (js[u] == i && code == 0) continue;
+      if (code) image_pos<D>(g, code, xj[u]);
+      double e2i = 0.0, e2j = 0.0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const double rr = dsub(xi[a], xj[u][a]);
+        const double ei = dmul(hi, rr), ej = dmul(hj[u], rr);
+        e2i = a == 0 ? dmul(ei, ei) : dadd(e2i, dmul(ei, ei));
+        e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
+      }
+      if (e2i <= ph.ext2 || e2j <= ph.ext2) {
+        const uint32_t e = (uint32_t)js[u] | ((uint32_t)code << kImgShift);
+        const int s = count & 7;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) chunk[q] = s == q ? e : chunk[q];
+        ++count;
+        if (s == 7 && count <= cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
+      }
+    }
+  }
   if ((count & 7) && count <= cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
   f.cnt[i] = count;
   if (count > cap) atomicMax(overflow, count);
+}
+
+// Per-run maximum of an order-preserving double key: a wavefront whose lanes share a run
+// reduces first, and an atomic is issued only when it would raise the running maximum
+// (most wavefronts do not: reading first keeps them off the contended word).
+__device__ __forceinline__ void run_max_bits(unsigned long long* word, size_t stride, int r, unsigned long long b,
+                                             bool valid) {
+  const int r0 = __shfl(r, 0);
+  if (__all(!valid || r == r0)) {
+    unsigned long long m = valid ? b : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(m, off);
+      m = o > m ? o : m;
+    }
+    unsigned long long* w = word + (size_t)r0*stride;
+    if ((threadIdx.x & 63) == 0 && __any(valid) && m > __atomic_load_n(w, __ATOMIC_RELAXED)) atomicMax(w, m);
+  } else if (valid) {
+    unsigned long long* w = word + (size_t)r*stride;
+    if (b > __atomic_load_n(w, __ATOMIC_RELAXED)) atomicMax(w, b);
+  }
+}
+
+// Displacement and support growth of every active node since its run's candidate list was
+// built (end-of-step positions, before boundary enforcement, which can only shorten them).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_skin_check(Fields f, Geo g, double ext, int n, Scalars* sc) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  int r = 0;
+  double dmax = 0.0, gmax = 0.0;
+  if (valid) {
+    r = f.run[i];
+    double d2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      double d = f.x[a][i] - f.xb[a][i];
+      if ((g.permask >> a) & 1) {
+        const double L = g.xmax[a] - g.xmin[a];
+        d = fabs(d);
+        d = fmin(d, fabs(L - d));
+      }
+      d2 += d*d;
+    }
+    const double hb = f.hb[i];
+    dmax = sqrt(d2)*hb/ext;
+    gmax = hb/f.hinv[i];
+  }
+  const size_t stride = sizeof(Scalars)/sizeof(unsigned long long);
+  run_max_bits(&sc[0].skinD, stride, r, ord_bits(dmax), valid);
+  run_max_bits(&sc[0].skinG, stride, r, ord_bits(gmax), valid);
+}
+
+// Moves the candidate lists of runs whose particle segment shifted in a reorder (a run
+// that keeps its list keeps its order, so old slot = new slot - delta):
+// dst row p = src row p - delta with every index shifted by delta.  delta[run] ==
+// kNoRemap marks runs whose rows are not moved.
+constexpr int kNoRemap = INT32_MIN;
+__global__ void __launch_bounds__(kBlock) k_remap_candidates(const uint32_t* src, const int32_t* srcCnt, uint32_t* dst,
+                                                             int32_t* dstCnt, const int32_t* run, const int* delta,
+                                                             size_t ld, int n) {
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int d = delta[run[p]];
+  if (d == kNoRemap) return;
+  const int q = p - d;
+  const int cnt = srcCnt[q];
+  dstCnt[p] = cnt;
+  for (int c = 0; c < cnt; c += 8) {
+    const uint4* s = reinterpret_cast<const uint4*>(src + ell_at(c, q, ld));
+    uint4* t = reinterpret_cast<uint4*>(dst + ell_at(c, p, ld));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint4 v = s[h];
+      v.x += (uint32_t)d; v.y += (uint32_t)d; v.z += (uint32_t)d; v.w += (uint32_t)d;
+      t[h] = v;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -293,17 +478,9 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int j = js[u];
-        if (DENS) {
-#pragma unroll
-          for (int a = 0; a < D; ++a) xj[u][a] = f.x[a][j];
-          hj[u] = f.hinv[j];
-          mj[u] = f.m[j];
-        }
-        if (DT) {
-#pragma unroll
-          for (int a = 0; a < D; ++a) vj[u][a] = f.v[a][j];
-          nsj[u] = f.ns[j];
-        }
+        if (DENS) load_xh<D>(f.pk, j, xj[u], hj[u]);
+        load_vm<D>(f.pk, j, vj[u], mj[u]);
+        if (DT) nsj[u] = f.ns[j];
       }
 #pragma unroll
       for (int u = 0; u < G; ++u) {
@@ -360,6 +537,8 @@ __global__ void k_select_dt(Scalars* sc, int nruns, Phys ph, double dtMin, doubl
   const int r = blockIdx.x*blockDim.x + threadIdx.x;
   if (r >= nruns) return;
   Scalars& s = sc[r];
+  s.skinD = 0ull;
+  s.skinG = 0ull;
   if (!s.active) { s.dt = 0.0; s.hdt = 0.0; return; }
   const double vote = ord_value(s.dtBits);
   const double hydro = vote*ph.cfl;

@@ -20,10 +20,10 @@ EXPORTS = (
     "spx_run_step", "spx_run_advance", "spx_run_time", "spx_run_update_connectivity", "spx_run_neighbor_counts",
     "spx_run_neighbors", "spx_run_sum_density", "spx_run_update_eos", "spx_run_post_state_update",
     "spx_run_evaluate_derivatives", "spx_run_dt", "spx_run_compatible_energy", "spx_run_conservation",
-    "spx_run_kernel_times", "spx_run_num_pairs",
+    "spx_run_kernel_times", "spx_run_num_pairs", "spx_run_neighbor_builds",
     "spx_batch_create", "spx_batch_destroy", "spx_batch_add_run", "spx_batch_remove_run", "spx_batch_step",
     "spx_batch_run_status", "spx_batch_get_field", "spx_batch_conservation", "spx_batch_usage",
-    "spx_batch_kernel_times", "spx_bandwidth_probe",
+    "spx_batch_kernel_times", "spx_batch_neighbor_builds", "spx_bandwidth_probe",
 )
 
 RUN_ACTIVE = 1
@@ -75,6 +75,7 @@ def lib():
             "spx_run_conservation": (C.c_int, [P, D]),
             "spx_run_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
             "spx_run_num_pairs": (C.c_int, [P, I64]),
+            "spx_run_neighbor_builds": (C.c_int, [P, I64]),
             "spx_batch_create": (C.c_int, [cfgp, C.c_int64, C.c_int32, C.c_int32, C.POINTER(P)]),
             "spx_batch_destroy": (None, [P]),
             "spx_batch_add_run": (C.c_int, [P, D, D, D, D, D, D, C.c_int64, C.c_double, I32]),
@@ -84,6 +85,7 @@ def lib():
             "spx_batch_get_field": (C.c_int, [P, C.c_int32, C.c_char_p, D, C.c_int64, I64]),
             "spx_batch_conservation": (C.c_int, [P, C.c_int32, D]),
             "spx_batch_usage": (C.c_int, [P, I64, I32]),
+            "spx_batch_neighbor_builds": (C.c_int, [P, C.c_int32, I64]),
             "spx_batch_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
             "spx_bandwidth_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, D]),
         }
@@ -230,6 +232,12 @@ class Run:
         _check(lib().spx_run_num_pairs(self.h, C.byref(n)))
         return n.value
 
+    def neighbor_builds(self):
+        """Candidate-list builds so far (start-up, forced rebuilds, and skin exhaustion)."""
+        n = C.c_int64()
+        _check(lib().spx_run_neighbor_builds(self.h, C.byref(n)))
+        return n.value
+
     def enable_timing(self):
         _check(lib().spx_run_kernel_times(self.h, None, -1, None, None, 0))
 
@@ -305,6 +313,11 @@ class Batch:
         p, r = C.c_int64(), C.c_int32()
         _check(lib().spx_batch_usage(self.h, C.byref(p), C.byref(r)))
         return p.value, r.value
+
+    def neighbor_builds(self, rid):
+        n = C.c_int64()
+        _check(lib().spx_batch_neighbor_builds(self.h, rid, C.byref(n)))
+        return n.value
 
     def enable_timing(self):
         _check(lib().spx_batch_kernel_times(self.h, None, -1, None, None, 0))

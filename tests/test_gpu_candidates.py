@@ -1,0 +1,111 @@
+"""Neighbour sets over many steps, where the engine filters each step's ConnectivityMap from
+candidate lists it rebuilds only when nodes have moved or grown enough to need it.
+
+Each step's sets must equal, bit for bit, what the oracle's from-scratch ConnectivityMap gives
+on the same positions and H (the state read back just before the step).  The cases cover
+periodic re-imaging of nodes that cross the box, reflecting walls, an open expanding blob
+(growing h) and a fast shear flow, so both list reuse and rebuilds are exercised."""
+import numpy as np
+import pytest
+
+from spheral_amd import abi, problems
+
+pytestmark = pytest.mark.gpu
+
+STATE_FIELDS = (("x", "position"), ("v", "velocity"), ("m", "mass"), ("rho", "mass density"),
+                ("eps", "specific thermal energy"), ("H", "H"))
+
+
+def cases():
+    out = []
+    cfg, st = problems.kelvin_helmholtz_2d(3000)
+    cfg.precision = abi.SPX_PREC_FP64
+    out.append(("kelvin_helmholtz_2d_periodic", cfg, st, 40))
+    cfg, st = problems.sod_1d(400)
+    out.append(("sod1d_reflecting", cfg, st, 40))
+    cfg, st = problems.taylor_green_3d(16)
+    st["v"] = st["v"]*40.0            # fast enough to cross the periodic planes within the test
+    out.append(("taylor_green_16^3_fast", cfg, st, 25))
+    rng = np.random.default_rng(5)
+    n = 2000
+    x = rng.normal(0, 0.3, (n, 3))
+    cfg = abi.default_config(3, nperh=1.51)
+    H = np.zeros((n, 3, 3))
+    for a in range(3):
+        H[:, a, a] = 1.0/0.08
+    st = dict(x=x, v=2.0*x, m=np.full(n, 1.0/n), rho=np.ones(n), eps=np.full(n, 0.5), H=H)
+    out.append(("open_3d_expanding_blob", cfg, st, 25))
+    cfg, st = problems.uniform_lattice(2, 24, kernel=abi.SPX_KERNEL_BSPLINE)
+    cfg.bc_lo[0] = cfg.bc_hi[0] = abi.SPX_BC_REFLECTING
+    st["v"] = np.random.default_rng(9).normal(0.0, 0.5, st["x"].shape)
+    out.append(("lattice_2d_reflecting_x_periodic_y", cfg, st, 30))
+    return out
+
+
+CASES = cases()
+IDS = [c[0] for c in CASES]
+
+
+def sets(off, nbr, img):
+    return [sorted(zip(nbr[off[i]:off[i + 1]].tolist(), img[off[i]:off[i + 1]].tolist()))
+            for i in range(len(off) - 1)]
+
+
+@pytest.mark.parametrize("name,cfg,st,nsteps", CASES, ids=IDS)
+def test_every_step_matches_a_fresh_connectivity(gpu, oracle_mod, name, cfg, st, nsteps):
+    print(f"[case] {name}", flush=True)
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    n = len(st["m"])
+    for k in range(nsteps):
+        cur = {key: g.field(f) for key, f in STATE_FIELDS}
+        g.step()
+        o = oracle_mod.OracleRun(cfg, n)
+        o.set_state(**cur)
+        o.initialize()
+        go, gn, gi = g.neighbors()
+        oo, on_, oi = o.neighbors()
+        assert np.array_equal(go, oo), (name, k)
+        assert sets(go, gn, gi) == sets(oo, on_, oi), (name, k)
+    builds = g.neighbor_builds()
+    print(f"[case] {name}: {builds} candidate-list builds in {nsteps} steps", flush=True)
+    assert 1 <= builds <= nsteps + 1
+
+
+def test_lists_are_reused_and_rebuilt(gpu):
+    """A slow flow reuses its candidate list for many steps; a fast one rebuilds often."""
+    cfg, st = problems.taylor_green_3d(16)
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    for _ in range(20):
+        g.step()
+    slow = g.neighbor_builds()
+    st["v"] = st["v"]*200.0
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    for _ in range(20):
+        g.step()
+    fast = g.neighbor_builds()
+    print(f"[builds] slow {slow}, fast {fast}", flush=True)
+    assert slow <= 2
+    assert fast >= 4
+
+
+def test_forced_rebuild_keeps_results(gpu):
+    """spx_run_update_connectivity between steps rebuilds every list; the sets it gives are the
+    ones the step path filtered."""
+    cfg, st = problems.kelvin_helmholtz_2d(2000)
+    cfg.precision = abi.SPX_PREC_FP64
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    for _ in range(5):
+        g.step()
+    b0 = g.neighbor_builds()
+    cur = {key: g.field(f) for key, f in STATE_FIELDS}
+    g.update_connectivity()
+    assert g.neighbor_builds() == b0 + 1
+    h = gpu.make_run(cfg, cur)
+    h.initialize()
+    a, b = g.neighbors(), h.neighbors()
+    assert np.array_equal(a[0], b[0])
+    assert sets(*a) == sets(*b)
