@@ -819,11 +819,22 @@ class Batch {
     bool ok = false;
     for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
       HIP_CHECK(hipMemsetAsync(dOverflow, 0, sizeof(int), stream));
-      launchD([&](auto d) {
-        constexpr int DD = decltype(d)::value;
-        hipLaunchKernelGGL((k_neighbor_filter<DD, kGroup>), dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f,
-                           geo, ph, (int)nActive, dOverflow);
-      });
+      const unsigned nbx = xcd_grid(nActive, kBlock);
+      if (withDensityDt) {
+        // the step's sum density and dt vote ride on the filter
+        hipLaunchKernelGGL(k_reset_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
+        launchD([&](auto d) {
+          constexpr int DD = decltype(d)::value;
+          launchPrec(k_density_dt<DD, true, true, double, kGroup, true>, k_density_dt<DD, true, true, float, kGroup, true>,
+                     (int)nbx, f, geo, ph, dtab, (int)nActive, dsc, dOverflow);
+        });
+      } else {
+        launchD([&](auto d) {
+          constexpr int DD = decltype(d)::value;
+          hipLaunchKernelGGL((k_neighbor_filter<DD, kGroup>), dim3(nbx), dim3(kBlock), 0, stream, f, geo, ph,
+                             (int)nActive, dOverflow);
+        });
+      }
       HIP_CHECK(hipGetLastError());
       download(hOverflow, dOverflow, sizeof(int));
       if (*hOverflow == 0) { ok = true; break; }
@@ -832,7 +843,6 @@ class Batch {
       ensureNbrCapacity(((need + need/4) + 7)/8*8);
     }
     if (!ok) raise(SPX_ERR_CAPACITY, "neighbour list capacity not reached");
-    if (withDensityDt) densityDt(true, true);
     for (int a = 0; a < 3; ++a) { hptrs[a] = f.xm[a]; hptrs[3 + a] = f.x[a]; }
     HIP_CHECK(hipMemcpyAsync(xsMid, hptrs, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
     HIP_CHECK(hipMemcpyAsync(xsState, hptrs + 3, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
@@ -894,13 +904,13 @@ class Batch {
       constexpr int DD = decltype(d)::value;
       if (dens && dtv)
         launchPrec(k_density_dt<DD, true, true, double, kGroup>, k_density_dt<DD, true, true, float, kGroup>, nb, f, geo, ph,
-                   dtab, (int)nActive, dsc);
+                   dtab, (int)nActive, dsc, (int*)nullptr);
       else if (dens)
         launchPrec(k_density_dt<DD, true, false, double, kGroup>, k_density_dt<DD, true, false, float, kGroup>, nb, f, geo,
-                   ph, dtab, (int)nActive, dsc);
+                   ph, dtab, (int)nActive, dsc, (int*)nullptr);
       else
         launchPrec(k_density_dt<DD, false, true, double, kGroup>, k_density_dt<DD, false, true, float, kGroup>, nb, f, geo,
-                   ph, dtab, (int)nActive, dsc);
+                   ph, dtab, (int)nActive, dsc, (int*)nullptr);
     });
   }
   void sphPass(bool full) {

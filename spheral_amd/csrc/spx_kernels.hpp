@@ -449,9 +449,13 @@ __global__ void __launch_bounds__(kBlock) k_remap_candidates(const uint32_t* src
 // -- node i receives m_j W(|Hj ri - Hj rj|, det Hj) -- fused with the GenericHydro::dt
 // vote (Physics/GenericHydro.cc:112-381): per-node limits on the new density and the
 // pairwise velocity-difference limit, min-reduced per run.
+// FILT: the pass walks the candidate lists instead, applies the neighbour criterion of
+// k_neighbor_filter to each candidate and writes the ELL neighbour list as it goes, so the
+// connectivity filter and the first pair pass of the step read the neighbours once.
 // ---------------------------------------------------------------------------
-template <int D, bool DENS, bool DT, typename R, int G>
-__global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc) {
+template <int D, bool DENS, bool DT, typename R, int G, bool FILT = false>
+__global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc,
+                                                       int* overflow) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   double dtmin = 1.7976931348623157e308;
   const bool valid = i < n;
@@ -470,24 +474,57 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
       rho = f.m[i]*(double)w;
     }
     double vmax2 = 0.0;       // pairwise limit, tracked as max of |v_ij|^2 / min(ns)^2
-    const int nn = f.cnt[i];
+    const size_t ld = (size_t)f.ld;
+    const int nn = FILT ? f.ccnt[i] : f.cnt[i];
+    const double hiF = FILT ? f.hinv[i] : 0.0;
+    int count = 0;
+    uint32_t chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int k0 = 0; k0 < nn; k0 += G) {
       int js[G], codes[G];
-      ell_group<G>(f.nbr, (size_t)f.ld, k0, i, nn, js, codes);
+      ell_group<G>(FILT ? f.cand : f.nbr, ld, k0, i, nn, js, codes);
       double xj[G][D], vj[G][D], hj[G], mj[G], nsj[G];
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int j = js[u];
-        if (DENS) load_xh<D>(f.pk, j, xj[u], hj[u]);
+        if (DENS || FILT) load_xh<D>(f.pk, j, xj[u], hj[u]);
         load_vm<D>(f.pk, j, vj[u], mj[u]);
         if (DT) nsj[u] = f.ns[j];
       }
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         if (k0 + u >= nn) continue;
-        const int code = codes[u];
-        if (DENS) {
+        int code = codes[u];
+        if (FILT) {
+          // the exact neighbour criterion (k_neighbor_filter) on the candidate
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            if ((g.permask >> a) & 1) {
+              const double half = 0.5*(g.xmax[a] - g.xmin[a]);
+              const double d = xj[u][a] - xi[a];
+              const int c = d > half ? 1 : (d < -half ? 2 : 0);
+              code = (code & ~(3 << (2*a))) | (c << (2*a));
+            }
+          }
+          if (js[u] == i && code == 0) continue;
           if (code) image_pos<D>(g, code, xj[u]);
+          double e2i = 0.0, e2j = 0.0;
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            const double rr = dsub(xi[a], xj[u][a]);
+            const double ei = dmul(hiF, rr), ej = dmul(hj[u], rr);
+            e2i = a == 0 ? dmul(ei, ei) : dadd(e2i, dmul(ei, ei));
+            e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
+          }
+          if (!(e2i <= ph.ext2 || e2j <= ph.ext2)) continue;
+          const uint32_t e = (uint32_t)js[u] | ((uint32_t)code << kImgShift);
+          const int sl = count & 7;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) chunk[q] = sl == q ? e : chunk[q];
+          ++count;
+          if (sl == 7 && count <= f.cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
+        }
+        if (DENS) {
+          if (!FILT && code) image_pos<D>(g, code, xj[u]);
           R s = (R)0;
           if (sizeof(R) == sizeof(double)) {
 #pragma unroll
@@ -508,6 +545,11 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
           vmax2 = fmax(vmax2, s/(nsm*nsm));
         }
       }
+    }
+    if (FILT) {
+      if ((count & 7) && count <= f.cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
+      f.cnt[i] = count;
+      if (count > f.cap) atomicMax(overflow, count);
     }
     if (DENS) f.rho[i] = rho; else rho = f.rho[i];
     if (DT) {
