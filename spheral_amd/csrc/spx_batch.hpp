@@ -115,8 +115,6 @@ class Batch {
   uint32_t* candAlt = nullptr;
   int32_t* ccntAlt = nullptr;
   int candAltCap = 0;
-  double** xsMid = nullptr;
-  double** xsState = nullptr;
   uint64_t *keys = nullptr, *keysAlt = nullptr;
   uint32_t *vals = nullptr, *valsAlt = nullptr;
   void* sortTmp = nullptr;
@@ -139,7 +137,6 @@ class Batch {
   int* hOverflow = nullptr;
   int* hbuild = nullptr;
   int* hdelta = nullptr;
-  double** hptrs = nullptr;
   int ld = 0;
   int mortonBits = 0, runBits = 0;
   int64_t nActive = 0;    // particles of active runs (front of the arrays after a sort)
@@ -176,7 +173,7 @@ class Batch {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto e : evs) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
-    for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow, (void*)hptrs,
+    for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow,
                     (void*)htiles, (void*)hbuild, (void*)hdelta})
       if (p) (void)hipHostFree(p);
     grav.release();
@@ -301,7 +298,7 @@ class Batch {
     std::vector<double**> all;
     Fields& F = f;
     for (int a = 0; a < D; ++a)
-      for (double** p : {&F.x[a], &F.v[a], &F.xm[a], &F.vm[a], &F.v1[a], &F.v12[a], &F.DxDt[a], &F.DvDt[a], &F.DvDtG[a],
+      for (double** p : {&F.x[a], &F.v[a], &F.xm[a], &F.vm[a], &F.v1[a], &F.DxDt[a], &F.DvDt[a], &F.DvDtG[a],
                          &F.gradRho[a], &F.m1[a], &F.xb[a]})
         all.push_back(p);
     for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.rhom, &F.epsm, &F.hinvm, &F.Pm,
@@ -316,7 +313,7 @@ class Batch {
     F.DvDx = slab + off; off += (size_t)D*D*ld;
     F.Mcorr = slab + off;
     for (int a = D; a < 3; ++a)
-      F.x[a] = F.v[a] = F.xm[a] = F.vm[a] = F.v1[a] = F.v12[a] = F.DxDt[a] = F.DvDt[a] = F.DvDtG[a] = F.gradRho[a] =
+      F.x[a] = F.v[a] = F.xm[a] = F.vm[a] = F.v1[a] = F.DxDt[a] = F.DvDt[a] = F.DvDtG[a] = F.gradRho[a] =
           F.m1[a] = F.xb[a] = nullptr;
     alt = f;
     auto cl = carried(alt);
@@ -326,7 +323,11 @@ class Batch {
     off = 0;
     for (double** p : cl) { *p = slabAlt + off; off += ld; }
     alt.DvDx = slabAlt + off;
-    f.pk = dalloc<double>((size_t)8*ld);
+    f.xh = dalloc<double>((size_t)4*ld);
+    f.vmr = dalloc<double>((size_t)4*ld);
+    f.xhm = dalloc<double>((size_t)4*ld);
+    f.pkm = dalloc<double>((size_t)8*ld);
+    f.pkv = dalloc<double>((size_t)6*ld);
     f.oid = dalloc<int64_t>(ld);
     oidAlt = dalloc<int64_t>(ld);
     f.run = dalloc<int32_t>(ld);
@@ -356,8 +357,6 @@ class Batch {
     dOverflow = dalloc<int>(1);
     dbuild = dalloc<int>(maxRuns);
     ddelta = dalloc<int>(maxRuns);
-    xsMid = dalloc<double*>(3);
-    xsState = dalloc<double*>(3);
     hgrids = halloc<RunGrid>(maxRuns);
     hsc = halloc<Scalars>(maxRuns);
     hext = halloc<unsigned long long>((size_t)maxRuns*7);
@@ -366,7 +365,6 @@ class Batch {
     hOverflow = halloc<int>(1);
     hbuild = halloc<int>(maxRuns);
     hdelta = halloc<int>(maxRuns);
-    hptrs = halloc<double*>(6);
     tilesAlloc = cap/kExtentTile + maxRuns + 1;
     dtiles = dalloc<int>(3*tilesAlloc);
     htiles = halloc<int>(3*tilesAlloc);
@@ -843,9 +841,6 @@ class Batch {
       ensureNbrCapacity(((need + need/4) + 7)/8*8);
     }
     if (!ok) raise(SPX_ERR_CAPACITY, "neighbour list capacity not reached");
-    for (int a = 0; a < 3; ++a) { hptrs[a] = f.xm[a]; hptrs[3 + a] = f.x[a]; }
-    HIP_CHECK(hipMemcpyAsync(xsMid, hptrs, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
-    HIP_CHECK(hipMemcpyAsync(xsState, hptrs + 3, 3*sizeof(double*), hipMemcpyHostToDevice, stream));
     connValid = true;
   }
 
@@ -875,14 +870,22 @@ class Batch {
     HIP_CHECK(hipGetLastError());
   }
   void omegaPass(bool mid, const int* runMask = nullptr) {
-    if (!cfg.gradh_correction || nActive == 0) return;
+    if (nActive == 0) return;
+    if (!cfg.gradh_correction) {
+      if (mid) packMid(true);
+      return;
+    }
     const int nb = (int)xcd_grid(nActive, kBlock);
-    double* const* xs = mid ? xsMid : xsState;
-    const double* hs = mid ? f.hinvm : f.hinv;
-    double* om = mid ? f.omegam : f.omega;
+    // midpoint: the record k_midstep wrote; state: the connectivity record (k_pack), or the
+    // end-of-step position k_full_update left in it
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      launchPrec(k_omega<DD, double>, k_omega<DD, float>, nb, f, geo, dtab, (int)nActive, xs, hs, om, runMask);
+      if (mid)
+        launchPrec(k_omega<DD, double, true>, k_omega<DD, float, true>, nb, f, geo, dtab, (int)nActive,
+                   (const double*)f.xhm, f.omegam, runMask);
+      else
+        launchPrec(k_omega<DD, double, false>, k_omega<DD, float, false>, nb, f, geo, dtab, (int)nActive,
+                   (const double*)f.xh, f.omega, runMask);
     });
   }
   // Packed neighbour records and nodeScale of the current state (active runs).
@@ -915,9 +918,6 @@ class Batch {
   }
   void sphPass(bool full) {
     const int nb = (int)xcd_grid(nActive, kBlock);
-    hipLaunchKernelGGL(k_prho, dim3(nb), dim3(kBlock), 0, stream, f, (int)nActive);
-    HIP_CHECK(hipGetLastError());
-    mark("prho");
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (full) launchPrec(k_sph<DD, true, double>, k_sph<DD, true, float>, nb, f, geo, ph, dtab, (int)nActive);
@@ -937,6 +937,16 @@ class Batch {
     };
     for (int a = 0; a < D; ++a) { cp(f.xm[a], f.x[a]); cp(f.vm[a], f.v[a]); }
     cp(f.rhom, f.rho); cp(f.epsm, f.eps); cp(f.hinvm, f.hinv); cp(f.Pm, f.P); cp(f.csm, f.cs); cp(f.omegam, f.omega);
+    packMid(false);
+  }
+  // Midpoint records for the SPH pass (k_pack_mid).
+  void packMid(bool keepOmega) {
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_pack_mid<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (int)nActive,
+                         keepOmega ? 1 : 0);
+    });
+    HIP_CHECK(hipGetLastError());
   }
 
   void mark(const char* name) {

@@ -117,8 +117,6 @@ struct Fields {
   double *xm[3], *vm[3], *rhom, *epsm, *hinvm, *Pm, *csm, *omegam, *prhom;
   // end-of-step velocity (swapped with v after the full update)
   double *v1[3];
-  // half-step velocity v + a dt/2 of the compatible-energy partition
-  double *v12[3];
   // derivatives carried across steps
   double *DxDt[3], *DvDt[3], *DvDtG[3], *DepsDt, *DrhoDt, *DHDt, *DvDx, *maxQ;
   // derivatives consumed within a step
@@ -126,9 +124,17 @@ struct Fields {
   // evaluateDerivatives outputs not consumed by the integration
   double *rhoSum, *norm, *effQ, *gradRho[3], *m1[3], *Mcorr;
   double *ns;        // nodeScale = 1/(lambda_max(H) nPerh) of the connectivity state
-  // Packed record of each node for the connectivity-time pair passes: x0 x1 x2 h^-1 v0 v1 v2 m
-  // (64 bytes, unused axes 0), so a neighbour read is four 16-byte loads from one line.
-  double *pk;
+  // Packed per-node records, so that a pair pass reads a neighbour with a few 16-byte loads
+  // from one line per record instead of one load per array.  Each record array holds only
+  // fields its readers use together (a pass that needs x and h^-1 alone reads 32 bytes per
+  // node).  Unused axes hold 0.
+  //   xh  (4 doubles): x0 x1 x2 h^-1 of the connectivity state (k_pack), then of the
+  //                    end-of-step state (k_full_update)
+  //   vmr (4):         v0 v1 v2 m of the connectivity state (k_pack)
+  //   xhm (4):         x0 x1 x2 h^-1 of the midpoint state (k_midstep)
+  //   pkm (8):         v0 v1 v2 m rho cs Prho 0 of the midpoint state (midpoint grad-h pass)
+  //   pkv (6):         v^{1/2} (3) DepsDt m 0 for the compatible-energy partition (k_v12)
+  double *xh, *vmr, *xhm, *pkm, *pkv;
   double *pot;       // gravitational potential
   uint32_t* nbr;
   // Candidate lists (same blocked layout as nbr): every j with |Hi rij| or |Hj rij| within

@@ -325,6 +325,7 @@ int spx_run_post_state_update(spx_run* run) {
     RUN_STATE(run);
     auto& B = *run->impl;
     B.requireConn();
+    B.pack();
     B.omegaPass(false);
     HIP_CHECK(hipStreamSynchronize(B.stream));
   });

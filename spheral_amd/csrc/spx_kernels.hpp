@@ -164,29 +164,30 @@ __global__ void __launch_bounds__(kBlock) k_pack(Fields f, Phys ph, int n) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double h = f.hinv[i];
-  double2* P = reinterpret_cast<double2*>(f.pk) + 4*(size_t)i;
+  double2* P = reinterpret_cast<double2*>(f.xh) + 2*(size_t)i;
   P[0] = make_double2(f.x[0][i], D >= 2 ? f.x[1 % D][i] : 0.0);
   P[1] = make_double2(D >= 3 ? f.x[2 % D][i] : 0.0, h);
-  P[2] = make_double2(f.v[0][i], D >= 2 ? f.v[1 % D][i] : 0.0);
-  P[3] = make_double2(D >= 3 ? f.v[2 % D][i] : 0.0, f.m[i]);
+  double2* V = reinterpret_cast<double2*>(f.vmr) + 2*(size_t)i;
+  V[0] = make_double2(f.v[0][i], D >= 2 ? f.v[1 % D][i] : 0.0);
+  V[1] = make_double2(D >= 3 ? f.v[2 % D][i] : 0.0, f.m[i]);
   f.ns[i] = 1.0/max_eigen_iso<D>(h)/ph.nperh;
 }
 
-// Position and h^-1 of node j from its packed record.
+// Position and h^-1 of node j from a 4-double (x, h^-1) record array.
 template <int D>
-__device__ __forceinline__ void load_xh(const double* pk, int j, double* x, double& h) {
-  const double2* P = reinterpret_cast<const double2*>(pk) + 4*(size_t)j;
+__device__ __forceinline__ void load_xh(const double* xh, int j, double* x, double& h) {
+  const double2* P = reinterpret_cast<const double2*>(xh) + 2*(size_t)j;
   const double2 a = P[0], b = P[1];
   x[0] = a.x;
   if (D >= 2) x[1 % D] = a.y;
   if (D >= 3) x[2 % D] = b.x;
   h = b.y;
 }
-// Velocity and mass of node j from its packed record.
+// Velocity and mass of node j from a 4-double (v, m) record array.
 template <int D>
-__device__ __forceinline__ void load_vm(const double* pk, int j, double* v, double& m) {
-  const double2* P = reinterpret_cast<const double2*>(pk) + 4*(size_t)j;
-  const double2 a = P[2], b = P[3];
+__device__ __forceinline__ void load_vm(const double* vm, int j, double* v, double& m) {
+  const double2* P = reinterpret_cast<const double2*>(vm) + 2*(size_t)j;
+  const double2 a = P[0], b = P[1];
   v[0] = a.x;
   if (D >= 2) v[1 % D] = a.y;
   if (D >= 3) v[2 % D] = b.x;
@@ -328,7 +329,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_filter(Fields f, Geo g, Phy
     ell_group<G>(f.cand, ld, k0, i, nc, js, codes);
     double xj[G][D], hj[G];
 #pragma unroll
-    for (int u = 0; u < G; ++u) load_xh<D>(f.pk, js[u], xj[u], hj[u]);
+    for (int u = 0; u < G; ++u) load_xh<D>(f.xh, js[u], xj[u], hj[u]);
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (k0 + u >= nc) continue;
@@ -486,8 +487,8 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int j = js[u];
-        if (DENS || FILT) load_xh<D>(f.pk, j, xj[u], hj[u]);
-        load_vm<D>(f.pk, j, vj[u], mj[u]);
+        if (DENS || FILT) load_xh<D>(f.xh, j, xj[u], hj[u]);
+        load_vm<D>(f.vmr, j, vj[u], mj[u]);
         if (DT) nsj[u] = f.ns[j];
       }
 #pragma unroll
@@ -635,11 +636,44 @@ __global__ void __launch_bounds__(kBlock) k_midstep(Fields f, Phys ph, const Sca
   const double rho = fmin(ph.rho_max, fmax(ph.rho_min, f.rho[i] + hdt*f.DrhoDt[i]));
   const double eps = f.eps[i] + hdt*f.DepsDt[i];
   const double h = fmin(ph.hminInv, fmax(ph.hmaxInv, f.hinv[i] + hdt*f.DHDt[i]));
+  const double cs = sqrt(fmax(0.0, ph.gamma*ph.g1*eps));
   f.rhom[i] = rho;
   f.epsm[i] = eps;
   f.hinvm[i] = h;
   f.Pm[i] = ph.g1*rho*eps;
-  f.csm[i] = sqrt(fmax(0.0, ph.gamma*ph.g1*eps));
+  f.csm[i] = cs;
+  double2* P = reinterpret_cast<double2*>(f.xhm) + 2*(size_t)i;
+  P[0] = make_double2(f.xm[0][i], D >= 2 ? f.xm[1 % D][i] : 0.0);
+  P[1] = make_double2(D >= 3 ? f.xm[2 % D][i] : 0.0, h);
+}
+
+// Per-node pressure term Prho = safeInv(omega)*P/rho^2 of the midpoint state (SPH.cc:425-426),
+// written with the rest of node i's midpoint record (Fields::pkm).
+template <int D>
+__device__ __forceinline__ void write_mid_record(const Fields& f, int i, double om) {
+  const double rho = f.rhom[i];
+  const double prho = (om/(om*om + 1.0e-30))*f.Pm[i]/(rho*rho);
+  f.prhom[i] = prho;
+  double2* P = reinterpret_cast<double2*>(f.pkm) + 4*(size_t)i;
+  P[0] = make_double2(f.vm[0][i], D >= 2 ? f.vm[1 % D][i] : 0.0);
+  P[1] = make_double2(D >= 3 ? f.vm[2 % D][i] : 0.0, f.m[i]);
+  P[2] = make_double2(rho, f.csm[i]);
+  P[3] = make_double2(prho, 0.0);
+}
+
+// The midpoint records (Fields::xhm, Fields::pkm) from the midpoint arrays.  keepOmega: the
+// midpoint grad-h correction is the state's (no grad-h pass: gradhCorrection off, where it
+// stays at its initial 1).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_pack_mid(Fields f, int n, int keepOmega) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2* P = reinterpret_cast<double2*>(f.xhm) + 2*(size_t)i;
+  P[0] = make_double2(f.xm[0][i], D >= 2 ? f.xm[1 % D][i] : 0.0);
+  P[1] = make_double2(D >= 3 ? f.xm[2 % D][i] : 0.0, f.hinvm[i]);
+  double om = f.omegam[i];
+  if (keepOmega) { om = f.omega[i]; f.omegam[i] = om; }
+  write_mid_record<D>(f, i, om);
 }
 
 // EOS policies on the state arrays (PressurePolicy, SoundSpeedPolicy; GammaLawGas.cc:185-245).
@@ -652,29 +686,28 @@ __global__ void __launch_bounds__(kBlock) k_eos(Fields f, Phys ph, int n) {
 
 // ---------------------------------------------------------------------------
 // computeSPHOmegaGradhCorrection (SPH/computeSPHOmegaGradhCorrection.cc:136-204), gather form.
-// runMask (optional): only nodes of runs with a non-zero entry are updated.
+// Positions and h^-1 come from an (x, h^-1) record array.  MID: the midpoint pass, which
+// also writes node i's midpoint record for the SPH pass (write_mid_record).  runMask
+// (optional): only nodes of runs with a non-zero entry are updated.
 // ---------------------------------------------------------------------------
-template <int D, typename R, int G = kGroup>
-__global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, double* const* xs,
-                                                  const double* hs, double* omega, const int* runMask) {
+template <int D, typename R, bool MID, int G = kGroup>
+__global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, const double* xh,
+                                                  double* omega, const int* runMask) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (runMask && !runMask[f.run[i]]) return;
-  double xi[D];
-#pragma unroll
-  for (int a = 0; a < D; ++a) xi[a] = xs[a][i];
-  const R hi = (R)hs[i];
+  double xi[D], hid;
+  load_xh<D>(xh, i, xi, hid);
+  const R hi = (R)hid;
   const R hdeti = hdet_iso<D, R>(hi);
   double wsum = 0.0, gsum = 0.0;
   const int nn = f.cnt[i];
   for (int k0 = 0; k0 < nn; k0 += G) {
     int js[G], codes[G];
     ell_group<G>(f.nbr, (size_t)f.ld, k0, i, nn, js, codes);
-    double xj[G][D];
+    double xj[G][D], hjunk[G];
 #pragma unroll
-    for (int u = 0; u < G; ++u)
-#pragma unroll
-      for (int a = 0; a < D; ++a) xj[u][a] = xs[a][js[u]];
+    for (int u = 0; u < G; ++u) load_xh<D>(xh, js[u], xj[u], hjunk[u]);
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (k0 + u < nn) {
@@ -698,6 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
     om = fmax(1.0e-30, -gsum/(double(D)*wsum));
   }
   omega[i] = om;
+  if (MID) write_mid_record<D>(f, i, om);
 }
 
 // Per-run extents for the next grid: min hinv and the bounding box, as ordered bits.

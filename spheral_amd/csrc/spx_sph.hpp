@@ -20,13 +20,20 @@ struct NbrRaw {
 };
 template <int D>
 __device__ __forceinline__ void load_raw(const Fields& f, int j, NbrRaw<D>& q) {
-#pragma unroll
-  for (int a = 0; a < D; ++a) { q.x[a] = f.xm[a][j]; q.v[a] = f.vm[a][j]; }
-  q.m = f.m[j];
-  q.rho = f.rhom[j];
-  q.cs = f.csm[j];
-  q.h = f.hinvm[j];
-  q.prho = f.prhom[j];
+  const double2* X = reinterpret_cast<const double2*>(f.xhm) + 2*(size_t)j;
+  const double2* P = reinterpret_cast<const double2*>(f.pkm) + 4*(size_t)j;
+  const double2 a = X[0], b = X[1], c = P[0], d = P[1], e = P[2], p = P[3];
+  q.x[0] = a.x;
+  if (D >= 2) q.x[1 % D] = a.y;
+  if (D >= 3) q.x[2 % D] = b.x;
+  q.h = b.y;
+  q.v[0] = c.x;
+  if (D >= 2) q.v[1 % D] = c.y;
+  if (D >= 3) q.v[2 % D] = d.x;
+  q.m = d.y;
+  q.rho = e.x;
+  q.cs = e.y;
+  q.prho = p.x;
 }
 // Image-transformed neighbour record in the pair arithmetic type.
 template <int D, typename R>
@@ -289,8 +296,13 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   const R* pdd = (const R*)f.pdd;       // pair terms in the pair arithmetic; the partition is FP64
   const double ms = f.m[i];
   double v12s[D];
-#pragma unroll
-  for (int a = 0; a < D; ++a) v12s[a] = f.v12[a][i];
+  const double2* PV = reinterpret_cast<const double2*>(f.pkv);
+  {
+    const double2 a = PV[3*(size_t)i], b = PV[3*(size_t)i + 1];
+    v12s[0] = a.x;
+    if (D >= 2) v12s[1 % D] = a.y;
+    if (D >= 3) v12s[2 % D] = b.x;
+  }
   const double de0s = f.DepsDt[i];
   const double epsm = 2.220446049250313e-16;
   double du = 0.0;
@@ -302,10 +314,14 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       const int j = js[u];
+      const double2 a = PV[3*(size_t)j], b = PV[3*(size_t)j + 1], c = PV[3*(size_t)j + 2];
+      vo[u][0] = a.x;
+      if (D >= 2) vo[u][1 % D] = a.y;
+      if (D >= 3) vo[u][2 % D] = b.x;
+      deo[u] = b.y;
+      mo[u] = c.x;
 #pragma unroll
-      for (int a = 0; a < D; ++a) { vo[u][a] = f.v12[a][j]; dd[u][a] = (double)pdd[((size_t)a*cap + k0 + u)*ld + i]; }
-      mo[u] = f.m[j];
-      deo[u] = f.DepsDt[j];
+      for (int q = 0; q < D; ++q) dd[u][q] = (double)pdd[((size_t)q*cap + k0 + u)*ld + i];
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) {
@@ -331,37 +347,43 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   const double eps1 = f.eps[i] + du*dt;
   f.eps[i] = eps1;
   if (ENERGY_ONLY) return;
+  double x1[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    f.x[a][i] = f.x[a][i] + dt*f.DxDt[a][i];
+    x1[a] = f.x[a][i] + dt*f.DxDt[a][i];
+    f.x[a][i] = x1[a];
     double va = f.v[a][i] + dt*f.DvDt[a][i];
     if (ph.gravity) va = va + dt*f.DvDtG[a][i];
     f.v1[a][i] = va;
   }
   const double rho1 = fmin(ph.rho_max, fmax(ph.rho_min, f.rho[i] + dt*f.DrhoDt[i]));
   f.rho[i] = rho1;
-  f.hinv[i] = fmin(ph.hminInv, fmax(ph.hmaxInv, f.Hideal[i]));
+  const double h1 = fmin(ph.hminInv, fmax(ph.hmaxInv, f.Hideal[i]));
+  f.hinv[i] = h1;
+  // end-of-step position and h^-1 for the grad-h pass that follows
+  double2* PK = reinterpret_cast<double2*>(f.xh) + 2*(size_t)i;
+  PK[0] = make_double2(x1[0], x1[1]);
+  PK[1] = make_double2(x1[2], h1);
   f.P[i] = ph.g1*rho1*eps1;
   f.cs[i] = sqrt(fmax(0.0, ph.gamma*ph.g1*eps1));
 }
 
 // Half-step velocity of the compatible-energy partition, v^{1/2} = v + a dt/2 with the
-// midpoint hydro acceleration (SpecificThermalEnergyPolicy.cc:97-99).  hdt < 0: the run's dt/2.
+// midpoint hydro acceleration (SpecificThermalEnergyPolicy.cc:97-99), packed with DepsDt and
+// m (Fields::pkv).  hdt < 0: the run's dt/2.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_v12(Fields f, const Scalars* sc, double hdtFixed, int n) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double hdt = hdtFixed < 0.0 ? 0.5*sc[f.run[i]].dt : hdtFixed;
+  double w[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int a = 0; a < D; ++a) f.v12[a][i] = f.v[a][i] + f.DvDt[a][i]*hdt;
+  for (int a = 0; a < D; ++a) w[a] = f.v[a][i] + f.DvDt[a][i]*hdt;
+  double2* P = reinterpret_cast<double2*>(f.pkv) + 3*(size_t)i;
+  P[0] = make_double2(w[0], w[1]);
+  P[1] = make_double2(w[2], f.DepsDt[i]);
+  P[2] = make_double2(f.m[i], 0.0);
 }
 
-// Per-node pressure term Prho = safeInv(omega)*P/rho^2 of the midpoint state (SPH.cc:425-426).
-__global__ void __launch_bounds__(kBlock) k_prho(Fields f, int n) {
-  const int i = xcd_block()*blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double om = f.omegam[i], rho = f.rhom[i];
-  f.prhom[i] = (om/(om*om + 1.0e-30))*f.Pm[i]/(rho*rho);
-}
 
 }  // namespace spx
