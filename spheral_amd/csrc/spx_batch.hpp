@@ -918,11 +918,21 @@ class Batch {
   }
   void sphPass(bool full) {
     const int nb = (int)xcd_grid(nActive, kBlock);
+    // the kernel table goes to LDS when three blocks per CU still fit (48 KB each)
+    const size_t tabBytes = (size_t)6*(dtab.n1 + 1)*(mixed ? sizeof(float) : sizeof(double));
+    const int useLds = tabBytes <= 48*1024 ? 1 : 0;
+    const size_t shm = useLds ? tabBytes : 0;
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      if (full) launchPrec(k_sph<DD, true, double>, k_sph<DD, true, float>, nb, f, geo, ph, dtab, (int)nActive);
-      else launchPrec(k_sph<DD, false, double>, k_sph<DD, false, float>, nb, f, geo, ph, dtab, (int)nActive);
+      if (full) {
+        if (mixed) hipLaunchKernelGGL((k_sph<DD, true, float>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
+        else hipLaunchKernelGGL((k_sph<DD, true, double>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
+      } else {
+        if (mixed) hipLaunchKernelGGL((k_sph<DD, false, float>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
+        else hipLaunchKernelGGL((k_sph<DD, false, double>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
+      }
     });
+    HIP_CHECK(hipGetLastError());
   }
   // TreeGravity::initialize (tree on the given positions) + evaluateDerivatives + dt vote
   void gravityPass(bool mid) {

@@ -260,6 +260,24 @@ __device__ __forceinline__ void kernel_wgw(const DevTable& t, R eta, R hdet, R& 
   }
 }
 
+// The same lookup on a given copy of the coefficients (global or staged in LDS); graw is the
+// gradient polynomial before the Hdet factor, so |graw| is kernelValueSPH at the same eta.
+template <typename R>
+__device__ __forceinline__ void kernel_wgw_c(const R* coef, const DevTable& t, R eta, R hdet, R& w, R& gw, R& graw) {
+  if (eta < (R)t.extent) {
+    int b = (int)(eta*(R)t.inv_xstep);
+    b = b < t.n1 ? b : t.n1;
+    const R* c = coef + 6*b;
+    w = hdet*(c[0] + (c[1] + c[2]*eta)*eta);
+    graw = c[3] + (c[4] + c[5]*eta)*eta;
+    gw = hdet*graw;
+  } else {
+    w = (R)0;
+    gw = (R)0;
+    graw = (R)0;
+  }
+}
+
 // kernelValueSPH: |dW/deta| at Hdet = 1 (Kernel/TableKernelViewInline.hh:112-120)
 template <typename R = double>
 __device__ __forceinline__ R kernel_wsph(const DevTable& t, R eta) {

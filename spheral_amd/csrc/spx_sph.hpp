@@ -68,12 +68,13 @@ __device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D, R>& s) 
 template <int D, typename R>
 struct PairS {
   R r[D], vso[D], gs[D], go[D], es[D], dd[D];
-  R Ws, Wo, Qs, workQs, etaMagS;
+  R Ws, Wo, Qs, workQs, etaMagS, wsphS;
 };
 
+// coef: the kernel table's coefficients in arithmetic R (global, or the block's LDS copy).
 template <int D, typename R>
-__device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, const Nbr<D, R>& S, const Nbr<D, R>& O,
-                                          bool aFirst, PairS<D, R>& e) {
+__device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, const Phys& ph, const Nbr<D, R>& S,
+                                          const Nbr<D, R>& O, bool aFirst, PairS<D, R>& e) {
   R eo[D], e2s = (R)0, e2o = (R)0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
@@ -88,9 +89,10 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
   const R etaMagO = sqrt(e2o);
   const R tiny = sizeof(R) == sizeof(double) ? (R)1.0e-30 : (R)1.0e-30f;
   const R is = (R)1/fmax(tiny, e.etaMagS), io = (R)1/fmax(tiny, etaMagO);   // safeInvVar
-  R gws, gwo;
-  kernel_wgw<R>(tab, e.etaMagS, hdet_iso<D, R>(S.h), e.Ws, gws);
-  kernel_wgw<R>(tab, etaMagO, hdet_iso<D, R>(O.h), e.Wo, gwo);
+  R gws, gwo, graws, grawo;
+  kernel_wgw_c<R>(coef, tab, e.etaMagS, hdet_iso<D, R>(S.h), e.Ws, gws, graws);
+  kernel_wgw_c<R>(coef, tab, etaMagO, hdet_iso<D, R>(O.h), e.Wo, gwo, grawo);
+  e.wsphS = fabs(graws);                                      // kernelValueSPH(etaMagS)
   const R ss = gws*S.h, so = gwo*O.h;                        // (gW*H)*etaUnit
   R mus = (R)0, muo = (R)0;
 #pragma unroll
@@ -127,8 +129,19 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const Phys& ph, c
 // FULL also produces the outputs the integration does not consume (sum density,
 // normalisation, effective Q, density gradient, first moment, M).
 // ---------------------------------------------------------------------------
+// ldsTable != 0: the table's coefficients are staged in the block's dynamic LDS first (the
+// pair loop makes three lookups per pair at scattered intervals).
 template <int D, bool FULL, typename R>
-__global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n) {
+__global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable) {
+  extern __shared__ __align__(16) unsigned char spx_lds[];
+  const R* coef = table_coef<R>(tab);
+  if (ldsTable) {
+    R* t = reinterpret_cast<R*>(spx_lds);
+    const int nc = 6*(tab.n1 + 1);
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) t[k] = coef[k];
+    __syncthreads();
+    coef = t;
+  }
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
@@ -175,7 +188,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
     Nbr<D, R> O;
     finish_mid<D, R>(g, code, rc, O);
     PairS<D, R> e;
-    pair_eval<D, R>(tab, ph, S, O, code != 0 || i < j, e);
+    pair_eval<D, R>(tab, coef, ph, S, O, code != 0 || i < j, e);
 #pragma unroll
     for (int c = 0; c < D; ++c) pdd[((size_t)c*cap + k)*ld + i] = e.dd[c];
     const double mo = rc.m;
@@ -192,7 +205,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
     }
     depsdt += mo*(double)(S.prho*vdg + e.workQs);
     maxq = fmax(maxq, (double)e.Qs);
-    const R wsph = kernel_wsph<R>(tab, e.etaMagS);
+    const R wsph = e.wsphS;
     m0 += (double)wsph;
     if (FULL) {
       const double orho = rc.rho;
