@@ -38,6 +38,9 @@ namespace spx {
 
 static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1)/kBlock); }
 
+// Largest kernel table a pair pass stages in LDS (three 256-thread blocks per CU still fit).
+constexpr size_t kLdsTableMax = 48*1024;
+
 // Particles per block of the extent reduction.
 constexpr int64_t kExtentTile = 8192;
 
@@ -823,8 +826,9 @@ class Batch {
         hipLaunchKernelGGL(k_reset_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
         launchD([&](auto d) {
           constexpr int DD = decltype(d)::value;
-          launchPrec(k_density_dt<DD, true, true, double, kGroup, true>, k_density_dt<DD, true, true, float, kGroup, true>,
-                     (int)nbx, f, geo, ph, dtab, (int)nActive, dsc, dOverflow);
+          launchPrecTable(k_density_dt<DD, true, true, double, kGroup, true>,
+                          k_density_dt<DD, true, true, float, kGroup, true>, (int)nbx, f, geo, ph, dtab, (int)nActive, dsc,
+                          dOverflow);
         });
       } else {
         launchD([&](auto d) {
@@ -863,6 +867,18 @@ class Batch {
     else hipLaunchKernelGGL(k64, dim3(nb), dim3(kBlock), 0, stream, args...);
     HIP_CHECK(hipGetLastError());
   }
+  // As launchPrec for pair passes that stage the kernel table in LDS (their last argument
+  // is the staging flag).
+  template <typename K64, typename K32, typename... Args>
+  void launchPrecTable(K64 k64, K32 k32, int nb, Args... args) {
+    const size_t bytes = tableBytes();
+    const int lds = bytes <= kLdsTableMax ? 1 : 0;
+    const size_t shm = lds ? bytes : 0;
+    if (mixed) hipLaunchKernelGGL(k32, dim3(nb), dim3(kBlock), shm, stream, args..., lds);
+    else hipLaunchKernelGGL(k64, dim3(nb), dim3(kBlock), shm, stream, args..., lds);
+    HIP_CHECK(hipGetLastError());
+  }
+  size_t tableBytes() const { return (size_t)6*(dtab.n1 + 1)*(mixed ? sizeof(float) : sizeof(double)); }
 
   void eos() {
     if (nActive == 0) return;
@@ -881,11 +897,11 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (mid)
-        launchPrec(k_omega<DD, double, true>, k_omega<DD, float, true>, nb, f, geo, dtab, (int)nActive,
-                   (const double*)f.xhm, f.omegam, runMask);
+        launchPrecTable(k_omega<DD, double, true>, k_omega<DD, float, true>, nb, f, geo, dtab, (int)nActive,
+                        (const double*)f.xhm, f.omegam, runMask);
       else
-        launchPrec(k_omega<DD, double, false>, k_omega<DD, float, false>, nb, f, geo, dtab, (int)nActive,
-                   (const double*)f.xh, f.omega, runMask);
+        launchPrecTable(k_omega<DD, double, false>, k_omega<DD, float, false>, nb, f, geo, dtab, (int)nActive,
+                        (const double*)f.xh, f.omega, runMask);
     });
   }
   // Packed neighbour records and nodeScale of the current state (active runs).
@@ -906,21 +922,20 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (dens && dtv)
-        launchPrec(k_density_dt<DD, true, true, double, kGroup>, k_density_dt<DD, true, true, float, kGroup>, nb, f, geo, ph,
-                   dtab, (int)nActive, dsc, (int*)nullptr);
+        launchPrecTable(k_density_dt<DD, true, true, double, kGroup>, k_density_dt<DD, true, true, float, kGroup>, nb, f,
+                        geo, ph, dtab, (int)nActive, dsc, (int*)nullptr);
       else if (dens)
-        launchPrec(k_density_dt<DD, true, false, double, kGroup>, k_density_dt<DD, true, false, float, kGroup>, nb, f, geo,
-                   ph, dtab, (int)nActive, dsc, (int*)nullptr);
+        launchPrecTable(k_density_dt<DD, true, false, double, kGroup>, k_density_dt<DD, true, false, float, kGroup>, nb,
+                        f, geo, ph, dtab, (int)nActive, dsc, (int*)nullptr);
       else
-        launchPrec(k_density_dt<DD, false, true, double, kGroup>, k_density_dt<DD, false, true, float, kGroup>, nb, f, geo,
-                   ph, dtab, (int)nActive, dsc, (int*)nullptr);
+        launchPrecTable(k_density_dt<DD, false, true, double, kGroup>, k_density_dt<DD, false, true, float, kGroup>, nb,
+                        f, geo, ph, dtab, (int)nActive, dsc, (int*)nullptr);
     });
   }
   void sphPass(bool full) {
     const int nb = (int)xcd_grid(nActive, kBlock);
-    // the kernel table goes to LDS when three blocks per CU still fit (48 KB each)
-    const size_t tabBytes = (size_t)6*(dtab.n1 + 1)*(mixed ? sizeof(float) : sizeof(double));
-    const int useLds = tabBytes <= 48*1024 ? 1 : 0;
+    const size_t tabBytes = tableBytes();
+    const int useLds = tabBytes <= kLdsTableMax ? 1 : 0;
     const size_t shm = useLds ? tabBytes : 0;
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;

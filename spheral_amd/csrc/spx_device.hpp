@@ -278,6 +278,20 @@ __device__ __forceinline__ void kernel_wgw_c(const R* coef, const DevTable& t, R
   }
 }
 
+// The kernel table's coefficients for a pair pass: staged in the block's dynamic LDS when
+// lds != 0 (every thread of the block must call this before any returns), else global.
+template <typename R>
+__device__ __forceinline__ const R* stage_table(const DevTable& tab, int lds) {
+  extern __shared__ __align__(16) unsigned char spx_lds[];
+  const R* coef = table_coef<R>(tab);
+  if (!lds) return coef;
+  R* t = reinterpret_cast<R*>(spx_lds);
+  const int nc = 6*(tab.n1 + 1);
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) t[k] = coef[k];
+  __syncthreads();
+  return t;
+}
+
 // kernelValueSPH: |dW/deta| at Hdet = 1 (Kernel/TableKernelViewInline.hh:112-120)
 template <typename R = double>
 __device__ __forceinline__ R kernel_wsph(const DevTable& t, R eta) {

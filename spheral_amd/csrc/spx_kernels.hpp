@@ -456,7 +456,8 @@ __global__ void __launch_bounds__(kBlock) k_remap_candidates(const uint32_t* src
 // ---------------------------------------------------------------------------
 template <int D, bool DENS, bool DT, typename R, int G, bool FILT = false>
 __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc,
-                                                       int* overflow) {
+                                                       int* overflow, int ldsTable) {
+  const R* coef = stage_table<R>(tab, DENS ? ldsTable : 0);
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   double dtmin = 1.7976931348623157e308;
   const bool valid = i < n;
@@ -470,8 +471,8 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
     double rho = 0.0;
     if (DENS) {
       const R hi = (R)f.hinv[i];
-      R w, gw;
-      kernel_wgw<R>(tab, (R)0, hdet_iso<D, R>(hi), w, gw);      // self term W(|Hi ri - Hi ri|) = W(0)
+      R w, gw, graw;
+      kernel_wgw_c<R>(coef, tab, (R)0, hdet_iso<D, R>(hi), w, gw, graw);   // self term W(|Hi ri - Hi ri|) = W(0)
       rho = f.m[i]*(double)w;
     }
     double vmax2 = 0.0;       // pairwise limit, tracked as max of |v_ij|^2 / min(ns)^2
@@ -534,8 +535,8 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
 #pragma unroll
             for (int a = 0; a < D; ++a) { const R d = (R)hj[u]*(R)(xi[a] - xj[u][a]); s += d*d; }
           }
-          R w, gw;
-          kernel_wgw<R>(tab, sqrt(s), hdet_iso<D, R>((R)hj[u]), w, gw);
+          R w, gw, graw;
+          kernel_wgw_c<R>(coef, tab, sqrt(s), hdet_iso<D, R>((R)hj[u]), w, gw, graw);
           rho += mj[u]*(double)w;
         }
         if (DT) {
@@ -692,7 +693,8 @@ __global__ void __launch_bounds__(kBlock) k_eos(Fields f, Phys ph, int n) {
 // ---------------------------------------------------------------------------
 template <int D, typename R, bool MID, int G = kGroup>
 __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, const double* xh,
-                                                  double* omega, const int* runMask) {
+                                                  double* omega, const int* runMask, int ldsTable) {
+  const R* coef = stage_table<R>(tab, ldsTable);
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (runMask && !runMask[f.run[i]]) return;
@@ -716,8 +718,8 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
 #pragma unroll
         for (int a = 0; a < D; ++a) { const R rr = hi*(R)(xi[a] - xj[u][a]); s += rr*rr; }
         const R eta = sqrt(s);
-        R w, gw;
-        kernel_wgw<R>(tab, eta, hdeti, w, gw);
+        R w, gw, graw;
+        kernel_wgw_c<R>(coef, tab, eta, hdeti, w, gw, graw);
         wsum += (double)w;
         gsum += (double)(eta*gw);
       }

@@ -133,15 +133,7 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
 // pair loop makes three lookups per pair at scattered intervals).
 template <int D, bool FULL, typename R>
 __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable) {
-  extern __shared__ __align__(16) unsigned char spx_lds[];
-  const R* coef = table_coef<R>(tab);
-  if (ldsTable) {
-    R* t = reinterpret_cast<R*>(spx_lds);
-    const int nc = 6*(tab.n1 + 1);
-    for (int k = threadIdx.x; k < nc; k += blockDim.x) t[k] = coef[k];
-    __syncthreads();
-    coef = t;
-  }
+  const R* coef = stage_table<R>(tab, ldsTable);
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
