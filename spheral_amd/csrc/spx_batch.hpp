@@ -14,7 +14,7 @@
 //
 // Neighbour lists are filtered every step from per-run candidate lists (k_candidates,
 // k_neighbor_filter).  A run re-sorts and rebuilds its candidates only when its own nodes
-// have moved or grown enough that the list might miss a pair (k_skin_check), so the
+// have moved or grown enough that the list might miss a pair (measured in k_full_update), so the
 // rebuild schedule of a run, and with it every bit of its results, does not depend on
 // the other runs of the batch.
 #pragma once
@@ -84,7 +84,7 @@ class Batch {
     bool needInit = false;
     bool stepOut = false;   // "new H", "mass zeroth moment" of this run's last evaluation are in place
     bool fullOut = false;   // the evaluate-only outputs (sum density, M, ...) are in place
-    bool candValid = false; // the run's candidate list covers its nodes (see k_skin_check)
+    bool candValid = false; // the run's candidate list covers its nodes (see k_candidates)
     double skin = kSkinInit;
     long long builtCycle = 0;   // the run's step count when its candidate list was built
     int64_t builds = 0;         // candidate-list builds of the run
@@ -262,6 +262,7 @@ class Batch {
     ph.Cq = cfg.Cq;
     ph.cfl = cfg.cfl;
     ph.nperh = cfg.nperh;
+    ph.ext = cfg.neighbor_extent;
     ph.ext2 = cfg.neighbor_extent*cfg.neighbor_extent;
     ph.hmin = cfg.hmin;
     ph.hmax = cfg.hmax;
@@ -1077,8 +1078,8 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)dsc, -1.0, (int)nActive);
-      launchPrec(k_full_update<DD, false, double, kGroup>, k_full_update<DD, false, float, kGroup>, nb, f, geo, ph,
-                 (const Scalars*)dsc, 0.0, (int)nActive);
+      launchPrec(k_full_update<DD, false, double, kGroupUpdate>, k_full_update<DD, false, float, kGroupUpdate>, nb, f, geo,
+                 ph, dsc, 0.0, (int)nActive);
     });
     for (int a = 0; a < D; ++a) {
       std::swap(f.v[a], f.v1[a]);
@@ -1090,12 +1091,6 @@ class Batch {
     mark("energy_update");
     omegaPass(false);
     mark("omega");
-    launchD([&](auto d) {
-      constexpr int DD = decltype(d)::value;
-      hipLaunchKernelGGL(k_skin_check<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, geo, cfg.neighbor_extent,
-                         (int)nActive, dsc);
-    });
-    HIP_CHECK(hipGetLastError());
     wrapPending = true;
     hipLaunchKernelGGL(k_advance_time, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
     HIP_CHECK(hipGetLastError());
@@ -1155,8 +1150,8 @@ class Batch {
       constexpr int DD = decltype(d)::value;
       hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)dsc,
                          0.5*multiplier, (int)nActive);
-      launchPrec(k_full_update<DD, true, double, kGroup>, k_full_update<DD, true, float, kGroup>, (int)xcd_grid(nActive, kBlock), f, geo, ph,
-                 (const Scalars*)dsc, multiplier, (int)nActive);
+      launchPrec(k_full_update<DD, true, double, kGroupUpdate>, k_full_update<DD, true, float, kGroupUpdate>,
+                 (int)xcd_grid(nActive, kBlock), f, geo, ph, dsc, multiplier, (int)nActive);
     });
     eos();
     HIP_CHECK(hipStreamSynchronize(stream));

@@ -25,16 +25,22 @@ __host__ __device__ __forceinline__ size_t ell_at(int k, int i, size_t ld) {
   return (((size_t)(k >> 3)*ld + (size_t)i) << 3) + (size_t)(k & 7);
 }
 
-// Neighbour entries are consumed in groups of kGroup: one 16-byte load of the blocked list,
-// then every neighbour read of the group is issued before any is used, so a wavefront waits
-// on memory once per group instead of twice per neighbour.  Positions past the end of the list
+// Neighbour entries are consumed in groups of G (4 or 8): one or two 16-byte loads of the
+// blocked list, then every neighbour read of the group is issued before any is used, so a
+// wavefront waits on memory once per group instead of twice per neighbour.  Positions past the end of the list
 // (k0 + u >= nn) become the lane's own slot with no image, and are skipped by the caller.
 constexpr int kGroup = 4;
+// The compatible-energy update does little arithmetic per neighbour; it keeps 8 in flight.
+constexpr int kGroupUpdate = 8;
 template <int G>
 __device__ __forceinline__ void ell_group(const uint32_t* nbr, size_t ld, int k0, int i, int nn, int* js, int* codes) {
   uint32_t e[G];
   const uint32_t* p = nbr + ell_at(k0, i, ld);
-  if (G == 4) {
+  if (G == 8) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p), r = *reinterpret_cast<const uint4*>(p + 4);
+    e[0] = q.x; e[1 % G] = q.y; e[2 % G] = q.z; e[3 % G] = q.w;
+    e[4 % G] = r.x; e[5 % G] = r.y; e[6 % G] = r.z; e[7 % G] = r.w;
+  } else if (G == 4) {
     const uint4 q = *reinterpret_cast<const uint4*>(p);
     e[0] = q.x; e[1 % G] = q.y; e[2 % G] = q.z; e[3 % G] = q.w;
   } else if (G == 2) {
@@ -105,7 +111,7 @@ struct RunGrid {
 
 // Physics constants of the run.
 struct Phys {
-  double gamma, g1, Cl, Cq, cfl, nperh, ext2, hmin, hmax, rho_min, rho_max;
+  double gamma, g1, Cl, Cq, cfl, nperh, ext, ext2, hmin, hmax, rho_min, rho_max;
   double hminInv, hmaxInv;
   int correctVG, useVmagDt, gravity;
 };

@@ -215,7 +215,7 @@ __device__ __forceinline__ void flush_chunk(uint32_t* nbr, int k, int i, size_t 
 // step.  The candidates stay complete while no pair can have closed in from beyond the
 // widened radius: with D = max |x - xb| hb/ext and G = max hb/h^-1 over the run's nodes,
 // |rij(now)| <= max(Si, Sj)(now) implies |rij(built)| <= max(Si, Sj)(built) (G + 2D), so a
-// list is valid while G + 2D <= 1 + skin (k_skin_check measures both after every step).
+// list is valid while G + 2D <= 1 + skin (the full update measures both after every step).
 // Reflecting images are isometries of the node, and periodic images are re-chosen as the
 // nearest copy each step (the box spans >= 3 widened supports), so both obey the bound.
 // ---------------------------------------------------------------------------
@@ -386,36 +386,6 @@ __device__ __forceinline__ void run_max_bits(unsigned long long* word, size_t st
     unsigned long long* w = word + (size_t)r*stride;
     if (b > __atomic_load_n(w, __ATOMIC_RELAXED)) atomicMax(w, b);
   }
-}
-
-// Displacement and support growth of every active node since its run's candidate list was
-// built (end-of-step positions, before boundary enforcement, which can only shorten them).
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_skin_check(Fields f, Geo g, double ext, int n, Scalars* sc) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  int r = 0;
-  double dmax = 0.0, gmax = 0.0;
-  if (valid) {
-    r = f.run[i];
-    double d2 = 0.0;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      double d = f.x[a][i] - f.xb[a][i];
-      if ((g.permask >> a) & 1) {
-        const double L = g.xmax[a] - g.xmin[a];
-        d = fabs(d);
-        d = fmin(d, fabs(L - d));
-      }
-      d2 += d*d;
-    }
-    const double hb = f.hb[i];
-    dmax = sqrt(d2)*hb/ext;
-    gmax = hb/f.hinv[i];
-  }
-  const size_t stride = sizeof(Scalars)/sizeof(unsigned long long);
-  run_max_bits(&sc[0].skinD, stride, r, ord_bits(dmax), valid);
-  run_max_bits(&sc[0].skinG, stride, r, ord_bits(gmax), valid);
 }
 
 // Moves the candidate lists of runs whose particle segment shifted in a reorder (a run

@@ -289,13 +289,18 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 // (Hydro/SpecificThermalEnergyPolicy.cc:79-144) on the pair accelerations the SPH
 // pass stored per ELL entry (each already in canonical summation order), then the
 // increment and replace policies and the EOS.  v0 is read from f.v, the new velocity
-// goes to f.v1.  ENERGY_ONLY: the package-level policy call (eps only).
+// goes to f.v1.  The step variant also measures, per run, how far nodes have moved and h
+// grown since the candidate list was built (see k_candidates).  ENERGY_ONLY: the
+// package-level policy call (eps only).
 // ---------------------------------------------------------------------------
 template <int D, bool ENERGY_ONLY, typename R, int G>
-__global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, const Scalars* sc, double multiplier,
+__global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, Scalars* sc, double multiplier,
                                                         int n) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const bool valid = i < n;
+  int skr = 0;                       // candidate-list check (k_skin_check's measure, fused)
+  double skd = 0.0, skg = 0.0;
+  if (valid) {
   const double dt = ENERGY_ONLY ? multiplier : sc[f.run[i]].dt;
   const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
   const R* pdd = (const R*)f.pdd;       // pair terms in the pair arithmetic; the partition is FP64
@@ -369,8 +374,30 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   double2* PK = reinterpret_cast<double2*>(f.xh) + 2*(size_t)i;
   PK[0] = make_double2(x1[0], x1[1]);
   PK[1] = make_double2(x1[2], h1);
+  // displacement and support growth since the run's candidate list was built
+  skr = f.run[i];
+  double d2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double d = x1[a] - f.xb[a][i];
+    if ((g.permask >> a) & 1) {
+      const double L = g.xmax[a] - g.xmin[a];
+      d = fabs(d);
+      d = fmin(d, fabs(L - d));
+    }
+    d2 += d*d;
+  }
+  const double hb = f.hb[i];
+  skd = sqrt(d2)*hb/ph.ext;
+  skg = hb/h1;
   f.P[i] = ph.g1*rho1*eps1;
   f.cs[i] = sqrt(fmax(0.0, ph.gamma*ph.g1*eps1));
+  }
+  if (!ENERGY_ONLY) {
+    const size_t stride = sizeof(Scalars)/sizeof(unsigned long long);
+    run_max_bits(&sc[0].skinD, stride, skr, ord_bits(skd), valid);
+    run_max_bits(&sc[0].skinG, stride, skr, ord_bits(skg), valid);
+  }
 }
 
 // Half-step velocity of the compatible-energy partition, v^{1/2} = v + a dt/2 with the
