@@ -200,8 +200,9 @@ int spx_run_compatible_energy(spx_run* run, double multiplier);
    out[0]=mass, out[1..3]=linear momentum, out[4]=kinetic, out[5]=thermal,
    out[6]=gravitational extra energy (0.5 sum m phi). */
 int spx_run_conservation(spx_run* run, double* out7);
-/* Device time of the last step's kernels, per kernel class (ms), measured with
-   HIP events on the run's stream.  names: comma-separated labels written to buf. */
+/* Device time per kernel class (ms), measured with HIP events on the run's stream and
+   summed over the steps since timing was enabled; max_entries < 0 enables timing and
+   clears the sums.  names: comma-separated labels written to buf. */
 int spx_run_kernel_times(spx_run* run, double* ms, int32_t max_entries,
                          int32_t* n_entries, char* names, int64_t names_cap);
 /* Unique pairs of the current connectivity (NodePairList size). */
@@ -259,7 +260,7 @@ int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7);
 int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs);
 /* As spx_run_neighbor_builds for one run of a batch. */
 int spx_batch_neighbor_builds(spx_batch* batch, int32_t run, int64_t* builds);
-/* As spx_run_kernel_times for the batch's last step (max_entries < 0 enables timing). */
+/* As spx_run_kernel_times for the batch's steps. */
 int spx_batch_kernel_times(spx_batch* batch, double* ms, int32_t max_entries,
                            int32_t* n_entries, char* names, int64_t names_cap);
 

@@ -149,11 +149,13 @@ class Batch {
   int64_t candBuilds = 0;   // candidate-list builds, summed over runs
   int64_t npairsCache = -1;
   bool mixed = false;
-  // per-class kernel time of the last step (ms)
+  // per-class kernel time (ms) summed over the steps since timing was enabled
   bool timing = false;
   std::vector<std::string> tnames;
   std::vector<double> tms;
-  std::vector<hipEvent_t> evs;
+  std::vector<hipEvent_t> evPool;       // reused across steps (recording costs no allocation)
+  std::vector<const char*> evNames;
+  size_t evCount = 0;
 
   Batch(const spx_config& c, int64_t capacity, int nruns, int dev, bool retireDone)
       : cfg(c), D(c.dim), device(dev), cap(capacity), maxRuns(nruns), retire(retireDone) {
@@ -174,7 +176,7 @@ class Batch {
   ~Batch() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    for (auto e : evs) (void)hipEventDestroy(e);
+    for (auto e : evPool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow,
                     (void*)htiles, (void*)hbuild, (void*)hdelta})
@@ -977,28 +979,28 @@ class Batch {
 
   void mark(const char* name) {
     if (!timing) return;
-    hipEvent_t e;
-    HIP_CHECK(hipEventCreate(&e));
-    HIP_CHECK(hipEventRecord(e, stream));
-    evs.push_back(e);
-    tnames.push_back(name);
+    if (evCount == evPool.size()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      evPool.push_back(e);
+      evNames.push_back(nullptr);
+    }
+    HIP_CHECK(hipEventRecord(evPool[evCount], stream));
+    evNames[evCount] = name;
+    ++evCount;
   }
   void collectTimes() {
-    if (!timing || evs.empty()) return;
-    HIP_CHECK(hipEventSynchronize(evs.back()));
-    std::vector<std::string> names;
-    std::vector<double> ms;
-    for (size_t k = 1; k < evs.size(); ++k) {
+    if (!timing || evCount == 0) return;
+    HIP_CHECK(hipEventSynchronize(evPool[evCount - 1]));
+    // summed per class over the steps since timing was enabled
+    for (size_t k = 1; k < evCount; ++k) {
       float v = 0.f;
-      HIP_CHECK(hipEventElapsedTime(&v, evs[k - 1], evs[k]));
-      auto it = std::find(names.begin(), names.end(), tnames[k]);
-      if (it == names.end()) { names.push_back(tnames[k]); ms.push_back(v); }
-      else ms[it - names.begin()] += v;
+      HIP_CHECK(hipEventElapsedTime(&v, evPool[k - 1], evPool[k]));
+      auto it = std::find(tnames.begin(), tnames.end(), std::string(evNames[k]));
+      if (it == tnames.end()) { tnames.push_back(evNames[k]); tms.push_back(v); }
+      else tms[it - tnames.begin()] += v;
     }
-    for (auto e : evs) (void)hipEventDestroy(e);
-    evs.clear();
-    tnames = names;
-    tms = ms;
+    evCount = 0;
   }
 
   // ---------------------------------------------------------------- lifecycle
@@ -1043,9 +1045,7 @@ class Batch {
   void step() {
     if (!retire && !initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called before stepping");
     if (timing) {
-      for (auto e : evs) (void)hipEventDestroy(e);
-      evs.clear();
-      tnames.clear();
+      evCount = 0;
       mark("start");
     }
     const bool fresh = prepareNewRuns();

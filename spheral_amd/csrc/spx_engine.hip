@@ -140,7 +140,7 @@ static void check_device(int32_t device) {
 
 static void kernel_times(spx::Batch& B, double* ms, int32_t max_entries, int32_t* n_entries, char* names,
                          int64_t names_cap) {
-  if (max_entries < 0) { B.timing = true; return; }   // enable timing
+  if (max_entries < 0) { B.timing = true; B.tnames.clear(); B.tms.clear(); return; }   // enable timing, clear sums
   const int32_t k = (int32_t)std::min<size_t>(B.tms.size(), (size_t)max_entries);
   std::string all;
   for (int32_t i = 0; i < k; ++i) { if (ms) ms[i] = B.tms[i]; all += (i ? "," : "") + B.tnames[i]; }
