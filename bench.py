@@ -381,7 +381,9 @@ def main():
         "roofline": {
             "kernel": "k_sph (SPH + SPHSmoothingScale pair pass)",
             "bound": "mfma",
-            "pipe": "VALU FP64 (no MFMA: no dense contraction); peak = MI355X FP64 78.6 TFLOP/s",
+            "pipe": ("VALU FP64 (no MFMA: no dense contraction); peak = MI355X FP64 78.6 TFLOP/s"
+                     if prec == abi.SPX_PREC_FP64 else
+                     "VALU FP32 pair arithmetic, FP64 accumulation (no MFMA); peak = MI355X FP32 157.3 TFLOP/s"),
             "achieved": achieved/1e12,
             "peak": peak/1e12,
             "unit": "TFLOP/s",
