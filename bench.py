@@ -167,6 +167,7 @@ def evrard(args):
             print(f"[bench] evrard {name}: step {k} t {t:.4f}", file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         out[name] = {"seconds": wall, "steps": k, "t": t, "particle_steps_per_s": n*k/wall,
+                     "builds": r.neighbor_builds(),
                      "fields": {f: r.field(f) for f in ("mass density", "specific thermal energy")}}
         r.close()
     m, d = out["mixed"], out["fp64"]
@@ -179,7 +180,7 @@ def evrard(args):
         "config": {"workload": f"evrard_3d_{args.particles}_to_t{t_end}", "particles": args.particles, "t_end": t_end,
                    "opening": 0.5, "softening": 0.01, "kernel": "BSpline TableKernel(1000), nPerh 1.51"},
         "t_reached": m["t"], "fp64_particle_steps_per_s": d["particle_steps_per_s"], "fp64_steps": d["steps"],
-        "l1_mixed_vs_fp64": l1,
+        "l1_mixed_vs_fp64": l1, "neighbor_list_builds": {"mixed": m["builds"], "fp64": d["builds"]},
     }
     print(json.dumps(line), flush=True)
 
