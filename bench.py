@@ -97,10 +97,13 @@ def ensemble(args):
     t0 = time.perf_counter()
     b = engine.Batch(cfg_m, capacity=total, max_runs=args.runs)
     ids = [b.add_run(s, args.t_end) for s in states]
+    t_admit = time.perf_counter() - t0
+    b.enable_timing()
     bsteps = 0
     while b.step() > 0:
         bsteps += 1
     t_batch = time.perf_counter() - t0
+    per_class = {k: v/max(bsteps, 1) for k, v in b.kernel_times().items()}
     cycles = np.array([b.status(r)["cycle"] for r in ids])
     psteps = int((cycles*sizes).sum())
     print(f"[bench] batched: {t_batch:.2f} s, {bsteps} batch steps", file=sys.stderr, flush=True)
@@ -121,6 +124,9 @@ def ensemble(args):
                    "runs": args.runs, "particles": total, "seed": args.seed, "t_end": args.t_end,
                    "kernel": "BSpline TableKernel(1000), nPerh 2", "boundary": "periodic unit square"},
         "batched_seconds": t_batch,
+        "admission_seconds": t_admit,
+        "kernel_ms_per_batch_step": per_class,
+        "neighbor_list_builds": int(sum(b.neighbor_builds(r) for r in ids)),
         "particle_steps": psteps,
         "particle_steps_per_s": psteps/t_batch,
     }

@@ -114,6 +114,9 @@ class Batch {
   int* dinit = nullptr;
   int* dOverflow = nullptr;
   int* dbuild = nullptr;      // per run: 1 if it builds its candidate list this step
+  int* dsub = nullptr;        // per run: segment start, compacted start, segment index
+  uint64_t *skeys = nullptr, *skeysAlt = nullptr;   // compacted sort of the rebuilding runs
+  uint32_t *spos = nullptr, *sposAlt = nullptr;
   int* ddelta = nullptr;      // per run: shift of its segment in a reorder (k_remap_candidates)
   uint32_t* candAlt = nullptr;
   int32_t* ccntAlt = nullptr;
@@ -140,6 +143,7 @@ class Batch {
   int* hOverflow = nullptr;
   int* hbuild = nullptr;
   int* hdelta = nullptr;
+  int* hsub = nullptr;
   int ld = 0;
   int mortonBits = 0, runBits = 0;
   int64_t nActive = 0;    // particles of active runs (front of the arrays after a sort)
@@ -179,7 +183,7 @@ class Batch {
     for (auto e : evPool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow,
-                    (void*)htiles, (void*)hbuild, (void*)hdelta})
+                    (void*)htiles, (void*)hbuild, (void*)hdelta, (void*)hsub})
       if (p) (void)hipHostFree(p);
     grav.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -362,6 +366,11 @@ class Batch {
     dinit = dalloc<int>(maxRuns);
     dOverflow = dalloc<int>(1);
     dbuild = dalloc<int>(maxRuns);
+    dsub = dalloc<int>(3*(size_t)maxRuns);
+    skeys = dalloc<uint64_t>(ld);
+    skeysAlt = dalloc<uint64_t>(ld);
+    spos = dalloc<uint32_t>(ld);
+    sposAlt = dalloc<uint32_t>(ld);
     ddelta = dalloc<int>(maxRuns);
     hgrids = halloc<RunGrid>(maxRuns);
     hsc = halloc<Scalars>(maxRuns);
@@ -370,6 +379,7 @@ class Batch {
     hinit = halloc<int>(maxRuns);
     hOverflow = halloc<int>(1);
     hbuild = halloc<int>(maxRuns);
+    hsub = halloc<int>(3*(size_t)maxRuns);
     hdelta = halloc<int>(maxRuns);
     tilesAlloc = cap/kExtentTile + maxRuns + 1;
     dtiles = dalloc<int>(3*tilesAlloc);
@@ -745,6 +755,78 @@ class Batch {
     }
   }
 
+  // Re-sort only the runs that rebuild (hbuild), each within its own segment, and gather
+  // only their particles; the other runs, their candidate lists and every offset stay where
+  // they are.  For steps on which no run was added, finished or removed.  The rebuilding
+  // runs' particles are compacted with keys [segment | cell Morton code], sorted with one
+  // device-wide radix sort, and the k-th sorted particle goes to the k-th of their slots.
+  void sortRebuildingRuns(int bitsPerAxis) {
+    mortonBits = D*bitsPerAxis;
+    if (mortonBits + runBits + 2 > 64) raise(SPX_ERR_CAPACITY, "sort key exceeds 64 bits");
+    for (int r = 0; r < maxRuns; ++r) hstate[r] = runs[r].state;
+    HIP_CHECK(hipMemcpyAsync(dstate, hstate, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_keys<DD>, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)dgrids,
+                         (const int*)dstate, (const int*)dbuild, (int)nTotal, wrapPending ? 1 : 0, 1, mortonBits, runBits,
+                         keys, vals);
+    });
+    HIP_CHECK(hipGetLastError());
+    wrapPending = false;
+    int nseg = 0;
+    int64_t nsub = 0;
+    for (int r = 0; r < maxRuns; ++r) {
+      hsub[3*r] = (int)runs[r].offset;
+      hsub[3*r + 1] = (int)nsub;
+      hsub[3*r + 2] = nseg;
+      if (!hbuild[r]) continue;
+      nsub += runs[r].n;
+      ++nseg;
+    }
+    if (nseg == 0) return;
+    int segBits = 0;
+    while ((1 << segBits) < nseg) ++segBits;
+    if (mortonBits + segBits > 64) raise(SPX_ERR_CAPACITY, "sort key exceeds 64 bits");
+    HIP_CHECK(hipMemcpyAsync(dsub, hsub, sizeof(int)*3*maxRuns, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_subset_keys, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, (const uint64_t*)keys,
+                       (const int32_t*)f.run, (const int*)dbuild, (const int*)dsub, mortonBits, (int)nActive, skeys, spos);
+    HIP_CHECK(hipGetLastError());
+    const int endBit = std::max(1, mortonBits + segBits);
+    size_t tmp = 0;
+    HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, skeys, skeysAlt, spos, sposAlt, (size_t)nsub, 0, endBit, stream));
+    if (tmp > sortTmpBytes) {
+      HIP_CHECK(hipStreamSynchronize(stream));
+      dfree(sortTmp);
+      sortTmp = dalloc<char>(tmp);
+      sortTmpBytes = tmp;
+    }
+    tmp = sortTmpBytes;
+    HIP_CHECK(rocprim::radix_sort_pairs(sortTmp, tmp, skeys, skeysAlt, spos, sposAlt, (size_t)nsub, 0, endBit, stream));
+    // gather into the spare arrays at the compacted slots, then copy those slots back
+    auto src = carried(f);
+    auto dst = carried(alt);
+    GatherSet there{}, back{};
+    for (size_t k = 0; k < src.size(); ++k) {
+      there.src[there.count] = *src[k]; there.dst[there.count++] = *dst[k];
+      back.src[back.count] = *dst[k]; back.dst[back.count++] = *src[k];
+    }
+    for (int c = 0; c < D*D; ++c) {
+      there.src[there.count] = f.DvDx + (size_t)c*ld; there.dst[there.count++] = alt.DvDx + (size_t)c*ld;
+      back.src[back.count] = alt.DvDx + (size_t)c*ld; back.dst[back.count++] = f.DvDx + (size_t)c*ld;
+    }
+    const int nb = nblocks(nsub);
+    hipLaunchKernelGGL(k_gather_list, dim3(nb), dim3(kBlock), 0, stream, there, (const uint32_t*)spos,
+                       (const uint32_t*)sposAlt, (const int64_t*)f.oid, oidAlt, (const uint64_t*)skeysAlt, keysAlt,
+                       (int)nsub);
+    hipLaunchKernelGGL(k_gather_list, dim3(nb), dim3(kBlock), 0, stream, back, (const uint32_t*)spos,
+                       (const uint32_t*)nullptr, (const int64_t*)oidAlt, f.oid, (const uint64_t*)nullptr,
+                       (uint64_t*)nullptr, (int)nsub);
+    HIP_CHECK(hipGetLastError());
+    for (int r = 0; r < maxRuns; ++r)
+      if (hbuild[r]) runs[r].stepOut = runs[r].fullOut = false;   // per-step outputs were not carried
+    npairsCache = -1;
+  }
+
   // The candidate list of run r no longer provably holds every neighbour pair (or the
   // measurement is not finite).
   bool skinTripped(int r) const {
@@ -777,8 +859,12 @@ class Batch {
     if (any || orderDirty) {
       HIP_CHECK(hipMemcpyAsync(dbuild, hbuild, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
       const int bits = any ? setupGrids() : 0;
-      sortAndGather(bits);
+      // a change of the batch's composition moves segments: full reorder; otherwise only the
+      // rebuilding runs are re-sorted, in place
+      if (orderDirty) sortAndGather(bits);
+      else sortRebuildingRuns(bits);
       orderDirty = false;
+      mark("reorder");
     } else {
       applyPendingBoundaries();
     }
@@ -788,8 +874,9 @@ class Batch {
     if (any) {
       launchD([&](auto d) {
         constexpr int DD = decltype(d)::value;
-        hipLaunchKernelGGL(k_cell_bounds<DD>, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t*)keysAlt, (int)nActive,
-                           (const RunGrid*)dgrids, (const int*)dbuild, mortonBits, cellStart, cellEnd);
+        hipLaunchKernelGGL(k_cell_bounds<DD>, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t*)keysAlt,
+                           (const int32_t*)f.run, (int)nActive, (const RunGrid*)dgrids, (const int*)dbuild, mortonBits,
+                           cellStart, cellEnd);
       });
       HIP_CHECK(hipGetLastError());
       bool ok = false;
@@ -809,6 +896,7 @@ class Batch {
         ensureCandCapacity(((need + need/4) + 7)/8*8);
       }
       if (!ok) raise(SPX_ERR_CAPACITY, "candidate list capacity not reached");
+      mark("candidates");
       for (int r = 0; r < maxRuns; ++r) {
         if (!hbuild[r]) continue;
         RunInfo& ri = runs[r];

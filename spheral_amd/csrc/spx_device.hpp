@@ -138,7 +138,7 @@ struct Fields {
   //                    end-of-step state (k_full_update)
   //   vmr (4):         v0 v1 v2 m of the connectivity state (k_pack)
   //   xhm (4):         x0 x1 x2 h^-1 of the midpoint state (k_midstep)
-  //   pkm (8):         v0 v1 v2 m rho cs Prho 0 of the midpoint state (midpoint grad-h pass)
+  //   pkm (8):         v0 v1 v2 m rho cs Prho 1/rho of the midpoint state (midpoint grad-h pass)
   //   pkv (6):         v^{1/2} (3) DepsDt m 0 for the compatible-energy partition (k_v12)
   double *xh, *vmr, *xhm, *pkm, *pkv;
   double *pot;       // gravitational potential

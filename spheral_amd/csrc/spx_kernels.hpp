@@ -127,18 +127,50 @@ __device__ __forceinline__ void unspread(uint64_t key, int mortonBits, int* c) {
 }
 
 // Cell ranges of the sorted keys of the runs that build candidate lists: start[cell],
-// end[cell] (both 0 when empty).
+// end[cell] (both 0 when empty).  The low mortonBits of a key are the cell's Morton code;
+// a run's particles are one contiguous segment (run[] tells where it ends).
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, int n, const RunGrid* grids, const int* build,
-                                                        int mortonBits, int* start, int* end) {
+__global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, const int32_t* run, int n,
+                                                        const RunGrid* grids, const int* build, int mortonBits,
+                                                        int* start, int* end) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const uint64_t k = keys[p];
-  const int r = (int)(k >> mortonBits);          // active keys carry rank 0 above the run field
+  const int r = run[p];
   if (!build[r]) return;
+  const uint64_t mask = (1ull << mortonBits) - 1ull;
+  const uint64_t k = keys[p] & mask;
   int c[3];
-  if (p == 0 || keys[p - 1] != k) { unspread<D>(k, mortonBits, c); start[lin_cell<D>(grids[r], c)] = p; }
-  if (p == n - 1 || keys[p + 1] != k) { unspread<D>(k, mortonBits, c); end[lin_cell<D>(grids[r], c)] = p + 1; }
+  if (p == 0 || run[p - 1] != r || (keys[p - 1] & mask) != k) { unspread<D>(k, mortonBits, c); start[lin_cell<D>(grids[r], c)] = p; }
+  if (p == n - 1 || run[p + 1] != r || (keys[p + 1] & mask) != k) { unspread<D>(k, mortonBits, c); end[lin_cell<D>(grids[r], c)] = p + 1; }
+}
+
+// Sort keys of the particles of the runs that rebuild (build[run] != 0), compacted: run r's
+// particles go to [sub[3r+1], +n) in position order, keyed [its segment index | cell Morton
+// code] (sub: 3 ints per run: segment start, compacted start, segment index).
+__global__ void __launch_bounds__(kBlock) k_subset_keys(const uint64_t* keys, const int32_t* run, const int* build,
+                                                        const int* sub, int mortonBits, int n, uint64_t* skeys,
+                                                        uint32_t* spos) {
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int r = run[p];
+  if (!build[r]) return;
+  const int k = sub[3*r + 1] + (p - sub[3*r]);
+  const uint64_t mask = (1ull << mortonBits) - 1ull;
+  skeys[k] = ((uint64_t)sub[3*r + 2] << mortonBits) | (keys[p] & mask);
+  spos[k] = (uint32_t)p;
+}
+
+// Gather over a list of positions: dst[a][pos[k]] = src[a][from ? from[k] : pos[k]], oid alike;
+// with sortedKeys, keysOut[pos[k]] = sortedKeys[k].
+__global__ void __launch_bounds__(kBlock) k_gather_list(GatherSet s, const uint32_t* pos, const uint32_t* from,
+                                                        const int64_t* oidSrc, int64_t* oidDst,
+                                                        const uint64_t* sortedKeys, uint64_t* keysOut, int nsub) {
+  const int k = blockIdx.x*blockDim.x + threadIdx.x;
+  if (k >= nsub) return;
+  const uint32_t p = pos[k], q = from ? from[k] : p;
+  for (int a = 0; a < s.count; ++a) s.dst[a][p] = s.src[a][q];
+  oidDst[p] = oidSrc[q];
+  if (sortedKeys) keysOut[p] = sortedKeys[k];
 }
 
 // Running per-run minimum of a positive double: one atomic per wavefront when its lanes share a run.
@@ -157,7 +189,7 @@ __device__ __forceinline__ void run_min(Scalars* sc, int r, double v, bool valid
   }
 }
 
-// Packed neighbour records (Fields::pk) of the current state, and nodeScale =
+// Packed neighbour records (Fields::xh, Fields::vmr) of the current state, and nodeScale =
 // 1/(lambda_max(H) nPerh) of each node (GenericHydro::dt, Physics/GenericHydro.cc:142-150).
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_pack(Fields f, Phys ph, int n) {
@@ -445,7 +477,9 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
       kernel_wgw_c<R>(coef, tab, (R)0, hdet_iso<D, R>(hi), w, gw, graw);   // self term W(|Hi ri - Hi ri|) = W(0)
       rho = f.m[i]*(double)w;
     }
-    double vmax2 = 0.0;       // pairwise limit, tracked as max of |v_ij|^2 / min(ns)^2
+    // pairwise limit: the pair with the largest |v_ij|^2 / min(ns)^2, tracked as the fraction
+    // (vS / vN) and compared by cross-multiplication, so only the winner is divided
+    double vS = 0.0, vN = 1.0;
     const size_t ld = (size_t)f.ld;
     const int nn = FILT ? f.ccnt[i] : f.cnt[i];
     const double hiF = FILT ? f.hinv[i] : 0.0;
@@ -514,7 +548,8 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
 #pragma unroll
           for (int a = 0; a < D; ++a) { const double d = vi[a] - refl_sign<D>(g, code, a)*vj[u][a]; s += d*d; }
           const double nsm = fmin(nsi, nsj[u]);
-          vmax2 = fmax(vmax2, s/(nsm*nsm));
+          const double n2 = nsm*nsm;
+          if (s*vN > vS*n2) { vS = s; vN = n2; }
         }
       }
     }
@@ -527,6 +562,7 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
     if (DT) {
       const double tiny = 2.220446049250313e-16;
       // min(ns_i, ns_j)/max(tiny, |v_ij|) minimised over the pairs
+      const double vmax2 = vS/vN;
       if (vmax2 > 0.0) dtmin = fmin(dtmin, 1.0/sqrt(vmax2));
       dtmin = fmin(dtmin, nsi/(f.cs[i] + tiny));
       const double csq = sqrt(f.maxQ[i]/rho);
@@ -619,7 +655,8 @@ __global__ void __launch_bounds__(kBlock) k_midstep(Fields f, Phys ph, const Sca
 }
 
 // Per-node pressure term Prho = safeInv(omega)*P/rho^2 of the midpoint state (SPH.cc:425-426),
-// written with the rest of node i's midpoint record (Fields::pkm).
+// written with the rest of node i's midpoint record (Fields::pkm), which also carries 1/rho
+// (the viscous pressure Q/rho of every pair the node takes part in is formed with it).
 template <int D>
 __device__ __forceinline__ void write_mid_record(const Fields& f, int i, double om) {
   const double rho = f.rhom[i];
@@ -629,7 +666,7 @@ __device__ __forceinline__ void write_mid_record(const Fields& f, int i, double 
   P[0] = make_double2(f.vm[0][i], D >= 2 ? f.vm[1 % D][i] : 0.0);
   P[1] = make_double2(D >= 3 ? f.vm[2 % D][i] : 0.0, f.m[i]);
   P[2] = make_double2(rho, f.csm[i]);
-  P[3] = make_double2(prho, 0.0);
+  P[3] = make_double2(prho, 1.0/rho);
 }
 
 // The midpoint records (Fields::xhm, Fields::pkm) from the midpoint arrays.  keepOmega: the

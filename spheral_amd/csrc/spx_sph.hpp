@@ -9,14 +9,14 @@ namespace spx {
 template <int D, typename R>
 struct Nbr {
   double x[D], v[D];     // positions and velocities stay double: differences are formed before rounding to R
-  R m, rho, cs, h, prho;
+  R m, rho, cs, h, prho, irho;
 };
 
 // A neighbour's midpoint record as loaded (no image transform yet), so the loads of the
 // next neighbour can be issued before the current pair is evaluated.
 template <int D>
 struct NbrRaw {
-  double x[D], v[D], m, rho, cs, h, prho;
+  double x[D], v[D], m, rho, cs, h, prho, irho;
 };
 template <int D>
 __device__ __forceinline__ void load_raw(const Fields& f, int j, NbrRaw<D>& q) {
@@ -34,6 +34,7 @@ __device__ __forceinline__ void load_raw(const Fields& f, int j, NbrRaw<D>& q) {
   q.rho = e.x;
   q.cs = e.y;
   q.prho = p.x;
+  q.irho = p.y;
 }
 // Image-transformed neighbour record in the pair arithmetic type.
 template <int D, typename R>
@@ -46,6 +47,7 @@ __device__ __forceinline__ void finish_mid(const Geo& g, int code, const NbrRaw<
   q.cs = (R)r.cs;
   q.h = (R)r.h;
   q.prho = (R)r.prho;
+  q.irho = (R)r.irho;
 }
 template <int D, typename R>
 __device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D, R>& s) {
@@ -56,6 +58,7 @@ __device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D, R>& s) 
   s.cs = (R)f.csm[i];
   s.h = (R)f.hinvm[i];
   s.prho = (R)f.prhom[i];
+  s.irho = (R)reinterpret_cast<const double2*>(f.pkm)[4*(size_t)i + 3].y;   // the value partners read
 }
 
 // Pair quantities of SPH::evaluateDerivativesImpl (SPH/SPH.cc:362-430) and
@@ -85,10 +88,11 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
     e2o += eo[c]*eo[c];
     e.vso[c] = (R)(S.v[c] - O.v[c]);
   }
-  e.etaMagS = sqrt(e2s);
-  const R etaMagO = sqrt(e2o);
+  // |eta| and safeInvVar(|eta|) = 1/max(1e-30, |eta|) from one reciprocal square root each
   const R tiny = sizeof(R) == sizeof(double) ? (R)1.0e-30 : (R)1.0e-30f;
-  const R is = (R)1/fmax(tiny, e.etaMagS), io = (R)1/fmax(tiny, etaMagO);   // safeInvVar
+  const R is = e2s > tiny*tiny ? rsqrt(e2s) : (R)1/tiny, io = e2o > tiny*tiny ? rsqrt(e2o) : (R)1/tiny;
+  e.etaMagS = e2s*is;
+  const R etaMagO = e2o*io;
   R gws, gwo, graws, grawo;
   kernel_wgw_c<R>(coef, tab, e.etaMagS, hdet_iso<D, R>(S.h), e.Ws, gws, graws);
   kernel_wgw_c<R>(coef, tab, etaMagO, hdet_iso<D, R>(O.h), e.Wo, gwo, grawo);
@@ -108,7 +112,7 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
   const R Cl = (R)ph.Cl, Cq = (R)ph.Cq;
   const R es_ = -Cl*S.cs*mms + Cq*(mms*mms);
   const R eo_ = -Cl*O.cs*mmo + Cq*(mmo*mmo);
-  const R QPs = es_/S.rho, QPo = eo_/O.rho;
+  const R QPs = es_*S.irho, QPo = eo_*O.irho;
   e.Qs = S.rho*es_;
   R wqs = (R)0;
 #pragma unroll
@@ -189,10 +193,11 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
     for (int c = 0; c < D; ++c) {
       dvdt[c] -= mo*(double)e.dd[c];
       vdg += e.vso[c]*e.gs[c];
+      const double mv = mo*(double)e.vso[c], mr = mo*(double)e.r[c];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        dvdx[c*D + d] -= mo*(double)(e.vso[c]*e.gs[d]);
-        Mc[c*D + d] -= mo*(double)(e.r[c]*e.gs[d]);
+        dvdx[c*D + d] -= mv*(double)e.gs[d];
+        Mc[c*D + d] -= mr*(double)e.gs[d];
       }
     }
     depsdt += mo*(double)(S.prho*vdg + e.workQs);
