@@ -165,6 +165,7 @@ def evrard(args):
         n = len(st["m"])
         r = engine.make_run(cfg, st)
         r.initialize()
+        r.enable_timing()
         t0 = time.perf_counter()
         k, t = 0, 0.0
         while t < t_end*(1.0 - 1e-15) and k < args.max_steps:
@@ -174,6 +175,7 @@ def evrard(args):
         wall = time.perf_counter() - t0
         out[name] = {"seconds": wall, "steps": k, "t": t, "particle_steps_per_s": n*k/wall,
                      "builds": r.neighbor_builds(),
+                     "kernel_ms_per_step": {c: v/max(k, 1) for c, v in r.kernel_times().items()},
                      "fields": {f: r.field(f) for f in ("mass density", "specific thermal energy")}}
         r.close()
     m, d = out["mixed"], out["fp64"]
@@ -187,6 +189,7 @@ def evrard(args):
                    "opening": 0.5, "softening": 0.01, "kernel": "BSpline TableKernel(1000), nPerh 1.51"},
         "t_reached": m["t"], "fp64_particle_steps_per_s": d["particle_steps_per_s"], "fp64_steps": d["steps"],
         "l1_mixed_vs_fp64": l1, "neighbor_list_builds": {"mixed": m["builds"], "fp64": d["builds"]},
+        "kernel_ms_per_step": m["kernel_ms_per_step"], "fp64_kernel_ms_per_step": d["kernel_ms_per_step"],
     }
     print(json.dumps(line), flush=True)
 

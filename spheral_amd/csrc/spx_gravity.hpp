@@ -179,55 +179,66 @@ __global__ void __launch_bounds__(kBlock) k_tree_gather(double* const* xs, const
   tm[p] = m[q];
 }
 
+// Device-library wavefront minimum (DPP lane exchanges, no LDS round trip).
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+
 // applyTreeForces for the point in tree slot p; results scattered to its hydro slot.
 // The wavefront walks the pre-order cells together: each lane keeps the next cell its own
 // walk would visit, the wavefront visits the smallest of them, and only the lanes waiting
 // on that cell act on it.  Every lane therefore takes exactly the decisions, in exactly the
-// order, of an independent stackless walk (same bits), while cell and leaf reads become
-// uniform across the wavefront instead of 64 scattered ones.
-__global__ void __launch_bounds__(kBlock) k_tree_walk(TreeView t, const uint32_t* perm, double soft2, double open2,
-                                                      double soft, double G, const double* cs2, double* const* acc,
-                                                      double* pot, double* blockMin) {
+// order, of an independent stackless walk (same bits).  The visited cell is wavefront-
+// uniform (readfirstlane), so its data and its leaf points are read through the scalar
+// cache; the tree arrays come as separate restrict-qualified arguments, which tells the
+// compiler they are not written here.
+__global__ void __launch_bounds__(kBlock) k_tree_walk(
+    const double* __restrict__ tx0, const double* __restrict__ tx1, const double* __restrict__ tx2,
+    const double* __restrict__ tm, const double* __restrict__ cx0, const double* __restrict__ cx1,
+    const double* __restrict__ cx2, const double* __restrict__ cM, const double* __restrict__ crc2,
+    const int* __restrict__ cfirst, const int* __restrict__ clast, const int* __restrict__ cskip,
+    const unsigned char* __restrict__ clevel, const double* __restrict__ ccs2, int ncells, int npts,
+    const uint32_t* __restrict__ perm, double soft2, double open2, double soft, double G, double* __restrict__ acc0,
+    double* __restrict__ acc1, double* __restrict__ acc2, double* __restrict__ pot, double* __restrict__ blockMin) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
-  const bool valid = p < t.n;
+  const bool valid = p < npts;
   double dtv = 1.7976931348623157e308;
   double xi = 0.0, yi = 0.0, zi = 0.0;
-  if (valid) { xi = t.tx[0][p]; yi = t.tx[1][p]; zi = t.tx[2][p]; }
+  if (valid) { xi = tx0[p]; yi = tx1[p]; zi = tx2[p]; }
   double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
-  int next = valid ? 1 : t.ncells;
+  int next = valid ? 1 : ncells;
   while (true) {
-    int c = next;
-    for (int off = 32; off > 0; off >>= 1) c = min(c, __shfl_xor(c, off));
-    if (c >= t.ncells) break;                     // every lane of the wavefront is done
+    const int c = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+    if (c >= ncells) break;                       // every lane of the wavefront is done
+    // the cell's data, read by the whole wavefront at one (uniform) address
+    const int lvl = clevel[c];
+    const double ccx = cx0[c], ccy = cx1[c], ccz = cx2[c], Mc = cM[c], open = ccs2[lvl]/open2 + crc2[c];
+    const int q0 = cfirst[c], q1 = clast[c], sk = cskip[c];
     if (next != c) continue;
-    const int lvl = t.level[c];
-    const double dx = t.cx[0][c] - xi, dy = t.cx[1][c] - yi, dz = t.cx[2][c] - zi;
+    const double dx = ccx - xi, dy = ccy - yi, dz = ccz - zi;
     const double r2 = dx*dx + dy*dy + dz*dz;
-    if (r2 > cs2[lvl]/open2 + t.rc2[c]) {
+    if (r2 > open) {
       const double mag = sqrt(r2);
       const double inv = mag > 1.0e-50 ? 1.0/mag : 0.0;
       const double nx = mag > 1.0e-50 ? dx*inv : 1.0, ny = dy*inv, nz = dz*inv;
       const double rs = r2 + soft2;
-      const double Mc = t.M[c];
       const double f = Mc*(1.0/rs);
       ax += f*nx; ay += f*ny; az += f*nz;
       phi -= Mc*(1.0/sqrt(rs));
-      next = t.skip[c];
-    } else if (t.first[c] == t.last[c] || lvl == kTreeBits) {
-      for (int q = t.first[c]; q <= t.last[c]; ++q) {
-        const double ex = t.tx[0][q] - xi, ey = t.tx[1][q] - yi, ez = t.tx[2][q] - zi;
+      next = sk;
+    } else if (q0 == q1 || lvl == kTreeBits) {
+      for (int q = q0; q <= q1; ++q) {
+        const double ex = tx0[q] - xi, ey = tx1[q] - yi, ez = tx2[q] - zi;
         const double q2 = ex*ex + ey*ey + ez*ez;
         if (q2/soft2 > 1.0e-10) {
           const double mag = sqrt(q2);
           const double inv = 1.0/mag;
           const double rs = q2 + soft2;
-          const double mq = t.tm[q];
+          const double mq = tm[q];
           const double f = mq*(1.0/rs);
           ax += f*(ex*inv); ay += f*(ey*inv); az += f*(ez*inv);
           phi -= mq*(1.0/sqrt(rs));
         }
       }
-      next = t.skip[c];
+      next = sk;
     } else {
       next = c + 1;
     }
@@ -236,9 +247,9 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(TreeView t, const uint32_t
     const double amag = sqrt(ax*ax + ay*ay + az*az);
     if (amag > 0.0) dtv = sqrt(soft/amag);
     const uint32_t s = perm[p];
-    acc[0][s] = G*ax;
-    acc[1][s] = G*ay;
-    acc[2][s] = G*az;
+    acc0[s] = G*ax;
+    acc1[s] = G*ay;
+    acc2[s] = G*az;
     pot[s] = G*phi;
   }
   __shared__ double red[kBlock];
@@ -284,7 +295,6 @@ class GravityTree {
   double* part = nullptr;
   double* hpart = nullptr;
   double* cs2 = nullptr;
-  double** accPtrs = nullptr;
   double** xsPtrs = nullptr;
   // cells
   int64_t cellCap = 0;
@@ -323,7 +333,6 @@ class GravityTree {
     part = alloc<double>((size_t)nb*6 + 8);
     HIP_CHECK(hipHostMalloc((void**)&hpart, ((size_t)nb*6 + 8)*sizeof(double), hipHostMallocDefault));
     cs2 = alloc<double>(32);
-    accPtrs = alloc<double*>(3);
     xsPtrs = alloc<double*>(3);
     size_t t1 = 0, t2 = 0;
     HIP_CHECK(rocprim::radix_sort_pairs(nullptr, t1, codes, codesSorted, idx, perm, (size_t)n, 0, 60, stream));
@@ -407,10 +416,10 @@ class GravityTree {
   void evaluate(double* const* acc, double* pot, int nn, double G, double soft, double opening, Scalars* sc,
                 double ftimestep) {
     const int nb = (int)((nn + kBlock - 1)/kBlock);
-    double* ha[3] = {acc[0], acc[1], acc[2]};
-    HIP_CHECK(hipMemcpyAsync(accPtrs, ha, sizeof(ha), hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_tree_walk, dim3(nb), dim3(kBlock), 0, stream, view, perm, soft*soft, opening*opening, soft, G, cs2,
-                       accPtrs, pot, part);
+    hipLaunchKernelGGL(k_tree_walk, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1], view.tx[2], view.tm,
+                       view.cx[0], view.cx[1], view.cx[2], view.M, view.rc2, view.first, view.last, view.skip, view.level,
+                       (const double*)cs2, view.ncells, view.n, (const uint32_t*)perm, soft*soft, opening*opening, soft, G,
+                       acc[0], acc[1], acc[2], pot, part);
     hipLaunchKernelGGL(k_tree_dt, dim3(1), dim3(kBlock), 0, stream, part, nb, sc, G, ftimestep);
     HIP_CHECK(hipGetLastError());
   }
