@@ -121,6 +121,9 @@ class Batch {
   uint32_t* candAlt = nullptr;
   int32_t* ccntAlt = nullptr;
   int candAltCap = 0;
+  uint32_t* tlist = nullptr;  // candidate entries handed over by partners (k_candidates)
+  int* tcnt = nullptr;
+  int tcap = 0;
   uint64_t *keys = nullptr, *keysAlt = nullptr;
   uint32_t *vals = nullptr, *valsAlt = nullptr;
   void* sortTmp = nullptr;
@@ -361,10 +364,10 @@ class Batch {
     sortTmp = dalloc<char>(tmp);
     dgrids = dalloc<RunGrid>(maxRuns);
     dsc = dalloc<Scalars>(maxRuns);
-    dext = dalloc<unsigned long long>((size_t)maxRuns*7);
+    dext = dalloc<unsigned long long>((size_t)maxRuns*kExt);
     dstate = dalloc<int>(maxRuns);
     dinit = dalloc<int>(maxRuns);
-    dOverflow = dalloc<int>(1);
+    dOverflow = dalloc<int>(2);
     dbuild = dalloc<int>(maxRuns);
     dsub = dalloc<int>(3*(size_t)maxRuns);
     skeys = dalloc<uint64_t>(ld);
@@ -374,10 +377,10 @@ class Batch {
     ddelta = dalloc<int>(maxRuns);
     hgrids = halloc<RunGrid>(maxRuns);
     hsc = halloc<Scalars>(maxRuns);
-    hext = halloc<unsigned long long>((size_t)maxRuns*7);
+    hext = halloc<unsigned long long>((size_t)maxRuns*kExt);
     hstate = halloc<int>(maxRuns);
     hinit = halloc<int>(maxRuns);
-    hOverflow = halloc<int>(1);
+    hOverflow = halloc<int>(2);
     hbuild = halloc<int>(maxRuns);
     hsub = halloc<int>(3*(size_t)maxRuns);
     hdelta = halloc<int>(maxRuns);
@@ -387,6 +390,17 @@ class Batch {
     upload(dsc, hsc, sizeof(Scalars)*maxRuns);
     ensureNbrCapacity(D == 1 ? 16 : (D == 2 ? 32 : 64));
     ensureCandCapacity(D == 1 ? 16 : (D == 2 ? 48 : 96));
+    tcnt = dalloc<int>(ld);
+    ensureHandoverCapacity(16);
+  }
+
+  // Room for c handed-over candidate entries per node (contents are rebuilt on every use).
+  void ensureHandoverCapacity(int c) {
+    if (c <= tcap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    dfree(tlist);
+    tlist = dalloc<uint32_t>((size_t)c*ld);
+    tcap = c;
   }
 
   void dfree(void* p) {
@@ -567,7 +581,7 @@ class Batch {
   // Cell grids of the runs that build candidate lists (hbuild), from their current extents;
   // returns the Morton bits per axis needed.
   int setupGrids() {
-    hipLaunchKernelGGL(k_reset_extent, dim3(nblocks((int64_t)maxRuns*7)), dim3(kBlock), 0, stream, dext, maxRuns);
+    hipLaunchKernelGGL(k_reset_extent, dim3(nblocks((int64_t)maxRuns*kExt)), dim3(kBlock), 0, stream, dext, maxRuns);
     int64_t nt = 0;
     for (int r = 0; r < maxRuns; ++r) {
       if (runs[r].state != RUN_ACTIVE || !hbuild[r]) continue;
@@ -587,7 +601,7 @@ class Batch {
       });
       HIP_CHECK(hipGetLastError());
     }
-    download(hext, dext, sizeof(unsigned long long)*maxRuns*7);
+    download(hext, dext, sizeof(unsigned long long)*maxRuns*kExt);
     const int64_t maxAxis = D == 3 ? (1 << 15) : (D == 2 ? (1 << 20) : (1 << 26));
     int64_t base = 0;
     int bits = 0;
@@ -598,58 +612,69 @@ class Batch {
       gr.hinvMin = 0.0;
       gr.ext2c = 0.0;
       if (runs[r].state != RUN_ACTIVE || !hbuild[r]) continue;
-      const double hinvMin = ord_value(hext[r*7]);
+      const double hinvMin = ord_value(hext[r*kExt]), hinvMax = ord_value(hext[r*kExt + 7]);
       if (!(hinvMin > 0.0) || !std::isfinite(hinvMin)) raise(SPX_ERR_NUMERIC, "non-positive or non-finite H on the device");
       gr.hinvMin = hinvMin;
       const double extc = cfg.neighbor_extent*(1.0 + runs[r].skin);
       gr.ext2c = extc*extc;
       const double support = extc/hinvMin*(1.0 + 1.0e-9);
+      // where h varies, cells shrink toward the smaller supports (by up to 4x, while the grid
+      // stays within ~8 cells per node) and each node scans the cells its own support reaches
+      int F = 1;
+      if (hinvMax >= 2.0*hinvMin) F = (int)std::min(4.0, std::floor(hinvMax/hinvMin));
       int64_t total = 1;
-      for (int a = 0; a < D; ++a) {
-        const double lo = ord_value(hext[r*7 + 1 + 2*a]), hi = ord_value(hext[r*7 + 2 + 2*a]);
-        if (!std::isfinite(lo) || !std::isfinite(hi)) raise(SPX_ERR_NUMERIC, "non-finite position on the device");
-        const int blo = geo.bc_lo[a], bhi = geo.bc_hi[a];
-        const double L = cfg.xmax[a] - cfg.xmin[a];
-        int64_t nc;
-        double cell, origin;
-        if (blo == BC_PERIODIC) {
-          nc = (int64_t)std::floor(L/support);
-          if (nc < 3 && runs[r].skin > 0.0) {
-            // too small a box for the widened radius: fall back to the plain one
-            runs[r].skin = 0.0;
-            return setupGrids();
+      for (;;) {
+        total = 1;
+        int runBitsAxis = 0;
+        for (int a = 0; a < D; ++a) {
+          const double lo = ord_value(hext[r*kExt + 1 + 2*a]), hi = ord_value(hext[r*kExt + 2 + 2*a]);
+          if (!std::isfinite(lo) || !std::isfinite(hi)) raise(SPX_ERR_NUMERIC, "non-finite position on the device");
+          const int blo = geo.bc_lo[a], bhi = geo.bc_hi[a];
+          const double L = cfg.xmax[a] - cfg.xmin[a];
+          const double target = support/F;
+          int64_t nc;
+          double cell, origin;
+          if (blo == BC_PERIODIC) {
+            if (std::floor(L/support) < 3 && runs[r].skin > 0.0) {
+              // too small a box for the widened radius: fall back to the plain one
+              runs[r].skin = 0.0;
+              return setupGrids();
+            }
+            if (std::floor(L/support) < 3) raise(SPX_ERR_UNSUPPORTED, "periodic box must span at least 3 support radii on every axis");
+            nc = std::min<int64_t>((int64_t)std::floor(L/target), maxAxis);
+            cell = L/nc;
+            origin = cfg.xmin[a];
+          } else if (blo == BC_REFLECTING && bhi == BC_REFLECTING) {
+            nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor(L/target), maxAxis));
+            cell = L/nc;
+            origin = cfg.xmin[a];
+          } else if (blo == BC_REFLECTING) {
+            const double top = std::max(hi, cfg.xmin[a]);
+            nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor((top - cfg.xmin[a])/target) + 1, maxAxis));
+            cell = std::max(target, (top - cfg.xmin[a])/nc*(1.0 + 1e-12));
+            origin = cfg.xmin[a];
+          } else if (bhi == BC_REFLECTING) {
+            const double bot = std::min(lo, cfg.xmax[a]);
+            nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor((cfg.xmax[a] - bot)/target) + 1, maxAxis));
+            cell = std::max(target, (cfg.xmax[a] - bot)/nc*(1.0 + 1e-12));
+            origin = cfg.xmax[a] - nc*cell;
+          } else {
+            const double span = std::max(0.0, hi - lo);
+            nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor(span/target) + 1, maxAxis));
+            cell = std::max(target, span/nc*(1.0 + 1e-12));
+            origin = lo;
           }
-          if (nc < 3) raise(SPX_ERR_UNSUPPORTED, "periodic box must span at least 3 support radii on every axis");
-          nc = std::min<int64_t>(nc, maxAxis);
-          cell = L/nc;
-          origin = cfg.xmin[a];
-        } else if (blo == BC_REFLECTING && bhi == BC_REFLECTING) {
-          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor(L/support), maxAxis));
-          cell = L/nc;
-          origin = cfg.xmin[a];
-        } else if (blo == BC_REFLECTING) {
-          const double top = std::max(hi, cfg.xmin[a]);
-          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor((top - cfg.xmin[a])/support) + 1, maxAxis));
-          cell = std::max(support, (top - cfg.xmin[a])/nc*(1.0 + 1e-12));
-          origin = cfg.xmin[a];
-        } else if (bhi == BC_REFLECTING) {
-          const double bot = std::min(lo, cfg.xmax[a]);
-          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor((cfg.xmax[a] - bot)/support) + 1, maxAxis));
-          cell = std::max(support, (cfg.xmax[a] - bot)/nc*(1.0 + 1e-12));
-          origin = cfg.xmax[a] - nc*cell;
-        } else {
-          const double span = std::max(0.0, hi - lo);
-          nc = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor(span/support) + 1, maxAxis));
-          cell = std::max(support, span/nc*(1.0 + 1e-12));
-          origin = lo;
+          gr.nc[a] = (int)nc;
+          gr.inv_cell[a] = 1.0/cell;
+          gr.origin[a] = origin;
+          total *= nc;
+          int b = 0;
+          while (((int64_t)1 << b) < nc) ++b;
+          runBitsAxis = std::max(runBitsAxis, b);
         }
-        gr.nc[a] = (int)nc;
-        gr.inv_cell[a] = 1.0/cell;
-        gr.origin[a] = origin;
-        total *= nc;
-        int b = 0;
-        while (((int64_t)1 << b) < nc) ++b;
-        bits = std::max(bits, b);
+        if (F > 1 && total > 8*runs[r].n + 4096) { --F; continue; }
+        bits = std::max(bits, runBitsAxis);
+        break;
       }
       base += total;
       if (base > ((int64_t)1 << 28)) raise(SPX_ERR_CAPACITY, "cell grids too large (support radius tiny relative to the domain)");
@@ -880,20 +905,30 @@ class Batch {
       });
       HIP_CHECK(hipGetLastError());
       bool ok = false;
-      for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
-        HIP_CHECK(hipMemsetAsync(dOverflow, 0, sizeof(int), stream));
+      for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
+        HIP_CHECK(hipMemsetAsync(dOverflow, 0, 2*sizeof(int), stream));
+        HIP_CHECK(hipMemsetAsync(tcnt, 0, nActive*sizeof(int), stream));
         launchD([&](auto d) {
           constexpr int DD = decltype(d)::value;
           hipLaunchKernelGGL(k_candidates<DD>, dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f, geo,
                              (const RunGrid*)dgrids, (const int*)dbuild, (int)nActive, (const int*)cellStart,
-                             (const int*)cellEnd, dOverflow);
+                             (const int*)cellEnd, tlist, tcnt, tcap, dOverflow);
         });
+        hipLaunchKernelGGL(k_candidates_merge, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const int*)dbuild,
+                           (int)nActive, (const uint32_t*)tlist, (const int*)tcnt, tcap, dOverflow);
         HIP_CHECK(hipGetLastError());
-        download(hOverflow, dOverflow, sizeof(int));
-        if (*hOverflow == 0) { ok = true; break; }
-        const int need = *hOverflow;
-        if (need > 8192) raise(SPX_ERR_CAPACITY, "more than 8192 neighbour candidates for one node");
-        ensureCandCapacity(((need + need/4) + 7)/8*8);
+        download(hOverflow, dOverflow, 2*sizeof(int));
+        if (hOverflow[0] == 0 && hOverflow[1] == 0) { ok = true; break; }
+        if (hOverflow[0] > 0) {
+          const int need = hOverflow[0];
+          if (need > 8192) raise(SPX_ERR_CAPACITY, "more than 8192 neighbour candidates for one node");
+          ensureCandCapacity(((need + need/4) + 7)/8*8);
+        }
+        if (hOverflow[1] > 0) {
+          const int need = hOverflow[1];
+          if (need > 8192) raise(SPX_ERR_CAPACITY, "more than 8192 handed-over candidates for one node");
+          ensureHandoverCapacity(((need + need/4) + 7)/8*8);
+        }
       }
       if (!ok) raise(SPX_ERR_CAPACITY, "candidate list capacity not reached");
       mark("candidates");

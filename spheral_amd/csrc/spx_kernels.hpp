@@ -251,9 +251,28 @@ __device__ __forceinline__ void flush_chunk(uint32_t* nbr, int k, int i, size_t 
 // Reflecting images are isometries of the node, and periodic images are re-chosen as the
 // nearest copy each step (the box spans >= 3 widened supports), so both obey the bound.
 // ---------------------------------------------------------------------------
+// Image code of the same pair seen from the other node: a periodic image flips side, a
+// reflecting image is its own inverse.
+__device__ __forceinline__ int inverse_image(const Geo& g, int code) {
+  int out = code;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int c = (code >> (2*a)) & 3;
+    if (c && ((g.permask >> a) & 1)) out = (out & ~(3 << (2*a))) | ((3 - c) << (2*a));
+  }
+  return out;
+}
+
+// Candidate lists of the runs that rebuild.  Each node gathers the j within its own widened
+// support (|Hi rij|^2 <= ext^2 (1 + skin)^2), scanning the cells that support reaches (cells
+// may be smaller than the largest support when h varies; the stencil then grows per node).
+// A j whose own support does not reach i cannot find the pair itself, so i hands it the
+// mirrored entry (tlist, merged by k_candidates_merge).  Together each list holds exactly the
+// j with |Hi rij| or |Hj rij| within the widened extent.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const RunGrid* grids, const int* build, int n,
-                                                       const int* cstart, const int* cend, int* overflow) {
+                                                       const int* cstart, const int* cend, uint32_t* tlist, int* tcnt,
+                                                       int tcap, int* overflow) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int r = f.run[i];
@@ -270,19 +289,22 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
   const double hi = f.hinv[i];
   f.hb[i] = hi;
   const double ext2c = gr.ext2c;
-  // Every candidate lies within extc/min(hi, hj) <= extc/min(hi, hinvMin) of node i.  Farther
-  // candidates are rejected by an FP32 distance test first; its 1e-4 relative margin is far
-  // above the FP32 rounding of the test, so only the FP64 criterion decides membership.
-  const double hq = fmin(hi, gr.hinvMin);
-  const float reach2 = (float)(ext2c/(hq*hq)*(1.0 + 1.0e-4));
+  const double reach = sqrt(ext2c)/hi;
+  // stencil half-width per axis: the cells the node's own widened support reaches
+  int kw[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) kw[a] = max(1, (int)ceil(reach*gr.inv_cell[a]*(1.0 + 1.0e-12)));
+  // FP32 distance pre-filter with a 1e-4 relative margin (far above its own rounding): only the
+  // FP64 criterion decides membership
+  const float reach2 = (float)(ext2c/(hi*hi)*(1.0 + 1.0e-4));
   const size_t ld = (size_t)f.ld;
   const int cap = f.ccap;
   int count = 0;
   uint32_t chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int r1 = D >= 2 ? 1 : 0, r2 = D >= 3 ? 1 : 0;
-  for (int dz = -r2; dz <= r2; ++dz)
-    for (int dy = -r1; dy <= r1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
+  const int kz = D >= 3 ? kw[2] : 0, ky = D >= 2 ? kw[1] : 0;
+  for (int dz = -kz; dz <= kz; ++dz)
+    for (int dy = -ky; dy <= ky; ++dy)
+      for (int dx = -kw[0]; dx <= kw[0]; ++dx) {
         const int off[3] = {dx, dy, dz};
         int cc[3] = {0, 0, 0};
         int code = 0;
@@ -293,11 +315,11 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
           const bool per = (g.permask >> a) & 1;
           if (c < 0) {
             if (per) { c += gr.nc[a]; code |= 1 << (2*a); }
-            else if ((g.reflmask >> (2*a)) & 1) { c = -c - 1; code |= 1 << (2*a); }
+            else if ((g.reflmask >> (2*a)) & 1) { c = -c - 1; code |= 1 << (2*a); skip |= c >= gr.nc[a]; }
             else skip = true;
           } else if (c >= gr.nc[a]) {
             if (per) { c -= gr.nc[a]; code |= 2 << (2*a); }
-            else if ((g.reflmask >> (2*a + 1)) & 1) { c = 2*gr.nc[a] - 1 - c; code |= 2 << (2*a); }
+            else if ((g.reflmask >> (2*a + 1)) & 1) { c = 2*gr.nc[a] - 1 - c; code |= 2 << (2*a); skip |= c < 0; }
             else skip = true;
           }
           cc[a] = c;
@@ -324,20 +346,47 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
             e2i = a == 0 ? dmul(ei, ei) : dadd(e2i, dmul(ei, ei));
             e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
           }
-          if (e2i <= ext2c || e2j <= ext2c) {
-            // write-combine 8 entries in registers, flush them as one 32-byte chunk
-            const uint32_t e = (uint32_t)j | ((uint32_t)code << kImgShift);
-            const int s = count & 7;
+          if (!(e2i <= ext2c)) continue;
+          // write-combine 8 entries in registers, flush them as one 32-byte chunk
+          const uint32_t e = (uint32_t)j | ((uint32_t)code << kImgShift);
+          const int s = count & 7;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) chunk[u] = s == u ? e : chunk[u];
-            ++count;
-            if (s == 7 && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
+          for (int u = 0; u < 8; ++u) chunk[u] = s == u ? e : chunk[u];
+          ++count;
+          if (s == 7 && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
+          if (!(e2j <= ext2c)) {
+            // j's own support does not reach i: j receives the pair from here
+            const int slot = atomicAdd(&tcnt[j], 1);
+            if (slot < tcap) tlist[ell_at(slot, j, ld)] = (uint32_t)i | ((uint32_t)inverse_image(g, code) << kImgShift);
           }
         }
       }
   if ((count & 7) && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
   f.ccnt[i] = count;
   if (count > cap) atomicMax(overflow, count);
+}
+
+// Appends each rebuilding node's handed-over entries (tlist) to its candidate list, in
+// ascending entry order (the order they arrived in is not deterministic).  overflow[0]: the
+// candidate capacity needed, overflow[1]: the handed-over capacity needed.
+__global__ void __launch_bounds__(kBlock) k_candidates_merge(Fields f, const int* build, int n, const uint32_t* tlist,
+                                                             const int* tcnt, int tcap, int* overflow) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n || !build[f.run[i]]) return;
+  const int t = tcnt[i];
+  if (t == 0) return;
+  if (t > tcap) { atomicMax(&overflow[1], t); return; }
+  const size_t ld = (size_t)f.ld;
+  const int c0 = f.ccnt[i];
+  if (c0 + t > f.ccap) { atomicMax(&overflow[0], c0 + t); return; }
+  // selection by rank: entry k goes to position c0 + (number of smaller entries); entries are distinct
+  for (int k = 0; k < t; ++k) {
+    const uint32_t e = tlist[ell_at(k, i, ld)];
+    int rank = 0;
+    for (int q = 0; q < t; ++q) rank += tlist[ell_at(q, i, ld)] < e ? 1 : 0;
+    f.cand[ell_at(c0 + rank, i, ld)] = e;
+  }
+  f.ccnt[i] = c0 + t;
 }
 
 // The exact criterion over each active node's candidates; writes the ELL neighbour list in
@@ -743,20 +792,23 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
   if (MID) write_mid_record<D>(f, i, om);
 }
 
-// Per-run extents for the next grid: min hinv and the bounding box, as ordered bits.
+// Per-run extents for the next grid: min and max hinv and the bounding box, as ordered bits.
 // One block per tile (run, begin, end) of a run's contiguous particle segment; a block
 // reduces its tile and adds one set of atomics.
-// ext layout per run: [hinv_min, xmin0, xmax0, xmin1, xmax1, xmin2, xmax2].
+// ext layout per run (kExt values): [hinv_min, xmin0, xmax0, xmin1, xmax1, xmin2, xmax2, hinv_max].
+constexpr int kExt = 8;
+__host__ __device__ __forceinline__ bool ext_is_min(int q) { return q == 0 || (q < 7 && (q & 1)); }
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_extent(Fields f, const int* tiles, unsigned long long* ext) {
-  __shared__ unsigned long long red[7][kBlock/64];
+  __shared__ unsigned long long red[kExt][kBlock/64];
   const int r = tiles[3*blockIdx.x], b = tiles[3*blockIdx.x + 1], e = tiles[3*blockIdx.x + 2];
-  unsigned long long v[7];
+  unsigned long long v[kExt];
 #pragma unroll
-  for (int q = 0; q < 7; ++q) v[q] = (q == 0 || (q & 1)) ? ~0ull : 0ull;
+  for (int q = 0; q < kExt; ++q) v[q] = ext_is_min(q) ? ~0ull : 0ull;
   for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
     const unsigned long long h = ord_bits(f.hinv[i]);
     v[0] = h < v[0] ? h : v[0];
+    v[7] = h > v[7] ? h : v[7];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       const unsigned long long x = ord_bits(f.x[a][i]);
@@ -765,24 +817,25 @@ __global__ void __launch_bounds__(kBlock) k_extent(Fields f, const int* tiles, u
     }
   }
 #pragma unroll
-  for (int q = 0; q < 7; ++q) {
+  for (int q = 0; q < kExt; ++q) {
     unsigned long long x = v[q];
     for (int o = 32; o > 0; o >>= 1) {
       const unsigned long long y = __shfl_xor(x, o);
-      x = (q == 0 || (q & 1)) ? (y < x ? y : x) : (y > x ? y : x);
+      x = ext_is_min(q) ? (y < x ? y : x) : (y > x ? y : x);
     }
     if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long* out = ext + (size_t)r*7;
-    for (int q = 0; q < 1 + 2*D; ++q) {
+    unsigned long long* out = ext + (size_t)r*kExt;
+    for (int q = 0; q < kExt; ++q) {
+      if (q >= 1 + 2*D && q < 7) continue;
       unsigned long long x = red[q][0];
       for (int w = 1; w < kBlock/64; ++w) {
         const unsigned long long y = red[q][w];
-        x = (q == 0 || (q & 1)) ? (y < x ? y : x) : (y > x ? y : x);
+        x = ext_is_min(q) ? (y < x ? y : x) : (y > x ? y : x);
       }
-      if (q == 0 || (q & 1)) atomicMin(&out[q], x);
+      if (ext_is_min(q)) atomicMin(&out[q], x);
       else atomicMax(&out[q], x);
     }
   }
@@ -790,9 +843,8 @@ __global__ void __launch_bounds__(kBlock) k_extent(Fields f, const int* tiles, u
 
 __global__ void k_reset_extent(unsigned long long* ext, int nruns) {
   const int k = blockIdx.x*blockDim.x + threadIdx.x;
-  if (k >= nruns*7) return;
-  const int q = k % 7;
-  ext[k] = (q == 0 || (q & 1)) ? ~0ull : 0ull;
+  if (k >= nruns*kExt) return;
+  ext[k] = ext_is_min(k % kExt) ? ~0ull : 0ull;
 }
 
 // Non-finite guard on the state (NaN/Inf -> per-run status word).
