@@ -194,3 +194,27 @@ def test_tree_gravity_matches_oracle(gpu, oracle_mod, n, opening):
         assert abs(g.step() - o.step()) <= TOL*o.time()[2]
     a, b = g.field("velocity"), o.field("velocity")
     assert np.all(np.abs(a - b) <= 1e-9*max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("name", ["sod1d", "kelvin_helmholtz_2d"])
+def test_rk2_without_gradh_tracks_oracle(gpu, oracle_mod, name):
+    """gradhCorrection=False: the grad-h correction stays at its initial 1 through every step
+    (midpoint included), on both sides."""
+    if name == "sod1d":
+        cfg, st = problems.sod_1d(400)
+    else:
+        cfg, st = problems.kelvin_helmholtz_2d(3000)
+        cfg.precision = abi.SPX_PREC_FP64
+    cfg.gradh_correction = 0
+    print(f"[case] {name} gradh off", flush=True)
+    g, o = pair(gpu, oracle_mod, cfg, st)
+    for _ in range(5):
+        dg = g.step()
+        do = o.step()
+        assert abs(dg - do) <= 1e-10*do
+    assert np.all(g.field("grad h corrections") == 1.0)
+    cs = o.field("sound speed").max()
+    for f in ("position", "velocity", "mass density", "specific thermal energy", "H"):
+        a, b = g.field(f), o.field(f)
+        sc = max(np.abs(b).max(), cs if f == "velocity" else 0.0)
+        assert np.all(np.abs(a - b) <= 1e-9*sc), f
