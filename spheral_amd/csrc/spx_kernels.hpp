@@ -526,9 +526,13 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
       kernel_wgw_c<R>(coef, tab, (R)0, hdet_iso<D, R>(hi), w, gw, graw);   // self term W(|Hi ri - Hi ri|) = W(0)
       rho = f.m[i]*(double)w;
     }
-    // pairwise limit: the pair with the largest |v_ij|^2 / min(ns)^2, tracked as the fraction
-    // (vS / vN) and compared by cross-multiplication, so only the winner is divided
-    double vS = 0.0, vN = 1.0;
+    // pairwise limit min(ns_i, ns_j)/|v_ij| (GenericHydro.cc:196-210).  For isotropic H the
+    // nodeScale is 1/(h^-1 nPerh) (lambda_max(h^-1 I) = h^-1; the reference's eigen-solver may
+    // differ from it in the last bits), so the largest |v_ij|^2/min(ns)^2 is tracked as
+    // max |v_ij|^2 max(h_i^-1, h_j^-1)^2, scaled by nPerh^2 at the end: no per-pair division
+    // and no nodeScale read per neighbour
+    const double hiD = f.hinv[i];
+    double vmaxh = 0.0;
     const size_t ld = (size_t)f.ld;
     const int nn = FILT ? f.ccnt[i] : f.cnt[i];
     const double hiF = FILT ? f.hinv[i] : 0.0;
@@ -537,13 +541,12 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
     for (int k0 = 0; k0 < nn; k0 += G) {
       int js[G], codes[G];
       ell_group<G>(FILT ? f.cand : f.nbr, ld, k0, i, nn, js, codes);
-      double xj[G][D], vj[G][D], hj[G], mj[G], nsj[G];
+      double xj[G][D], vj[G][D], hj[G], mj[G];
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int j = js[u];
-        if (DENS || FILT) load_xh<D>(f.xh, j, xj[u], hj[u]);
+        load_xh<D>(f.xh, j, xj[u], hj[u]);
         load_vm<D>(f.vmr, j, vj[u], mj[u]);
-        if (DT) nsj[u] = f.ns[j];
       }
 #pragma unroll
       for (int u = 0; u < G; ++u) {
@@ -596,9 +599,8 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
           double s = 0.0;
 #pragma unroll
           for (int a = 0; a < D; ++a) { const double d = vi[a] - refl_sign<D>(g, code, a)*vj[u][a]; s += d*d; }
-          const double nsm = fmin(nsi, nsj[u]);
-          const double n2 = nsm*nsm;
-          if (s*vN > vS*n2) { vS = s; vN = n2; }
+          const double hm = fmax(hiD, hj[u]);
+          vmaxh = fmax(vmaxh, s*(hm*hm));
         }
       }
     }
@@ -611,7 +613,7 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
     if (DT) {
       const double tiny = 2.220446049250313e-16;
       // min(ns_i, ns_j)/max(tiny, |v_ij|) minimised over the pairs
-      const double vmax2 = vS/vN;
+      const double vmax2 = vmaxh*(ph.nperh*ph.nperh);
       if (vmax2 > 0.0) dtmin = fmin(dtmin, 1.0/sqrt(vmax2));
       dtmin = fmin(dtmin, nsi/(f.cs[i] + tiny));
       const double csq = sqrt(f.maxQ[i]/rho);
