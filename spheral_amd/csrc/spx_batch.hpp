@@ -1058,19 +1058,21 @@ class Batch {
                         f, geo, ph, dtab, (int)nActive, dsc, (int*)nullptr);
     });
   }
-  void sphPass(bool full) {
+  // withV12: the step's pass, which also writes the half-step velocity records.
+  void sphPass(bool full, bool withV12 = false) {
     const int nb = (int)xcd_grid(nActive, kBlock);
     const size_t tabBytes = tableBytes();
     const int useLds = tabBytes <= kLdsTableMax ? 1 : 0;
     const size_t shm = useLds ? tabBytes : 0;
+    const Scalars* sc = withV12 ? (const Scalars*)dsc : nullptr;
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (full) {
-        if (mixed) hipLaunchKernelGGL((k_sph<DD, true, float>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
-        else hipLaunchKernelGGL((k_sph<DD, true, double>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
+        if (mixed) hipLaunchKernelGGL((k_sph<DD, true, float>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
+        else hipLaunchKernelGGL((k_sph<DD, true, double>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
       } else {
-        if (mixed) hipLaunchKernelGGL((k_sph<DD, false, float>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
-        else hipLaunchKernelGGL((k_sph<DD, false, double>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds);
+        if (mixed) hipLaunchKernelGGL((k_sph<DD, false, float>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
+        else hipLaunchKernelGGL((k_sph<DD, false, double>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
       }
     });
     HIP_CHECK(hipGetLastError());
@@ -1195,12 +1197,11 @@ class Batch {
     mark("omega");
     // derivatives at the midpoint
     if (cfg.gravity) { gravityPass(true); mark("gravity"); }
-    sphPass(false);
+    sphPass(false, true);
     mark("sph");
     // full step from the start-of-step state, compatible energy first
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)dsc, -1.0, (int)nActive);
       launchPrec(k_full_update<DD, false, double, kGroupUpdate>, k_full_update<DD, false, float, kGroupUpdate>, nb, f, geo,
                  ph, dsc, 0.0, (int)nActive);
     });

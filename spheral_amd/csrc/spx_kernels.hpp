@@ -542,18 +542,21 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
       int js[G], codes[G];
       ell_group<G>(FILT ? f.cand : f.nbr, ld, k0, i, nn, js, codes);
       double xj[G][D], vj[G][D], hj[G], mj[G];
+      bool acc[G];
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int j = js[u];
         load_xh<D>(f.xh, j, xj[u], hj[u]);
-        load_vm<D>(f.vmr, j, vj[u], mj[u]);
+        if (!FILT) load_vm<D>(f.vmr, j, vj[u], mj[u]);
+        acc[u] = !FILT && k0 + u < nn;
       }
+      if (FILT) {
+        // the exact neighbour criterion (k_neighbor_filter) on the group's candidates first;
+        // velocities and masses are then read for the accepted ones only
 #pragma unroll
-      for (int u = 0; u < G; ++u) {
-        if (k0 + u >= nn) continue;
-        int code = codes[u];
-        if (FILT) {
-          // the exact neighbour criterion (k_neighbor_filter) on the candidate
+        for (int u = 0; u < G; ++u) {
+          if (k0 + u >= nn) continue;
+          int code = codes[u];
 #pragma unroll
           for (int a = 0; a < D; ++a) {
             if ((g.permask >> a) & 1) {
@@ -563,6 +566,7 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
               code = (code & ~(3 << (2*a))) | (c << (2*a));
             }
           }
+          codes[u] = code;
           if (js[u] == i && code == 0) continue;
           if (code) image_pos<D>(g, code, xj[u]);
           double e2i = 0.0, e2j = 0.0;
@@ -574,6 +578,7 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
             e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
           }
           if (!(e2i <= ph.ext2 || e2j <= ph.ext2)) continue;
+          acc[u] = true;
           const uint32_t e = (uint32_t)js[u] | ((uint32_t)code << kImgShift);
           const int sl = count & 7;
 #pragma unroll
@@ -581,6 +586,14 @@ __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph,
           ++count;
           if (sl == 7 && count <= f.cap) flush_chunk(f.nbr, count - 1, i, ld, chunk);
         }
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+          if (acc[u]) load_vm<D>(f.vmr, js[u], vj[u], mj[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (!acc[u]) continue;
+        const int code = codes[u];
         if (DENS) {
           if (!FILT && code) image_pos<D>(g, code, xj[u]);
           R s = (R)0;

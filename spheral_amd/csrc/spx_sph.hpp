@@ -134,9 +134,12 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
 // normalisation, effective Q, density gradient, first moment, M).
 // ---------------------------------------------------------------------------
 // ldsTable != 0: the table's coefficients are staged in the block's dynamic LDS first (the
-// pair loop makes three lookups per pair at scattered intervals).
+// pair loop makes three lookups per pair at scattered intervals).  sc != nullptr (the step):
+// the node's half-step velocity record for the compatible-energy update is written too (as
+// k_v12 would, with the run's dt/2).
 template <int D, bool FULL, typename R>
-__global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable) {
+__global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable,
+                                                   const Scalars* sc) {
   const R* coef = stage_table<R>(tab, ldsTable);
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -265,6 +268,16 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 #pragma unroll
   for (int a = 0; a < D; ++a) { f.DvDt[a][i] = dvdt[a]; f.DxDt[a][i] = f.vm[a][i]; }
   f.DepsDt[i] = depsdt;
+  if (sc) {
+    const double hdt = 0.5*sc[f.run[i]].dt;
+    double w[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) w[a] = f.v[a][i] + dvdt[a]*hdt;
+    double2* PV = reinterpret_cast<double2*>(f.pkv) + 3*(size_t)i;
+    PV[0] = make_double2(w[0], w[1]);
+    PV[1] = make_double2(w[2], depsdt);
+    PV[2] = make_double2(Sm, 0.0);
+  }
   f.DrhoDt[i] = -Srho*tr;
   f.maxQ[i] = maxq;
   // ---- SPHSmoothingScale finalize (SPHSmoothingScale.cc:228-245)
