@@ -42,6 +42,19 @@ extern "C" {
    Kernel/WendlandC4KernelInline.hh). */
 #define SPX_KERNEL_BSPLINE     0
 #define SPX_KERNEL_WENDLANDC4  1
+/* NBSplineKernel(order) (Kernel/NBSplineKernel.cc:20-110): the centred B-spline with
+   order+1 knots, extent (order+1)/2 in integer arithmetic, normalised by a Simpson
+   volume integral (Kernel/VolumeIntegrationFunctions.cc). Order in kernel_order. */
+#define SPX_KERNEL_NBSPLINE    2
+
+/* Artificial viscosity of the SPH package.  MonaghanGingold: MonaghanGingoldViscosity
+   (ArtificialViscosity/MonaghanGingoldViscosity.cc:41-101).  Limited:
+   LimitedMonaghanGingoldViscosity (ArtificialViscosity/LimitedMonaghanGingoldViscosity.cc:
+   123-219), the SPH() factory's default Q (SPH/SPHHydros.py:88-94): the pair velocity
+   jump is reduced by a van Leer-limited projection of each node's velocity gradient as
+   of the previous derivative evaluation (ArtificialViscosityHandle.cc:168-183). */
+#define SPX_Q_MONAGHAN_GINGOLD          0
+#define SPX_Q_LIMITED_MONAGHAN_GINGOLD  1
 
 /* Per-side boundary condition of one axis (Boundary/PeriodicBoundary.cc,
    Boundary/ReflectingBoundary.cc).  Periodic must be set on both sides. */
@@ -62,7 +75,10 @@ extern "C" {
      gamma : GammaLawGas; Cl, Cq : MonaghanGingoldViscosity
      cfl .. correct_velocity_gradient : SPH ctor (SPH/SPH.hh:40-56, SPH/SPHHydros.py:9-33)
      dt_min .. last_dt : Integrator members (Integrator/Integrator.cc:41-47)
-     G .. ftimestep : TreeGravity ctor (Gravity/TreeGravity.hh:46-50) */
+     G .. ftimestep : TreeGravity ctor (Gravity/TreeGravity.hh:46-50)
+     kernel_order : NBSplineKernel ctor (Kernel/NBSplineKernelInline.hh:19-29)
+     viscosity, eta_crit_frac, eta_fold_frac : the Q and the Limited Q's ctor arguments
+       (PYB11/ArtificialViscosity/LimitedMonaghanGingoldViscosity.py:24-32; defaults 1.0, 0.2) */
 typedef struct spx_config {
   int32_t dim;                       /* 1, 2 or 3                                    */
   int32_t kernel;                    /* SPX_KERNEL_*                                 */
@@ -87,6 +103,9 @@ typedef struct spx_config {
   int32_t gravity;                   /* 1: TreeGravity package appended              */
   int32_t reorder_interval;          /* 0 or 1: particles are re-sorted every step   */
   double  G, softening, opening, ftimestep;
+  int32_t kernel_order;              /* NBSplineKernel order (SPX_KERNEL_NBSPLINE)   */
+  int32_t viscosity;                 /* SPX_Q_*                                      */
+  double  eta_crit_frac, eta_fold_frac; /* LimitedMonaghanGingoldViscosity ctor     */
 } spx_config;
 
 typedef struct spx_run spx_run;
@@ -147,6 +166,15 @@ int spx_run_get_field(spx_run* run, const char* name, double* out, int64_t capac
    package: P and cs (SPH/SPHBase.cc:147-161), grad-h Omega, TreeGravity dry run
    (Gravity/TreeGravity.cc:343-356).  Derivatives start at zero. */
 int spx_run_initialize(spx_run* run);
+
+/* SpheralController.iterateIdealH at start-up (SimulationControl/SpheralController.py:231-232,
+   909-931 -> Utilities/iterateIdealH.cc:28-230, SPHSmoothingScale alone, no spherical start, no
+   fixed determinant): repeat { ghost images + connectivity on the current H; the package's ideal
+   H from the zeroth moment (SmoothingScale/SPHSmoothingScale.cc:154-245); replace H on every node
+   not yet converged } until no node's H moved by more than tolerance, or max_iterations passes
+   (the controller uses 50 and 1e-4).  Call after spx_run_initialize and before the first step;
+   *passes receives the number of passes.  Single runs only. */
+int spx_run_iterate_ideal_h(spx_run* run, int32_t max_iterations, double tolerance, int32_t* passes);
 
 /* Integrator::step(maxTime) with CheapSynchronousRK2::step
    (Integrator/Integrator.cc:67-111, Integrator/CheapSynchronousRK2.cc:40-132).

@@ -48,6 +48,8 @@ def lib():
         L.spxo_table_eval.argtypes = [C.POINTER(SpxConfig), D, C.c_int64, D, D, D]
         L.spxo_table_coefficients.argtypes = [C.POINTER(SpxConfig), I64, D, D, D]
         L.spxo_base_kernel.argtypes = [C.c_int, C.c_int, D, C.c_int64, D, D]
+        L.spxo_base_kernel_order.argtypes = [C.c_int, C.c_int, C.c_int, D, C.c_int64, D, D, D]
+        L.spxo_iterate_ideal_h.argtypes = [P, C.c_int, C.c_double]
         L.spxo_quadinterp.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, D,
                                       C.c_int64, D]
         L.spxo_conservation.argtypes = [P, D]
@@ -110,6 +112,10 @@ class OracleRun:
 
     def initialize(self):
         lib().spxo_initialize(self.h)
+
+    def iterate_ideal_h(self, max_iterations=50, tolerance=1.0e-4):
+        """SpheralController.iterateIdealH at start-up; returns the number of passes."""
+        return lib().spxo_iterate_ideal_h(self.h, int(max_iterations), float(tolerance))
 
     def update_connectivity(self):
         lib().spxo_update_connectivity(self.h)
@@ -198,6 +204,14 @@ def base_kernel(kind, dim, eta):
     gW = np.zeros_like(eta)
     lib().spxo_base_kernel(kind, dim, _dp(eta), eta.size, _dp(W), _dp(gW))
     return W, gW
+
+
+def base_kernel_order(kind, dim, order, eta):
+    """Base kernel value, gradient and second derivative at Hdet = 1 (NBSpline takes the order)."""
+    eta = np.ascontiguousarray(eta, np.float64)
+    W, gW, g2W = np.zeros_like(eta), np.zeros_like(eta), np.zeros_like(eta)
+    lib().spxo_base_kernel_order(kind, dim, order, _dp(eta), eta.size, _dp(W), _dp(gW), _dp(g2W))
+    return W, gW, g2W
 
 
 def quadinterp(c, xmin, xmax, n, xs):

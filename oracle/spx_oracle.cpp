@@ -46,6 +46,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -172,16 +173,52 @@ template<> inline double rootnu<3>(double x) { return sgn(x)*std::pow(std::abs(x
 // ---------------------------------------------------------------------------
 // Base kernels (Kernel/BSplineKernelInline.hh:8-91, Kernel/WendlandC4KernelInline.hh)
 // ---------------------------------------------------------------------------
+// NBSplineKernel (Kernel/NBSplineKernel.cc:20-110, NBSplineKernelInline.hh:19-110): the centred
+// B-spline M_k, k = order + 1, written as the truncated-power sum
+//   sum_{i=0..k} (-1)^i C(k,i) (eta - i + k/2)_+^p / p!,  p = k-1 (value), k-2 (gradient), k-3,
+// times A*Hdet.  The extent is (order+1)/2 in integer arithmetic and A is 1/(Simpson volume
+// integral over [0, extent] with 10000 bins) (Kernel/VolumeIntegrationFunctions.cc,
+// Utilities/simpsonsIntegration.hh).
+static int nbFactorial(int n) { if (n < 0) return INT_MAX; int r = 1; for (int i = 1; i < n + 1; ++i) r *= i; return r; }
+static int nbBinomial(int n, int m) { return nbFactorial(n)/(nbFactorial(n - m)*nbFactorial(m)); }
+static double nbPower(double x, int p) { return x >= 0.0 ? std::pow(x, (double)p) : 0.0; }
+static double nbSum(int order, double eta, int p) {
+  const int k = order + 1; const double halfk = 0.5*k;
+  double result = 0.0;
+  for (int i = 0; i <= k; ++i) result += std::pow(-1.0, (double)i)*nbBinomial(k, i)*nbPower(eta - i + halfk, p);
+  return result;
+}
+
 struct BaseKernel {
-  int kind, dim; double norm, extent;
-  BaseKernel(int kind_, int dim_): kind(kind_), dim(dim_) {
+  int kind, dim, order; double norm, extent;
+  BaseKernel(int kind_, int dim_, int order_ = 5): kind(kind_), dim(dim_), order(order_) {
     if (kind == SPX_KERNEL_BSPLINE) {
       extent = 2.0; norm = (dim == 1 ? 2.0/3.0 : dim == 2 ? 10.0/(7.0*M_PI) : 1.0/M_PI);
+    } else if (kind == SPX_KERNEL_NBSPLINE) {
+      extent = (double)((order + 1)/2);
+      norm = 1.0;
+      const unsigned nb = 10000u; const double dx = (extent - 0.0)/nb;
+      double result = 0.0;
+      for (unsigned i = 0u; i < nb + 1u; ++i) {
+        const double r = 0.0 + i*dx;
+        const double vol = dim == 1 ? 2.0 : dim == 2 ? 2.0*M_PI*r : 4.0*M_PI*r*r;
+        const double integrand = vol*W(r, 1.0);
+        if (i == 0u || i == nb) result += integrand;
+        else if (i % 2u == 0u) result += 2.0*integrand;
+        else result += 4.0*integrand;
+      }
+      result *= dx/3.0;
+      norm = 1.0/result;
     } else {
       extent = 1.0; norm = (dim == 1 ? 3.0/2.0 : dim == 2 ? 9.0/M_PI : 495.0/(32.0*M_PI));
     }
   }
   double W(double eta, double Hdet = 1.0) const {
+    if (kind == SPX_KERNEL_NBSPLINE) {
+      if (eta >= extent) return 0.0;
+      const int p = std::max(0, order);
+      return nbSum(order, eta, p)*(norm*Hdet/nbFactorial(p));
+    }
     if (kind == SPX_KERNEL_BSPLINE) {
       if (eta < 1.0) { const double eta2 = eta*eta; return norm*Hdet*(1.0 - 1.5*eta2 + 0.75*eta2*eta); }
       else if (eta < 2.0) { const double t = 2.0 - eta; return norm*Hdet*0.25*(t*t*t); }
@@ -192,6 +229,11 @@ struct BaseKernel {
     return norm*Hdet*(std::pow(1.0 - eta, 6)*(1.0 + 6.0*eta + (35.0/3.0)*eta2))*(eta < 1.0);
   }
   double gW(double eta, double Hdet = 1.0) const {
+    if (kind == SPX_KERNEL_NBSPLINE) {
+      if (eta >= extent) return 0.0;
+      const int p = std::max(-1, order - 1);
+      return nbSum(order, eta, p)*(norm*Hdet/nbFactorial(p));
+    }
     if (kind == SPX_KERNEL_BSPLINE) {
       if (eta < 1.0) return -norm*Hdet*(3.0 - 2.25*eta)*eta;
       else if (eta < 2.0) { const double t = 2.0 - eta; return -norm*Hdet*0.75*(t*t); }
@@ -201,6 +243,11 @@ struct BaseKernel {
     return norm*Hdet*((56.0/3.0)*std::pow(eta - 1.0, 5)*eta*(5.0*eta + 1.0))*(eta < 1.0);
   }
   double g2W(double eta, double Hdet = 1.0) const {
+    if (kind == SPX_KERNEL_NBSPLINE) {
+      if (eta >= extent) return 0.0;
+      const int p = std::max(-2, order - 2);
+      return nbSum(order, eta, p)*(norm*Hdet/nbFactorial(p));
+    }
     if (kind == SPX_KERNEL_BSPLINE) {
       if (eta < 1.0) return -norm*Hdet*(3 - 4.5*eta);
       else if (eta < 2.0) return norm*Hdet*1.5*(2 - eta);
@@ -330,9 +377,9 @@ struct Table {
     while (etar < extent) { const double q = etar/deta; result += 4.0*M_PI*(q*q)*kernelValueSPH(etar); etar += deta; }
     return std::pow(result, 1.0/3.0);
   }
-  void build(int kind, int dim_, unsigned numPoints, double minN, double maxN) {
+  void build(int kind, int dim_, unsigned numPoints, double minN, double maxN, int order = 5) {
     dim = dim_;
-    BaseKernel k(kind, dim);
+    BaseKernel k(kind, dim, order);
     Wi.init(0.0, k.extent, numPoints, [&](double x) { return k.W(x, 1.0); });
     gWi.init(0.0, k.extent, numPoints, [&](double x) { return k.gW(x, 1.0); });
     g2Wi.init(0.0, k.extent, numPoints, [&](double x) { return k.g2W(x, 1.0); });
@@ -401,6 +448,7 @@ template<int D> struct Sim {
   int64_t nInt = 0, nTot = 0;
   // node fields: internal [0, nInt) then ghosts
   std::vector<V> x, v; std::vector<double> m, rho, eps, P, cs, omega; std::vector<T> H;
+  std::vector<T> DvDxQ;   // LimitedMonaghanGingoldViscosity's state copy of the velocity gradient
   std::vector<int64_t> ctrl; std::vector<int32_t> img;
   // connectivity of internal nodes
   std::vector<std::vector<int64_t>> nbrs; std::vector<std::pair<int64_t, int64_t>> pairs;
@@ -415,7 +463,7 @@ template<int D> struct Sim {
   double dtMinAcc = 0.0, extraEnergy = 0.0;
 
   explicit Sim(const spx_config& c, int64_t n): cfg(c), nInt(n), nTot(n) {
-    W.build(cfg.kernel, D, cfg.kernel_points, cfg.min_nperh, cfg.max_nperh);
+    W.build(cfg.kernel, D, cfg.kernel_points, cfg.min_nperh, cfg.max_nperh, cfg.kernel_order);
     W0 = W.kernelValue(0.0, 1.0);
     lastDt = cfg.last_dt;
     resizeNodes(n);
@@ -423,7 +471,7 @@ template<int D> struct Sim {
   }
   void resizeNodes(int64_t n) {
     x.resize(n); v.resize(n); m.resize(n); rho.resize(n); eps.resize(n); P.resize(n); cs.resize(n);
-    omega.resize(n, 1.0); H.resize(n); ctrl.resize(n); img.resize(n);
+    omega.resize(n, 1.0); H.resize(n); ctrl.resize(n); img.resize(n); DvDxQ.resize(n, mzero<D>());
   }
   template<typename X> static void rs(std::vector<X>& a, int64_t n, const X& z) { a.resize(n, z); }
   void resizeDerivs() {
@@ -491,12 +539,31 @@ template<int D> struct Sim {
     for (int64_t g = nInt; g < nTot; ++g) { x[g] = imagePos(x[ctrl[g]], img[g]); H[g] = imageTen(H[ctrl[g]], img[g]); }
   }
   // SPHBase::applyGhostBoundaries (SPH/SPHBase.cc:556-591)
-  void applyGhostBoundaries() {
+  // The Limited Q's velocity gradient is state the Q registers (ArtificialViscosityHandle.cc:
+  // 96-108): its ghosts follow the tensor boundary rule unless withQ is false.
+  void applyGhostBoundaries(bool withQ = true) {
     updateGhostNodes();
+    const bool lq = withQ && cfg.viscosity == SPX_Q_LIMITED_MONAGHAN_GINGOLD;
     for (int64_t g = nInt; g < nTot; ++g) {
       const int64_t c = ctrl[g];
       m[g] = m[c]; rho[g] = rho[c]; eps[g] = eps[c]; v[g] = imageVec(v[c], img[g]); P[g] = P[c]; cs[g] = cs[c]; omega[g] = omega[c];
+      if (lq) DvDxQ[g] = imageTen(DvDxQ[c], img[g]);
     }
+  }
+  // ArtificialViscosityHandle::postStateUpdate (ArtificialViscosityHandle.cc:168-183) with
+  // rigorousVelocityGradient off: the Q's gradient becomes a copy of the hydro's DvDx from the
+  // last derivative evaluation, ghost values included (zero: the cycle's ghost rebuild cleared
+  // them and the hydro writes internal nodes only).
+  void copyQVelocityGradient() {
+    if (cfg.viscosity != SPX_Q_LIMITED_MONAGHAN_GINGOLD) return;
+    for (int64_t i = 0; i < nTot; ++i) DvDxQ[i] = i < nInt ? DvDx[i] : mzero<D>();
+  }
+  // Integrator::postStateUpdate (Integrator.cc:254-271): packages in order (the Q, then SPH's
+  // grad-h Omega); boundaries are re-applied when a package asks (SPH does with gradh on).
+  void postStateUpdate() {
+    copyQVelocityGradient();
+    gradhOmega();
+    applyGhostBoundaries(cfg.gradh_correction != 0);
   }
   // SPHBase::finalizeDerivatives: ghost accelerations and DepsDt (SPH/SPHBase.cc:499-521)
   void finalizeDerivatives() {
@@ -664,6 +731,8 @@ template<int D> struct Sim {
   void evaluateDerivativesSPH() {
     const double tiny = 1.0e-30, eps2 = 1.0e-2;         // ArtificialViscosityHandle.cc:45
     const double Cl = cfg.Cl, Cq = cfg.Cq;
+    const bool limitedQ = cfg.viscosity == SPX_Q_LIMITED_MONAGHAN_GINGOLD;
+    const double etaCrit = cfg.eta_crit_frac/cfg.nperh, etaFold = cfg.eta_fold_frac/cfg.nperh;
     const int nth = omp_get_max_threads();
     const V z = vzero<D>(); const T zt = mzero<D>();
     TC<double> t_rhoSum(nth, nTot, 0.0), t_norm(nth, nTot, 0.0), t_DepsDt(nth, nTot, 0.0), t_maxQ(nth, nTot, 0.0), t_effQ(nth, nTot, 0.0), t_m0(nth, nTot, 0.0);
@@ -698,8 +767,27 @@ template<int D> struct Sim {
         nm[i] += mi/rhoi*Wi; nm[j] += mj/rhoj*Wj;
         // MonaghanGingoldViscosity::QPiij (MonaghanGingoldViscosity.cc:64-100)
         const V vij = sub(vi, vj);
-        const double mui = dot(vij, etai)/(mag2(etai) + eps2);
-        const double muj = dot(vij, etaj)/(mag2(etaj) + eps2);
+        V vijQ = vij;
+        if (limitedQ) {
+          // LimitedMonaghanGingoldViscosity::QPiij (LimitedMonaghanGingoldViscosity.cc:160-195):
+          // van Leer limiter on the ratio of the two nodes' gradient projections along the
+          // half separation, phased back to the full Q for close pairs, then each node's
+          // velocity is extrapolated toward the midpoint.
+          const T& DvDxi = DvDxQ[i]; const T& DvDxj = DvDxQ[j];
+          const V xij = scale(0.5, rij);
+          const double gradi = dot(mv(DvDxi, xij), xij), gradj = dot(mv(DvDxj, xij), xij);
+          const double rri = gradi/(sgn(gradj)*std::max(1.0e-30, std::abs(gradj)));
+          const double rrj = gradj/(sgn(gradi)*std::max(1.0e-30, std::abs(gradi)));
+          const double rmin = std::min(rri, rrj);
+          double phi = rmin > 0.0 ? 2.0/(1.0 + rmin)*2.0*rmin/(1.0 + rmin) : 0.0;
+          const double etaij = std::min(etaMagi, etaMagj);
+          if (etaij < etaCrit) { const double q = (etaij - etaCrit)/etaFold; phi *= std::exp(-(q*q)); }
+          const V vi1 = sub(vi, mv(mscale(phi, DvDxi), xij));
+          const V vj1 = add(vj, mv(mscale(phi, DvDxj), xij));
+          vijQ = sub(vi1, vj1);
+        }
+        const double mui = dot(vijQ, etai)/(mag2(etai) + eps2);
+        const double muj = dot(vijQ, etaj)/(mag2(etaj) + eps2);
         const double Clij = 0.5*(1.0 + 1.0)*1.0*Cl, Cqij = 0.5*(1.0 + 1.0)*1.0*Cq;
         const double mmi = std::min(0.0, mui), mmj = std::min(0.0, muj);
         const double ei = -Clij*ci*mmi + Cqij*(mmi*mmi);
@@ -764,7 +852,11 @@ template<int D> struct Sim {
       DrhoDt[i] = -rhoi*trace(DvDx[i]);
       DxDt[i] = v[i];
     }
-    // SPHSmoothingScale finalize (SPHSmoothingScale.cc:228-245)
+    smoothingScaleFinalize();
+  }
+
+  // SPHSmoothingScale finalize (SPHSmoothingScale.cc:228-245): m0 holds the pair sums of WSPH.
+  void smoothingScaleFinalize() {
     const double nPerh = cfg.nperh, hmin = cfg.hmin, hmax = cfg.hmax;
 #pragma omp parallel for
     for (int64_t i = 0; i < nInt; ++i) {
@@ -781,6 +873,50 @@ template<int D> struct Sim {
       const double hi1 = std::min(hmax, std::max(hmin, hi0*(1.0 - a + a*s)));
       Hideal[i] = mident<D>(1.0/hi1);
     }
+  }
+
+  // iterateIdealH (Utilities/iterateIdealH.cc:28-230) as SpheralController.iterateIdealH runs it
+  // at start-up (SimulationControl/SpheralController.py:231-232, 909-931: the SPHSmoothingScale
+  // package alone, nPerhForIteration 0, no spherical start, no fixed determinant).  Each pass
+  // rebuilds ghosts and connectivity, evaluates the package's ideal H from the zeroth moment and
+  // replaces H on every node not yet converged; a node is done once its H changes by at most
+  // the tolerance (eigenvalues of H1^1/2 H^-1 H1^1/2 within tolerance of 1).
+  int iterateIdealH(int maxIterations, double tolerance) {
+    std::vector<char> done(nInt, 0);
+    double maxDeltaH = 2.0*tolerance;
+    int itr = 0;
+    while (itr < maxIterations && maxDeltaH > tolerance) {
+      ++itr;
+      maxDeltaH = 0.0;
+      updateConnectivity();
+      applyGhostBoundaries();
+      const T zt = mzero<D>();
+      std::fill(m0.begin(), m0.end(), 0.0); std::fill(DvDx.begin(), DvDx.end(), zt);
+      for (const auto& pr : pairs) {           // SPHSmoothingScale.cc:154-224, same NodeList
+        const int64_t i = pr.first, j = pr.second;
+        const V xij = sub(x[i], x[j]);
+        const double etaMagi = mag(mv(H[i], xij)), etaMagj = mag(mv(H[j], xij));
+        m0[i] += 1.0*W.kernelValueSPH(etaMagi)*1.0;
+        m0[j] += 1.0/1.0*W.kernelValueSPH(etaMagj)*1.0;
+      }
+      smoothingScaleFinalize();
+      for (int64_t i = 0; i < nInt; ++i) {
+        if (done[i]) continue;
+        double phimin = 0.0, phimax = 0.0;
+        for (int a = 0; a < D; ++a) {          // isotropic H and H1: the eigenvalues are the diagonal
+          const double h1s = std::sqrt(Hideal[i].c[a][a]);
+          const double ph = (h1s*(1.0/H[i].c[a][a]))*h1s;
+          phimin = a == 0 ? ph : std::min(phimin, ph); phimax = a == 0 ? ph : std::max(phimax, ph);
+        }
+        const double deltaHi = std::max(std::abs(phimin - 1.0), std::abs(phimax - 1.0));
+        if (deltaHi <= tolerance) done[i] = 1;
+        maxDeltaH = std::max(maxDeltaH, deltaHi);
+        H[i] = Hideal[i];
+      }
+    }
+    updateConnectivity();
+    applyGhostBoundaries();
+    return itr;
   }
 
   // ---- TreeGravity -----------------------------------------------------------
@@ -1022,8 +1158,7 @@ template<int D> struct Sim {
     advanceIncrements(hdt, /*fullStep=*/false);
     t += hdt;
     applyGhostBoundaries();
-    gradhOmega();
-    applyGhostBoundaries();
+    postStateUpdate();
     // derivatives at the midpoint
     if (cfg.gravity) gravityInitialize();
     evaluateDerivatives();
@@ -1036,7 +1171,7 @@ template<int D> struct Sim {
     advanceIncrements(dt, /*fullStep=*/true);
     t = t - hdt + dt;
     applyGhostBoundaries();
-    gradhOmega();
+    postStateUpdate();
     enforceBoundaries();
     applyGhostBoundaries();
     ++cycle; lastDt = dt;
@@ -1121,10 +1256,13 @@ int spxo_set_state(void* p, const double* x, const double* v, const double* m, c
 }
 
 int spxo_initialize(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, s->initialize()); return 0; }
+int spxo_iterate_ideal_h(void* p, int maxIterations, double tolerance) {
+  auto* h = static_cast<Handle*>(p); int r = 0; DISPATCH(h, r = s->iterateIdealH(maxIterations, tolerance)); return r;
+}
 int spxo_update_connectivity(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->updateConnectivity(); s->applyGhostBoundaries(); }); return 0; }
 int spxo_sum_density(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->sumDensity(); s->applyGhostBoundaries(); }); return 0; }
 int spxo_update_eos(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->updateEOS(); s->applyGhostBoundaries(); }); return 0; }
-int spxo_post_state_update(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->applyGhostBoundaries(); s->gradhOmega(); s->applyGhostBoundaries(); }); return 0; }
+int spxo_post_state_update(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->applyGhostBoundaries(); s->postStateUpdate(); }); return 0; }
 int spxo_evaluate_derivatives(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->applyGhostBoundaries(); s->evaluateDerivatives(); }); return 0; }
 int spxo_dt(void* p, double* hydro, double* grav) {
   auto* h = static_cast<Handle*>(p);
@@ -1192,7 +1330,7 @@ int64_t spxo_get_field(void* p, const char* name, double* out) {
 
 // Kernel table evaluation (host): W, gW at Hdet = 1 and nperh(Wsum).
 int spxo_table_eval(const spx_config* cfg, const double* eta, int64_t n, double* W, double* gW, double* nperh) {
-  Table t; t.build(cfg->kernel, cfg->dim, cfg->kernel_points, cfg->min_nperh, cfg->max_nperh);
+  Table t; t.build(cfg->kernel, cfg->dim, cfg->kernel_points, cfg->min_nperh, cfg->max_nperh, cfg->kernel_order);
   for (int64_t i = 0; i < n; ++i) {
     t.kernelAndGradValue(eta[i], 1.0, W[i], gW[i]);
     if (nperh) nperh[i] = t.equivalentNodesPerSmoothingScale(eta[i]);
@@ -1200,7 +1338,7 @@ int spxo_table_eval(const spx_config* cfg, const double* eta, int64_t n, double*
   return 0;
 }
 int spxo_table_coefficients(const spx_config* cfg, int64_t* ncoef, double* wc, double* gwc, double* xstep) {
-  Table t; t.build(cfg->kernel, cfg->dim, cfg->kernel_points, cfg->min_nperh, cfg->max_nperh);
+  Table t; t.build(cfg->kernel, cfg->dim, cfg->kernel_points, cfg->min_nperh, cfg->max_nperh, cfg->kernel_order);
   *ncoef = (int64_t)t.Wi.c.size();
   if (wc) std::memcpy(wc, t.Wi.c.data(), t.Wi.c.size()*sizeof(double));
   if (gwc) std::memcpy(gwc, t.gWi.c.data(), t.gWi.c.size()*sizeof(double));
@@ -1215,10 +1353,14 @@ int spxo_quadinterp(double c0, double c1, double c2, double xmin, double xmax, i
   return 0;
 }
 // Analytic base kernel (closed forms) for pinning the table.
-int spxo_base_kernel(int kind, int dim, const double* eta, int64_t n, double* W, double* gW) {
-  BaseKernel k(kind, dim);
+int spxo_base_kernel_order(int kind, int dim, int order, const double* eta, int64_t n, double* W, double* gW, double* g2W) {
+  BaseKernel k(kind, dim, order);
+  if (g2W) for (int64_t i = 0; i < n; ++i) g2W[i] = k.g2W(eta[i]);
   for (int64_t i = 0; i < n; ++i) { W[i] = k.W(eta[i]); gW[i] = k.gW(eta[i]); }
   return 0;
+}
+int spxo_base_kernel(int kind, int dim, const double* eta, int64_t n, double* W, double* gW) {
+  return spxo_base_kernel_order(kind, dim, 5, eta, n, W, gW, nullptr);
 }
 // Conservation totals (SpheralConservation.py)
 int spxo_conservation(void* p, double* out) {

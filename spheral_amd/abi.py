@@ -18,6 +18,10 @@ SPX_ERR_NUMERIC = -6
 
 SPX_KERNEL_BSPLINE = 0
 SPX_KERNEL_WENDLANDC4 = 1
+SPX_KERNEL_NBSPLINE = 2
+
+SPX_Q_MONAGHAN_GINGOLD = 0
+SPX_Q_LIMITED_MONAGHAN_GINGOLD = 1
 
 SPX_BC_NONE = 0
 SPX_BC_PERIODIC = 1
@@ -27,6 +31,14 @@ SPX_PREC_FP64 = 0
 SPX_PREC_MIXED = 1
 
 KERNEL_EXTENT = {SPX_KERNEL_BSPLINE: 2.0, SPX_KERNEL_WENDLANDC4: 1.0}
+
+
+def kernel_extent(kind, order=5):
+    """The base kernel's extent; NBSplineKernel sets (order + 1)/2 in integer arithmetic
+    (Kernel/NBSplineKernel.cc:99)."""
+    if kind == SPX_KERNEL_NBSPLINE:
+        return float((int(order) + 1) // 2)
+    return KERNEL_EXTENT[kind]
 
 
 class SpxConfig(C.Structure):
@@ -65,6 +77,10 @@ class SpxConfig(C.Structure):
         ("softening", C.c_double),
         ("opening", C.c_double),
         ("ftimestep", C.c_double),
+        ("kernel_order", C.c_int32),
+        ("viscosity", C.c_int32),
+        ("eta_crit_frac", C.c_double),
+        ("eta_fold_frac", C.c_double),
     ]
 
     def copy(self):
@@ -93,7 +109,7 @@ def default_config(dim=3, kernel=SPX_KERNEL_BSPLINE, **overrides):
     c.min_nperh = 0.25
     c.max_nperh = 64.0
     c.nperh = 2.01
-    c.neighbor_extent = KERNEL_EXTENT[kernel]
+    c.neighbor_extent = kernel_extent(kernel)
     c.hmin = 1.0e-20
     c.hmax = 1.0e20
     c.rho_min = 1.0e-10
@@ -121,6 +137,10 @@ def default_config(dim=3, kernel=SPX_KERNEL_BSPLINE, **overrides):
     c.softening = 0.01
     c.opening = 0.5
     c.ftimestep = 0.1
+    c.kernel_order = 5
+    c.viscosity = SPX_Q_MONAGHAN_GINGOLD
+    c.eta_crit_frac = 1.0
+    c.eta_fold_frac = 0.2
     for k, v in overrides.items():
         if k in ("bc_lo", "bc_hi", "xmin", "xmax"):
             arr = getattr(c, k)
@@ -131,7 +151,7 @@ def default_config(dim=3, kernel=SPX_KERNEL_BSPLINE, **overrides):
         else:
             setattr(c, k, v)
     if "neighbor_extent" not in overrides:
-        c.neighbor_extent = KERNEL_EXTENT[c.kernel]
+        c.neighbor_extent = kernel_extent(c.kernel, c.kernel_order)
     return c
 
 

@@ -169,7 +169,7 @@ class Batch {
     validate();
     HIP_CHECK(hipSetDevice(device));
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    table.build(cfg.kernel, D, (unsigned)cfg.kernel_points, cfg.min_nperh, cfg.max_nperh);
+    table.build(cfg.kernel, D, (unsigned)cfg.kernel_points, cfg.min_nperh, cfg.max_nperh, cfg.kernel_order);
     uploadTable();
     setPhys();
     mixed = cfg.precision == SPX_PREC_MIXED;
@@ -196,7 +196,14 @@ class Batch {
     if (cfg.dim < 1 || cfg.dim > 3) raise(SPX_ERR_ARG, "dim must be 1, 2 or 3");
     if (cap < 1 || cap > (int64_t)kIdxMask) raise(SPX_ERR_CAPACITY, "particle capacity must be in [1, 2^26)");
     if (maxRuns < 1 || maxRuns > 32768) raise(SPX_ERR_CAPACITY, "run slots must be in [1, 32768]");
-    if (cfg.kernel != SPX_KERNEL_BSPLINE && cfg.kernel != SPX_KERNEL_WENDLANDC4) raise(SPX_ERR_ARG, "unknown kernel");
+    if (cfg.kernel != SPX_KERNEL_BSPLINE && cfg.kernel != SPX_KERNEL_WENDLANDC4 && cfg.kernel != SPX_KERNEL_NBSPLINE)
+      raise(SPX_ERR_ARG, "unknown kernel");
+    if (cfg.kernel == SPX_KERNEL_NBSPLINE && (cfg.kernel_order < 2 || cfg.kernel_order > 9))
+      raise(SPX_ERR_ARG, "NBSplineKernel order must be in [2, 9]");
+    if (cfg.viscosity != SPX_Q_MONAGHAN_GINGOLD && cfg.viscosity != SPX_Q_LIMITED_MONAGHAN_GINGOLD)
+      raise(SPX_ERR_ARG, "unknown viscosity");
+    if (cfg.viscosity == SPX_Q_LIMITED_MONAGHAN_GINGOLD && !(cfg.eta_fold_frac > 0.0))
+      raise(SPX_ERR_ARG, "eta_fold_frac must be positive");
     if (cfg.kernel_points < 3) raise(SPX_ERR_ARG, "kernel_points must be >= 3");
     if (!cfg.compatible_energy) raise(SPX_ERR_UNSUPPORTED, "only compatibleEnergyEvolution=True is implemented");
     if (cfg.precision != SPX_PREC_FP64 && cfg.precision != SPX_PREC_MIXED) raise(SPX_ERR_ARG, "unknown precision");
@@ -280,6 +287,9 @@ class Batch {
     ph.hmaxInv = 1.0/cfg.hmax;
     ph.hminInv = 1.0/cfg.hmin;
     ph.correctVG = cfg.correct_velocity_gradient;
+    ph.qImages = 1;
+    ph.etaCrit = cfg.eta_crit_frac/cfg.nperh;
+    ph.etaFold = cfg.eta_fold_frac/cfg.nperh;
     ph.useVmagDt = cfg.use_velocity_magnitude_for_dt;
     ph.gravity = cfg.gravity;
     geo.permask = 0;
@@ -336,6 +346,14 @@ class Batch {
     off = 0;
     for (double** p : cl) { *p = slabAlt + off; off += ld; }
     alt.DvDx = slabAlt + off;
+    F.DvDxQ = alt.DvDxQ = nullptr;
+    if (limitedQ()) {
+      const size_t q = (size_t)D*D*ld;
+      F.DvDxQ = dalloc<double>(q);
+      alt.DvDxQ = dalloc<double>(q);
+      HIP_CHECK(hipMemsetAsync(F.DvDxQ, 0, q*sizeof(double), stream));
+      HIP_CHECK(hipMemsetAsync(alt.DvDxQ, 0, q*sizeof(double), stream));
+    }
     f.xh = dalloc<double>((size_t)4*ld);
     f.vmr = dalloc<double>((size_t)4*ld);
     f.xhm = dalloc<double>((size_t)4*ld);
@@ -515,6 +533,8 @@ class Batch {
     // derivatives start at zero (SPHBase::initializeProblemStartup)
     for (double* p : {f.DepsDt, f.DrhoDt, f.DHDt, f.maxQ, f.pot, f.P, f.cs}) zero(p);
     for (int c = 0; c < D*D; ++c) HIP_CHECK(hipMemsetAsync(f.DvDx + (size_t)c*ld + o, 0, n*sizeof(double), stream));
+    if (limitedQ())
+      for (int c = 0; c < D*D; ++c) HIP_CHECK(hipMemsetAsync(f.DvDxQ + (size_t)c*ld + o, 0, n*sizeof(double), stream));
     std::vector<int64_t> ids(n);
     std::vector<int32_t> rr(n, slot);
     for (int64_t i = 0; i < n; ++i) ids[i] = ((int64_t)slot << 40) | i;
@@ -735,11 +755,14 @@ class Batch {
     gs.count = 0;
     for (size_t k = 0; k < src.size(); ++k) { gs.src[gs.count] = *src[k]; gs.dst[gs.count] = *dst[k]; ++gs.count; }
     for (int c = 0; c < D*D; ++c) { gs.src[gs.count] = f.DvDx + (size_t)c*ld; gs.dst[gs.count] = alt.DvDx + (size_t)c*ld; ++gs.count; }
+    if (limitedQ())
+      for (int c = 0; c < D*D; ++c) { gs.src[gs.count] = f.DvDxQ + (size_t)c*ld; gs.dst[gs.count] = alt.DvDxQ + (size_t)c*ld; ++gs.count; }
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, stream, gs, valsAlt, (int)nTotal);
     hipLaunchKernelGGL(k_gather_ids, dim3(nb), dim3(kBlock), 0, stream, f.oid, oidAlt, f.run, runAlt, valsAlt, (int)nTotal);
     HIP_CHECK(hipGetLastError());
     for (size_t k = 0; k < src.size(); ++k) std::swap(*src[k], *dst[k]);
     std::swap(f.DvDx, alt.DvDx);
+    std::swap(f.DvDxQ, alt.DvDxQ);
     std::swap(f.oid, oidAlt);
     std::swap(f.run, runAlt);
     // segments in key order: active runs by slot, then finished runs by slot
@@ -839,6 +862,11 @@ class Batch {
       there.src[there.count] = f.DvDx + (size_t)c*ld; there.dst[there.count++] = alt.DvDx + (size_t)c*ld;
       back.src[back.count] = alt.DvDx + (size_t)c*ld; back.dst[back.count++] = f.DvDx + (size_t)c*ld;
     }
+    if (limitedQ())
+      for (int c = 0; c < D*D; ++c) {
+        there.src[there.count] = f.DvDxQ + (size_t)c*ld; there.dst[there.count++] = alt.DvDxQ + (size_t)c*ld;
+        back.src[back.count] = alt.DvDxQ + (size_t)c*ld; back.dst[back.count++] = f.DvDxQ + (size_t)c*ld;
+      }
     const int nb = nblocks(nsub);
     hipLaunchKernelGGL(k_gather_list, dim3(nb), dim3(kBlock), 0, stream, there, (const uint32_t*)spos,
                        (const uint32_t*)sposAlt, (const int64_t*)f.oid, oidAlt, (const uint64_t*)skeysAlt, keysAlt,
@@ -1059,15 +1087,28 @@ class Batch {
     });
   }
   // withV12: the step's pass, which also writes the half-step velocity records.
-  void sphPass(bool full, bool withV12 = false) {
+  // stepMid: the integrator's midpoint evaluation, where the Limited Q's image gradients are
+  // what Integrator::postStateUpdate left: tensor images with grad-h on (SPH asks for boundaries
+  // after the Q's copy), zero with it off.  Package calls see boundaries applied.
+  void sphPass(bool full, bool withV12 = false, bool stepMid = false) {
     const int nb = (int)xcd_grid(nActive, kBlock);
+    Phys ph = this->ph;
+    ph.qImages = stepMid ? (cfg.gradh_correction ? 1 : 0) : 1;
     const size_t tabBytes = tableBytes();
     const int useLds = tabBytes <= kLdsTableMax ? 1 : 0;
     const size_t shm = useLds ? tabBytes : 0;
     const Scalars* sc = withV12 ? (const Scalars*)dsc : nullptr;
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      if (full) {
+      if (limitedQ()) {
+        if (full) {
+          if (mixed) hipLaunchKernelGGL((k_sph<DD, true, float, true>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
+          else hipLaunchKernelGGL((k_sph<DD, true, double, true>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
+        } else {
+          if (mixed) hipLaunchKernelGGL((k_sph<DD, false, float, true>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
+          else hipLaunchKernelGGL((k_sph<DD, false, double, true>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
+        }
+      } else if (full) {
         if (mixed) hipLaunchKernelGGL((k_sph<DD, true, float>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
         else hipLaunchKernelGGL((k_sph<DD, true, double>), dim3(nb), dim3(kBlock), shm, stream, f, geo, ph, dtab, (int)nActive, useLds, sc);
       } else {
@@ -1159,11 +1200,55 @@ class Batch {
     for (int a = 0; a < D; ++a) { z(f.DxDt[a]); z(f.DvDt[a]); z(f.DvDtG[a]); }
     for (double* p : {f.DepsDt, f.DrhoDt, f.DHDt, f.maxQ}) z(p);
     for (int c = 0; c < D*D; ++c) z(f.DvDx + (size_t)c*ld);
+    if (limitedQ()) for (int c = 0; c < D*D; ++c) z(f.DvDxQ + (size_t)c*ld);
     for (auto& r : runs) if (r.state == RUN_ACTIVE) r.needInit = true;
     const bool fresh = prepareNewRuns();
     buildConnectivity(false);
     if (fresh) finishNewRuns();
     initialized = true;
+  }
+
+  // SpheralController.iterateIdealH at start-up (SimulationControl/SpheralController.py:231-232,
+  // 909-931; Utilities/iterateIdealH.cc:28-230) for a single run: each pass rebuilds the images
+  // and the connectivity on the current H, then k_ideal_h replaces H on the nodes not yet done.
+  // Ends with the connectivity of the final H.  Returns the number of passes.
+  int iterateIdealH(int maxIterations, double tolerance) {
+    if (!initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called first");
+    if (maxRuns != 1) raise(SPX_ERR_UNSUPPORTED, "iterateIdealH runs on a single run");
+    if (maxIterations < 0 || !(tolerance >= 0.0)) raise(SPX_ERR_ARG, "bad iteration count or tolerance");
+    int* ddone = nullptr;
+    unsigned long long* dmax = nullptr;
+    HIP_CHECK(hipMalloc(&ddone, std::max<int64_t>(nTotal, 1)*sizeof(int)));
+    HIP_CHECK(hipMalloc(&dmax, sizeof(unsigned long long)));
+    int itr = 0;
+    try {
+      HIP_CHECK(hipMemsetAsync(ddone, 0, nTotal*sizeof(int), stream));
+      double maxDeltaH = 2.0*tolerance;
+      while (itr < maxIterations && maxDeltaH > tolerance) {
+        ++itr;
+        buildConnectivity(false, true);
+        HIP_CHECK(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), stream));
+        launchD([&](auto d) {
+          constexpr int DD = decltype(d)::value;
+          hipLaunchKernelGGL(k_ideal_h<DD>, dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f, geo, ph,
+                             dtab, (int)nActive, tolerance, ddone, dmax);
+        });
+        HIP_CHECK(hipGetLastError());
+        unsigned long long bits = 0;
+        download(&bits, dmax, sizeof(bits));
+        std::memcpy(&maxDeltaH, &bits, sizeof(double));
+      }
+      buildConnectivity(false, true);
+    } catch (...) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(ddone);
+      (void)hipFree(dmax);
+      throw;
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(hipFree(ddone));
+    HIP_CHECK(hipFree(dmax));
+    return itr;
   }
 
   // One Integrator::step of every active run.
@@ -1197,7 +1282,10 @@ class Batch {
     mark("omega");
     // derivatives at the midpoint
     if (cfg.gravity) { gravityPass(true); mark("gravity"); }
-    sphPass(false, true);
+    // the Limited Q's postStateUpdate copy: its gradient becomes the previous evaluation's DvDx,
+    // which this evaluation then overwrites
+    if (limitedQ()) std::swap(f.DvDx, f.DvDxQ);
+    sphPass(false, true, true);
     mark("sph");
     // full step from the start-of-step state, compatible energy first
     launchD([&](auto d) {
@@ -1237,6 +1325,16 @@ class Batch {
     checkSlot(slot);
     hsc[slot].tEnd = tEnd;
     upload(&dsc[slot].tEnd, &hsc[slot].tEnd, sizeof(double));
+  }
+
+  bool limitedQ() const { return cfg.viscosity == SPX_Q_LIMITED_MONAGHAN_GINGOLD; }
+  // ArtificialViscosityHandle::postStateUpdate (ArtificialViscosityHandle.cc:168-183): the
+  // Limited Q's gradient becomes a copy of the hydro's DvDx of the last evaluation.
+  void copyQVelocityGradient() {
+    if (!limitedQ() || nTotal == 0) return;
+    for (int c = 0; c < D*D; ++c)
+      HIP_CHECK(hipMemcpyAsync(f.DvDxQ + (size_t)c*ld, f.DvDx + (size_t)c*ld, nTotal*sizeof(double),
+                               hipMemcpyDeviceToDevice, stream));
   }
 
   // ---------------------------------------------------------------- single-run package calls

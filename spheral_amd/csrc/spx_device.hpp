@@ -113,7 +113,11 @@ struct RunGrid {
 struct Phys {
   double gamma, g1, Cl, Cq, cfl, nperh, ext, ext2, hmin, hmax, rho_min, rho_max;
   double hminInv, hmaxInv;
+  double etaCrit, etaFold;   // LimitedMonaghanGingoldViscosity: etaCritFrac/nPerh, etaFoldFrac/nPerh
   int correctVG, useVmagDt, gravity;
+  // the Limited Q's gradient on image partners: 1 the control node's tensor image (boundaries
+  // applied after the Q's copy), 0 zero (the integrator's midpoint with grad-h off)
+  int qImages;
 };
 
 struct Fields {
@@ -125,6 +129,9 @@ struct Fields {
   double *v1[3];
   // derivatives carried across steps
   double *DxDt[3], *DvDt[3], *DvDtG[3], *DepsDt, *DrhoDt, *DHDt, *DvDx, *maxQ;
+  // the Limited Q's copy of DvDx from the previous derivative evaluation (D*D planes; null
+  // unless that Q is selected)
+  double *DvDxQ;
   // derivatives consumed within a step
   double *Hideal, *m0;
   // evaluateDerivatives outputs not consumed by the integration

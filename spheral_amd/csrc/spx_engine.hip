@@ -98,6 +98,10 @@ int spx_config_default(int32_t dim, spx_config* c) {
   c->softening = 0.01;
   c->opening = 0.5;
   c->ftimestep = 0.1;
+  c->kernel_order = 5;
+  c->viscosity = SPX_Q_MONAGHAN_GINGOLD;
+  c->eta_crit_frac = 1.0;
+  c->eta_fold_frac = 0.2;
   return SPX_OK;
 }
 
@@ -112,7 +116,7 @@ int spx_table_eval(const spx_config* cfg, const double* eta, int64_t n, double* 
   return guarded([&] {
     if (!cfg || !eta || !W || !gW) spx::raise(SPX_ERR_ARG, "null argument");
     spx::KernelTable t;
-    t.build(cfg->kernel, cfg->dim, (unsigned)cfg->kernel_points, cfg->min_nperh, cfg->max_nperh);
+    t.build(cfg->kernel, cfg->dim, (unsigned)cfg->kernel_points, cfg->min_nperh, cfg->max_nperh, cfg->kernel_order);
     for (int64_t i = 0; i < n; ++i) {
       t.kernelAndGrad(eta[i], 1.0, W[i], gW[i]);
       if (nperh) nperh[i] = t.nodesPerh(eta[i]);
@@ -124,7 +128,7 @@ int spx_table_coefficients(const spx_config* cfg, int64_t* ncoef, double* wc, do
   return guarded([&] {
     if (!cfg || !ncoef) spx::raise(SPX_ERR_ARG, "null argument");
     spx::KernelTable t;
-    t.build(cfg->kernel, cfg->dim, (unsigned)cfg->kernel_points, cfg->min_nperh, cfg->max_nperh);
+    t.build(cfg->kernel, cfg->dim, (unsigned)cfg->kernel_points, cfg->min_nperh, cfg->max_nperh, cfg->kernel_order);
     *ncoef = (int64_t)t.W.coef.size();
     if (wc) std::memcpy(wc, t.W.coef.data(), t.W.coef.size()*sizeof(double));
     if (gwc) std::memcpy(gwc, t.gW.coef.data(), t.gW.coef.size()*sizeof(double));
@@ -244,6 +248,15 @@ int spx_run_initialize(spx_run* run) {
   });
 }
 
+int spx_run_iterate_ideal_h(spx_run* run, int32_t max_iterations, double tolerance, int32_t* passes) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    RUN_STATE(run);
+    const int k = run->impl->iterateIdealH((int)max_iterations, tolerance);
+    if (passes) *passes = k;
+  });
+}
+
 int spx_run_step(spx_run* run, double max_time, double* dt_out) {
   RUN_OR_FAIL(run);
   return guarded([&] {
@@ -325,6 +338,7 @@ int spx_run_post_state_update(spx_run* run) {
     RUN_STATE(run);
     auto& B = *run->impl;
     B.requireConn();
+    B.copyQVelocityGradient();
     B.pack();
     B.omegaPass(false);
     HIP_CHECK(hipStreamSynchronize(B.stream));

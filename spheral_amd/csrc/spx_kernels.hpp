@@ -95,8 +95,8 @@ __global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, const RunGrid*
 
 // Gather a set of arrays through a permutation (dst[p] = src[perm[p]]).
 struct GatherSet {
-  const double* src[48];
-  double* dst[48];
+  const double* src[64];
+  double* dst[64];
   int count;
 };
 __global__ void __launch_bounds__(kBlock) k_gather(GatherSet s, const uint32_t* perm, int n) {
@@ -805,6 +805,52 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
   }
   omega[i] = om;
   if (MID) write_mid_record<D>(f, i, om);
+}
+
+// One pass of iterateIdealH (Utilities/iterateIdealH.cc:118-176) with the SPHSmoothingScale
+// package alone (SmoothingScale/SPHSmoothingScale.cc:154-245) on the connectivity state: the
+// zeroth moment of kernelValueSPH over the node's neighbours (images included) gives the ideal h;
+// a node not yet converged takes it, and is flagged done once its H moved by at most tol.  The
+// flags are indexed by the node's run-local id, so they survive the reorders between passes.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_ideal_h(Fields f, Geo g, Phys ph, DevTable tab, int n, double tol,
+                                                    int* done, unsigned long long* maxDelta) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t id = f.oid[i] & (((int64_t)1 << 40) - 1);
+  if (done[id]) return;
+  const double* coef = table_coef<double>(tab);
+  double xi[D], hi;
+  load_xh<D>(f.xh, i, xi, hi);
+  const size_t ld = (size_t)f.ld;
+  double m0 = 0.0;
+  const int nn = f.cnt[i];
+  for (int k = 0; k < nn; ++k) {
+    const uint32_t e = f.nbr[ell_at(k, i, ld)];
+    const int j = (int)(e & kIdxMask), code = (int)(e >> kImgShift);
+    double xj[D], hj;
+    load_xh<D>(f.xh, j, xj, hj);
+    if (code) image_pos<D>(g, code, xj);
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) { const double r = hi*(xi[a] - xj[a]); s += r*r; }
+    double w, gw, graw;
+    kernel_wgw_c<double>(coef, tab, sqrt(s), hdet_iso<D, double>(hi), w, gw, graw);
+    m0 += fabs(graw);
+  }
+  m0 = rootnu<D>(fmax(0.0, m0));
+  const double cur = (fabs(m0) <= 1.0e-15*fmax(1.0, fabs(m0))) ? 0.5*ph.nperh : nperh_lookup(tab, m0);
+  const double sr = fmin(4.0, fmax(0.25, ph.nperh/(cur + 1.0e-30)));
+  const double aa = sr < 1.0 ? 0.4*(1.0 + sr*sr) : 0.4*(1.0 + 1.0/(sr*sr*sr));
+  const double hi1 = fmin(ph.hmax, fmax(ph.hmin, (1.0/hi)*(1.0 - aa + aa*sr)));
+  const double H1 = 1.0/hi1;
+  // eigenvalues of H1^1/2 H^-1 H1^1/2 (isotropic: one value)
+  const double sq = sqrt(H1);
+  const double dH = fabs((sq*(1.0/hi))*sq - 1.0);
+  if (dH <= tol) done[id] = 1;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(dH);
+  if (b > *maxDelta) atomicMax(maxDelta, b);
+  f.hinv[i] = H1;
 }
 
 // Per-run extents for the next grid: min and max hinv and the bounding box, as ordered bits.

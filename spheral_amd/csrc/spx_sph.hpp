@@ -75,9 +75,13 @@ struct PairS {
 };
 
 // coef: the kernel table's coefficients in arithmetic R (global, or the block's LDS copy).
-template <int D, typename R>
+// LQ: LimitedMonaghanGingoldViscosity::QPiij (LimitedMonaghanGingoldViscosity.cc:160-195) in
+// place of the plain Q; TS, TO are the two nodes' velocity gradients as the Q holds them
+// (row-major D x D, the partner's already image-transformed).
+template <int D, typename R, bool LQ = false>
 __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, const Phys& ph, const Nbr<D, R>& S,
-                                          const Nbr<D, R>& O, bool aFirst, PairS<D, R>& e) {
+                                          const Nbr<D, R>& O, bool aFirst, PairS<D, R>& e,
+                                          const R* TS = nullptr, const R* TO = nullptr) {
   R eo[D], e2s = (R)0, e2o = (R)0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
@@ -98,13 +102,47 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
   kernel_wgw_c<R>(coef, tab, etaMagO, hdet_iso<D, R>(O.h), e.Wo, gwo, grawo);
   e.wsphS = fabs(graws);                                      // kernelValueSPH(etaMagS)
   const R ss = gws*S.h, so = gwo*O.h;                        // (gW*H)*etaUnit
+  // velocity jump the Q sees: the plain difference, or the limited midpoint extrapolation
+  R vq[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) vq[c] = e.vso[c];
+  if (LQ) {
+    // van Leer limiter on the ratio of the gradients' projections along the half separation,
+    // phased back toward the full Q when the pair is closer than etaCrit
+    R xij[D], gi = (R)0, gj = (R)0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) xij[c] = (R)0.5*e.r[c];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      R s1 = TS[a*D]*xij[0], s2 = TO[a*D]*xij[0];
+#pragma unroll
+      for (int b = 1; b < D; ++b) { s1 += TS[a*D + b]*xij[b]; s2 += TO[a*D + b]*xij[b]; }
+      gi = a == 0 ? s1*xij[0] : gi + s1*xij[a];
+      gj = a == 0 ? s2*xij[0] : gj + s2*xij[a];
+    }
+    const R tiny30 = sizeof(R) == sizeof(double) ? (R)1.0e-30 : (R)1.0e-30f;
+    const R ri = gi/((gj < (R)0 ? (R)-1 : (R)1)*fmax(tiny30, fabs(gj)));
+    const R rj = gj/((gi < (R)0 ? (R)-1 : (R)1)*fmax(tiny30, fabs(gi)));
+    const R rmin = fmin(ri, rj);
+    R phi = rmin > (R)0 ? (R)2/((R)1 + rmin)*(R)2*rmin/((R)1 + rmin) : (R)0;
+    const R etaij = fmin(e.etaMagS, etaMagO), ec = (R)ph.etaCrit;
+    if (etaij < ec) { const R q = (etaij - ec)/(R)ph.etaFold; phi *= exp(-(q*q)); }
+    // (phi*T).xij: the product with the scaled tensor, as the reference forms it
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      R s1 = (phi*TS[a*D])*xij[0], s2 = (phi*TO[a*D])*xij[0];
+#pragma unroll
+      for (int b = 1; b < D; ++b) { s1 += (phi*TS[a*D + b])*xij[b]; s2 += (phi*TO[a*D + b])*xij[b]; }
+      vq[a] = (R)((S.v[a] - (double)s1) - (O.v[a] + (double)s2));
+    }
+  }
   R mus = (R)0, muo = (R)0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
     e.gs[c] = ss*(e.es[c]*is);
     e.go[c] = so*(eo[c]*io);
-    mus += e.vso[c]*e.es[c];
-    muo += e.vso[c]*eo[c];
+    mus += vq[c]*e.es[c];
+    muo += vq[c]*eo[c];
   }
   mus = mus/(e2s + (R)1.0e-2);
   muo = muo/(e2o + (R)1.0e-2);
@@ -137,7 +175,7 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
 // pair loop makes three lookups per pair at scattered intervals).  sc != nullptr (the step):
 // the node's half-step velocity record for the compatible-energy update is written too (as
 // k_v12 would, with the run's dt/2).
-template <int D, bool FULL, typename R>
+template <int D, bool FULL, typename R, bool LQ = false>
 __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable,
                                                    const Scalars* sc) {
   const R* coef = stage_table<R>(tab, ldsTable);
@@ -156,6 +194,11 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 #pragma unroll
   for (int a = 0; a < D*D; ++a) { dvdx[a] = 0.0; Mc[a] = 0.0; }
 
+  R TS[LQ ? D*D : 1];
+  if (LQ) {
+#pragma unroll
+    for (int a = 0; a < D*D; ++a) TS[a] = (R)f.DvDxQ[a*ld + i];
+  }
   const int nn = f.cnt[i];
   // the ELL chunk (8 entries) and the next neighbour's record are fetched one pair ahead
   uint4 c0 = make_uint4(0u, 0u, 0u, 0u), c1 = c0;
@@ -187,7 +230,23 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
     Nbr<D, R> O;
     finish_mid<D, R>(g, code, rc, O);
     PairS<D, R> e;
-    pair_eval<D, R>(tab, coef, ph, S, O, code != 0 || i < j, e);
+    if (LQ) {
+      // the partner's gradient: ghosts carry the tensor image (reflected axes flip sign per
+      // index) when boundaries were applied after the Q's copy; otherwise the zero the cycle's
+      // ghost rebuild left (ArtificialViscosityHandle.cc:168-183, Integrator.cc:254-271)
+      R TO[D*D];
+      const bool img = code != 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+          const double t = f.DvDxQ[(size_t)(a*D + b)*ld + j];
+          TO[a*D + b] = img ? (ph.qImages ? (R)(refl_sign<D>(g, code, a)*refl_sign<D>(g, code, b)*t) : (R)0) : (R)t;
+        }
+      pair_eval<D, R, true>(tab, coef, ph, S, O, code != 0 || i < j, e, TS, TO);
+    } else {
+      pair_eval<D, R>(tab, coef, ph, S, O, code != 0 || i < j, e);
+    }
 #pragma unroll
     for (int c = 0; c < D; ++c) pdd[((size_t)c*cap + k)*ld + i] = e.dd[c];
     const double mo = rc.m;

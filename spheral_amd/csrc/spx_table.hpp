@@ -3,7 +3,9 @@
 // Builds the per-interval parabola coefficients of W and dW/deta and the
 // Wsum -> nPerh cubic-Hermite lookup exactly as the reference constructs its
 // TableKernel (Kernel/TableKernel.cc:169-209):
-//   * base kernels: Kernel/BSplineKernelInline.hh:8-91, Kernel/WendlandC4KernelInline.hh:8-95
+//   * base kernels: Kernel/BSplineKernelInline.hh:8-91, Kernel/WendlandC4KernelInline.hh:8-95,
+//     Kernel/NBSplineKernel.cc:20-110 (normalisation: Kernel/VolumeIntegrationFunctions.cc,
+//     Utilities/simpsonsIntegration.hh)
 //   * QuadraticInterpolator: Utilities/QuadraticInterpolator.cc:59-100 (odd sample count,
 //     3-point exact parabola per interval), QuadraticInterpolatorInline.hh:19-35
 //   * nPerh lookups: Kernel/TableKernel.cc:23-66 (sumKernelValues), CubicHermiteInterpolator
@@ -20,13 +22,42 @@
 
 namespace spx {
 
+// NBSplineKernel (Kernel/NBSplineKernel.cc:20-110): centred B-spline with k = order + 1 knots as
+// the truncated-power sum  sum_{i=0..k} (-1)^i C(k,i) (e - i + k/2)_+^p / p!  (p = k-1 for the
+// value, k-2 for the derivative), in the reference's operation order.
+inline int nb_factorial(int n) { int r = 1; for (int i = 1; i < n + 1; ++i) r *= i; return r; }
+inline double nb_sum(int order, double e, int p) {
+  const int k = order + 1;
+  const double halfk = 0.5*k;
+  double acc = 0.0;
+  for (int i = 0; i <= k; ++i) {
+    const double u = e - i + halfk;
+    acc += std::pow(-1.0, (double)i)*(nb_factorial(k)/(nb_factorial(k - i)*nb_factorial(i)))*
+           (u >= 0.0 ? std::pow(u, (double)p) : 0.0);
+  }
+  return acc;
+}
+
 struct KernelShape {
-  int kind, dim;
+  int kind, dim, order;
   double norm, extent;
-  KernelShape(int kind_, int dim_) : kind(kind_), dim(dim_) {
+  KernelShape(int kind_, int dim_, int order_ = 5) : kind(kind_), dim(dim_), order(order_) {
     if (kind == 0) {  // BSpline
       extent = 2.0;
       norm = dim == 1 ? 2.0/3.0 : (dim == 2 ? 10.0/(7.0*M_PI) : 1.0/M_PI);
+    } else if (kind == 2) {  // NBSpline: extent (order+1)/2 in integers, A = 1/Simpson volume integral
+      extent = (double)((order + 1)/2);
+      norm = 1.0;
+      const unsigned bins = 10000u;
+      const double dx = extent/bins;
+      double acc = 0.0;
+      for (unsigned i = 0u; i < bins + 1u; ++i) {
+        const double r = 0.0 + i*dx;
+        const double y = (dim == 1 ? 2.0 : dim == 2 ? 2.0*M_PI*r : 4.0*M_PI*r*r)*value(r);
+        acc += (i == 0u || i == bins) ? y : (i % 2u == 0u ? 2.0*y : 4.0*y);
+      }
+      acc *= dx/3.0;
+      norm = 1.0/acc;
     } else {          // WendlandC4
       extent = 1.0;
       norm = dim == 1 ? 1.5 : (dim == 2 ? 9.0/M_PI : 495.0/(32.0*M_PI));
@@ -35,6 +66,7 @@ struct KernelShape {
   // Value, first and second derivative at Hdet = 1, written in the base kernels' own
   // arithmetic order so the tabulated samples are the same doubles.
   double value(double e) const {
+    if (kind == 2) return e >= extent ? 0.0 : nb_sum(order, e, order)*(norm*1.0/nb_factorial(order));
     if (kind == 0) {
       if (e < 1.0) { const double e2 = e*e; return norm*1.0*(1.0 - 1.5*e2 + 0.75*e2*e); }
       if (e < 2.0) { const double t = 2.0 - e; return norm*1.0*0.25*(t*t*t); }
@@ -45,6 +77,7 @@ struct KernelShape {
     return norm*1.0*(std::pow(1.0 - e, 6)*(1.0 + 6.0*e + (35.0/3.0)*e2))*(e < 1.0);
   }
   double grad(double e) const {
+    if (kind == 2) return e >= extent ? 0.0 : nb_sum(order, e, order - 1)*(norm*1.0/nb_factorial(order - 1));
     if (kind == 0) {
       if (e < 1.0) return -norm*1.0*(3.0 - 2.25*e)*e;
       if (e < 2.0) { const double t = 2.0 - e; return -norm*1.0*0.75*(t*t); }
@@ -182,9 +215,9 @@ struct KernelTable {
     for (; e < extent; e += d) { const double q = e/d; acc += 4.0*M_PI*(q*q)*Wsph(e); }
     return std::pow(acc, 1.0/3.0);
   }
-  void build(int kind, int dim_, unsigned numPoints, double minN, double maxN) {
+  void build(int kind, int dim_, unsigned numPoints, double minN, double maxN, int order = 5) {
     dim = dim_;
-    const KernelShape k(kind, dim);
+    const KernelShape k(kind, dim, order);
     W.fit(0.0, k.extent, numPoints, [&](double x) { return k.value(x); });
     gW.fit(0.0, k.extent, numPoints, [&](double x) { return k.grad(x); });
     extent = k.extent;

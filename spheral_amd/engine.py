@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libspheral_amd.so")
 EXPORTS = (
     "spx_last_error", "spx_config_default", "spx_device_count", "spx_table_eval", "spx_table_coefficients",
     "spx_run_create", "spx_run_destroy", "spx_run_set_state", "spx_run_get_field", "spx_run_initialize",
+    "spx_run_iterate_ideal_h",
     "spx_run_step", "spx_run_advance", "spx_run_time", "spx_run_update_connectivity", "spx_run_neighbor_counts",
     "spx_run_neighbors", "spx_run_sum_density", "spx_run_update_eos", "spx_run_post_state_update",
     "spx_run_evaluate_derivatives", "spx_run_dt", "spx_run_compatible_energy", "spx_run_conservation",
@@ -60,6 +61,7 @@ def lib():
             "spx_run_set_state": (C.c_int, [P, D, D, D, D, D, D]),
             "spx_run_get_field": (C.c_int, [P, C.c_char_p, D, C.c_int64, I64]),
             "spx_run_initialize": (C.c_int, [P]),
+            "spx_run_iterate_ideal_h": (C.c_int, [P, C.c_int32, C.c_double, C.POINTER(C.c_int32)]),
             "spx_run_step": (C.c_int, [P, C.c_double, D]),
             "spx_run_advance": (C.c_int, [P, C.c_double, C.c_int64, I64, D]),
             "spx_run_time": (C.c_int, [P, D, I64, D]),
@@ -166,6 +168,12 @@ class Run:
 
     def initialize(self):
         _check(lib().spx_run_initialize(self.h))
+
+    def iterate_ideal_h(self, max_iterations=50, tolerance=1.0e-4):
+        """SpheralController.iterateIdealH at start-up; returns the number of passes."""
+        k = C.c_int32()
+        _check(lib().spx_run_iterate_ideal_h(self.h, int(max_iterations), float(tolerance), C.byref(k)))
+        return k.value
 
     def step(self, max_time=1.0e300):
         dt = C.c_double()

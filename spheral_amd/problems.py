@@ -76,6 +76,36 @@ def sod_1d(n=400, nperh=1.51, gamma=1.4, rho1=1.0, P1=1.0, rho2=0.125, P2=0.1, x
     return cfg, dict(x=x.reshape(n, 1), v=np.zeros((n, 1)), m=np.full(n, mi), rho=rho, eps=eps, H=H)
 
 
+def noh_planar_1d(nx1=100, nperh=1.35, order=5, gamma=5.0 / 3.0, rho1=1.0, eps1=0.0, x0=0.0, x1=1.0, vr0=-1.0):
+    """The reference's planar Noh test as Noh-planar-1d.py sets it up with its defaults
+    (tests/functional/Hydro/Noh/Noh-planar-1d.py:98-180, 330-560): NBSplineKernel(5) in a
+    1000-point TableKernel, nx1 nodes from GenerateNodeDistribution1d (x_i = x0 + (i+1/2)dx,
+    m = rho dx, h = nPerh dx), eps = 0, v = vr0 toward a reflecting wall at x0, the SPH()
+    factory defaults (LimitedMonaghanGingold Q with Cl = extent, Cq = extent^2/2,
+    SPHHydros.py:88-94), cfl 0.5, compatible energy, grad-h, corrected velocity gradient,
+    CheapSynchronousRK2 with lastDt 1e-4, dtMin 1e-5, dtMax 0.1, dtGrowth 2, hmin 1e-4,
+    hmax 0.1.  The run starts with SpheralController's iterateIdealH (50 passes, 1e-4)."""
+    dx = (x1 - x0) / nx1
+    h0 = nperh * dx
+    x = x0 + (np.arange(nx1) + 0.5) * dx
+    m = np.full(nx1, rho1 * dx)
+    H = _iso_H(nx1, 1, 1.0 / h0)
+    v = np.where(x > x0, vr0, -vr0)
+    extent = abi.kernel_extent(abi.SPX_KERNEL_NBSPLINE, order)
+    cfg = abi.default_config(1, kernel=abi.SPX_KERNEL_NBSPLINE, kernel_order=order, nperh=nperh, gamma=gamma,
+                             hmin=1.0e-4, hmax=0.1, cfl=0.5, Cl=2.0 * (extent / 2.0), Cq=2.0 * (extent / 2.0) ** 2,
+                             viscosity=abi.SPX_Q_LIMITED_MONAGHAN_GINGOLD, dt_min=1.0e-5, dt_max=0.1,
+                             dt_growth=2.0, last_dt=1.0e-4)
+    cfg.kernel = abi.SPX_KERNEL_NBSPLINE
+    cfg.neighbor_extent = extent
+    cfg.bc_lo[0] = abi.SPX_BC_REFLECTING
+    cfg.xmin[0] = x0
+    cfg.bc_hi[0] = abi.SPX_BC_NONE
+    cfg.xmax[0] = x1
+    return cfg, dict(x=x.reshape(nx1, 1), v=v.reshape(nx1, 1), m=m, rho=np.full(nx1, rho1),
+                     eps=np.full(nx1, eps1), H=H)
+
+
 def kelvin_helmholtz_2d(n_target=4096, seed=7, nperh=2.0, gamma=5.0 / 3.0, P0=2.5,
                         kernel=abi.SPX_KERNEL_BSPLINE, precision=abi.SPX_PREC_MIXED):
     """2-D periodic KH: lattice of spacing sqrt(1/N) with the band |y-0.5|<0.25 at twice the

@@ -50,8 +50,9 @@ def test_bad_arguments_report_errors():
 def test_engine_table_is_the_oracle_table(oracle_mod):
     """The engine builds its tables on the host in FP64; they must be the oracle's doubles."""
     for dim in (1, 2, 3):
-        for kern in (abi.SPX_KERNEL_BSPLINE, abi.SPX_KERNEL_WENDLANDC4):
-            cfg = abi.default_config(dim, kernel=kern)
+        for kern, order in ((abi.SPX_KERNEL_BSPLINE, 5), (abi.SPX_KERNEL_WENDLANDC4, 5), (abi.SPX_KERNEL_NBSPLINE, 3),
+                            (abi.SPX_KERNEL_NBSPLINE, 5), (abi.SPX_KERNEL_NBSPLINE, 7)):
+            cfg = abi.default_config(dim, kernel=kern, kernel_order=order)
             cfg.kernel = kern
             wo, go, xo = oracle_mod.table_coefficients(cfg)
             n = C.c_int64()
@@ -61,7 +62,7 @@ def test_engine_table_is_the_oracle_table(oracle_mod):
             dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
             assert engine.lib().spx_table_coefficients(C.byref(cfg), C.byref(n), dp(we), dp(ge), C.byref(xe)) == 0
             assert np.array_equal(we, wo) and np.array_equal(ge, go) and xe.value == xo
-            eta = np.linspace(0.0, 1.05*abi.KERNEL_EXTENT[kern], 997)
+            eta = np.linspace(0.0, 1.05*abi.kernel_extent(kern, order), 997)
             W1, G1, N1 = oracle_mod.table_eval(cfg, eta)
             W2, G2, N2 = np.zeros_like(eta), np.zeros_like(eta), np.zeros_like(eta)
             assert engine.lib().spx_table_eval(C.byref(cfg), dp(eta), eta.size, dp(W2), dp(G2), dp(N2)) == 0

@@ -106,3 +106,59 @@ def test_nperh_lookup_is_monotone_and_inverts_wsum(oracle_mod):
         grid = np.linspace(0.5, 5.0, 400)
         _, _, nph = oracle_mod.table_eval(cfg, grid)
         assert np.all(np.diff(nph) >= -1e-12)
+
+
+def m6(q):
+    """Centred quintic B-spline (six knots), piecewise polynomial form."""
+    q = np.abs(np.asarray(q, float))
+    return np.where(q < 1, (66 - 60*q**2 + 30*q**4 - 10*q**5)/120.0,
+                    np.where(q < 2, (51 + 75*q - 210*q**2 + 150*q**3 - 45*q**4 + 5*q**5)/120.0,
+                             np.where(q < 3, (3 - q)**5/120.0, 0.0)))
+
+
+def dm6(q):
+    q = np.abs(np.asarray(q, float))
+    return np.where(q < 1, (-120*q + 120*q**3 - 50*q**4)/120.0,
+                    np.where(q < 2, (75 - 420*q + 450*q**2 - 180*q**3 + 25*q**4)/120.0,
+                             np.where(q < 3, -5*(3 - q)**4/120.0, 0.0)))
+
+
+def _gauss_volume(f, ext, dim):
+    """Volume integral of f(r) over [0, ext] by 8-point Gauss-Legendre on each unit interval
+    (exact for the piecewise polynomials below)."""
+    x, w = np.polynomial.legendre.leggauss(8)
+    tot = 0.0
+    for a in range(int(ext)):
+        r = a + 0.5*(x + 1.0)
+        shell = {1: 2.0*np.ones_like(r), 2: 2*math.pi*r, 3: 4*math.pi*r**2}[dim]
+        tot += 0.5*np.sum(w*shell*f(r))
+    return tot
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_nbspline5_is_the_quintic_bspline(oracle_mod, dim):
+    """NBSplineKernel(5): the truncated-power sum equals the quintic B-spline's piecewise form,
+    the gradient is its derivative, the extent is 3, and the Simpson normalisation makes the
+    volume integral one (checked by Gauss quadrature, exact for the pieces)."""
+    eta = np.linspace(0.0, 3.3, 661)
+    W, gW, _ = oracle_mod.base_kernel_order(abi.SPX_KERNEL_NBSPLINE, dim, 5, eta)
+    A = 1.0/_gauss_volume(m6, 3, dim)
+    # the truncated-power sum cancels terms up to 6^5/5! ~ 65 near the edge: ~1e-13 absolute
+    assert np.allclose(W, A*m6(eta), rtol=1e-11, atol=1e-12*A)
+    assert np.allclose(gW, A*dm6(eta), rtol=1e-10, atol=1e-11*A)
+    assert np.all(W[eta >= 3.0] == 0.0)
+    if dim == 1:
+        assert abs(A - 1.0) < 1e-12          # the B-spline already integrates to one on the line
+    Wt, _, _ = oracle_mod.table_eval(abi.default_config(dim, kernel=abi.SPX_KERNEL_NBSPLINE, kernel_order=5), eta)
+    assert np.max(np.abs(Wt - A*m6(eta))) <= 1e-8*A*m6(0.0)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_nbspline3_is_the_cubic_bspline_kernel(oracle_mod, dim):
+    """The four-knot NBSpline is the cubic B-spline: after normalisation it must be the
+    BSplineKernel (a different closed form with its own constants) in every dimension."""
+    eta = np.linspace(0.0, 2.2, 441)
+    W3, gW3, _ = oracle_mod.base_kernel_order(abi.SPX_KERNEL_NBSPLINE, dim, 3, eta)
+    Wb, gWb = analytic(abi.SPX_KERNEL_BSPLINE, dim, eta)
+    assert np.allclose(W3, Wb, rtol=1e-11, atol=1e-13)
+    assert np.allclose(gW3, gWb, rtol=1e-10, atol=1e-12)
