@@ -1,7 +1,7 @@
 """Reference-named Python surface of the SPH + TreeGravity path.
 
 A Spheral script builds its problem from a handful of objects -- TableKernel,
-makeFluidNodeList, DataBase, MonaghanGingoldViscosity, SPH, TreeGravity, boundaries,
+makeFluidNodeList, DataBase, (Limited)MonaghanGingoldViscosity, SPH, TreeGravity, boundaries,
 CheapSynchronousRK2Integrator, SpheralController -- and then calls
 ``control.advance(goalTime)``.  This module provides those names with the reference's
 keyword arguments and defaults, and maps them onto one engine run (spx_run) on the GPU.
@@ -13,6 +13,9 @@ Reference signatures followed (paths relative to the reference's src/):
   TableKernel(kernel, numPoints=100, minNperh=0.25, maxNperh=64)    PYB11/Kernel/Kernel.py:260-275
   makeFluidNodeList(name, eos, ..., nPerh=2.01, kernelExtent=2.0)   NodeList/FluidNodeLists.py:15-35
   MonaghanGingoldViscosity(Clinear, Cquadratic, ...)                PYB11/ArtificialViscosity/MonaghanGingoldViscosity.py:25-30
+  LimitedMonaghanGingoldViscosity(Clinear, Cquadratic, ...,
+      etaCritFrac=1.0, etaFoldFrac=0.2)                             PYB11/ArtificialViscosity/LimitedMonaghanGingoldViscosity.py:24-32
+  NBSplineKernel(order)                                             Kernel/NBSplineKernel.hh:20-30
   SPH(W, WPi=None, ..., XSPH=True, ...)                             SPH/SPHHydros.py:9-33
   TreeGravity(G, softeningLength, opening, ftimestep, ...)          PYB11/Gravity/TreeGravity.py:23-28
   SpheralController(integrator, kernel=None, ...).advance(goal)    SimulationControl/SpheralController.py:23-57,383-426
@@ -100,10 +103,23 @@ class WendlandC4Kernel(_BaseKernel):
     extent = 1.0
 
 
+class NBSplineKernel(_BaseKernel):
+    """Kernel/NBSplineKernel(order): centred B-spline with order + 1 knots, extent (order + 1)/2
+    in integer arithmetic (Kernel/NBSplineKernel.cc:99)."""
+    code = abi.SPX_KERNEL_NBSPLINE
+
+    def __init__(self, order):
+        if not 2 <= int(order) <= 9:
+            raise NotImplementedError("NBSplineKernel orders 2..9 are implemented on the GPU path")
+        self.order = int(order)
+        self.extent = abi.kernel_extent(abi.SPX_KERNEL_NBSPLINE, self.order)
+
+
 class TableKernel:
     def __init__(self, kernel, numPoints=100, minNperh=0.25, maxNperh=64.0):
         if not isinstance(kernel, _BaseKernel):
-            raise NotImplementedError("TableKernel on the GPU path supports BSplineKernel and WendlandC4Kernel")
+            raise NotImplementedError("TableKernel on the GPU path supports BSplineKernel, WendlandC4Kernel and "
+                                      "NBSplineKernel")
         self.base = kernel
         self.numPoints = int(numPoints)
         self.minNperh = float(minNperh)
@@ -207,13 +223,51 @@ class DataBase:
     def numSolidNodeLists(self):
         return 0
 
+    @property
+    def maxKernelExtent(self):
+        return max(nl.kernelExtent for nl in self.nodeLists)
+
 
 class MonaghanGingoldViscosity:
+    code = abi.SPX_Q_MONAGHAN_GINGOLD
+
     def __init__(self, Clinear=1.0, Cquadratic=1.0, kernel=None, linearInExpansion=False, quadraticInExpansion=False):
-        if linearInExpansion or quadraticInExpansion:
-            raise NotImplementedError("MonaghanGingoldViscosity in-expansion terms are not implemented")
         self.Cl = float(Clinear)
         self.Cq = float(Cquadratic)
+        # ArtificialViscosityHandle defaults (ArtificialViscosityHandle.cc:43-50); checked when the
+        # run is configured, so a script may set them as the reference's scripts do
+        self.linearInExpansion = bool(linearInExpansion)
+        self.quadraticInExpansion = bool(quadraticInExpansion)
+        self.epsilon2 = 1.0e-2
+        self.balsaraShearCorrection = False
+        self.rigorousVelocityGradient = False
+
+    def requireVelocityGradient(self):
+        return False
+
+    def _check(self):
+        if self.linearInExpansion or self.quadraticInExpansion:
+            raise NotImplementedError(f"{type(self).__name__} in-expansion terms are not implemented")
+        if self.epsilon2 != 1.0e-2:
+            raise NotImplementedError("a Q epsilon2 other than 1e-2 is not implemented")
+        if self.balsaraShearCorrection:
+            raise NotImplementedError("the Balsara shear correction is not implemented")
+        if self.rigorousVelocityGradient:
+            raise NotImplementedError("rigorousVelocityGradient is not implemented")
+
+
+class LimitedMonaghanGingoldViscosity(MonaghanGingoldViscosity):
+    """ArtificialViscosity/LimitedMonaghanGingoldViscosity: the SPH() factory's default Q."""
+    code = abi.SPX_Q_LIMITED_MONAGHAN_GINGOLD
+
+    def __init__(self, Clinear=1.0, Cquadratic=1.0, kernel=None, linearInExpansion=False, quadraticInExpansion=False,
+                 etaCritFrac=1.0, etaFoldFrac=0.2):
+        super().__init__(Clinear, Cquadratic, kernel, linearInExpansion, quadraticInExpansion)
+        self.etaCritFrac = float(etaCritFrac)
+        self.etaFoldFrac = float(etaFoldFrac)
+
+    def requireVelocityGradient(self):
+        return True
 
 
 class _Package:
@@ -255,7 +309,10 @@ def SPH(W, WPi=None, WGrad=None, dataBase=None, Q=None, filter=None, cfl=0.25, u
             raise NotImplementedError(f"SPH: {what} is not implemented on the GPU path")
     no(dataBase is None, "a DataBase is required;")
     no(dataBase.numSolidNodeLists > 0, "solid NodeLists")
-    no(Q is None, "the default LimitedMonaghanGingoldViscosity (pass Q=MonaghanGingoldViscosity(...))")
+    if Q is None:
+        # SPH/SPHHydros.py:88-94 (Cl, Cq pass through "%g" there: exact for these values)
+        ext = dataBase.maxKernelExtent
+        Q = LimitedMonaghanGingoldViscosity(Clinear=2.0*(ext/2.0), Cquadratic=2.0*(ext/2.0)**2, kernel=W)
     no(not isinstance(Q, MonaghanGingoldViscosity), f"viscosity {type(Q).__name__}")
     no(WPi is not None and WPi is not W, "a separate WPi kernel")
     no(not compatibleEnergyEvolution, "compatibleEnergyEvolution=False")
@@ -333,7 +390,8 @@ class CheapSynchronousRK2Integrator:
             if not same:
                 raise NotImplementedError("NodeLists with different materials or node parameters")
         W = h.W
-        cfg = abi.default_config(dim, kernel=W.base.code, nperh=n0.nPerh)
+        h.Q._check()
+        cfg = abi.default_config(dim, kernel=W.base.code, kernel_order=getattr(W.base, "order", 5), nperh=n0.nPerh)
         cfg.kernel_points = W.numPoints
         cfg.min_nperh, cfg.max_nperh = W.minNperh, W.maxNperh
         cfg.neighbor_extent = n0.kernelExtent
@@ -341,6 +399,9 @@ class CheapSynchronousRK2Integrator:
         cfg.rho_min, cfg.rho_max = n0.rhoMin, n0.rhoMax
         cfg.gamma = n0.eos.gamma
         cfg.Cl, cfg.Cq = h.Q.Cl, h.Q.Cq
+        cfg.viscosity = h.Q.code
+        if isinstance(h.Q, LimitedMonaghanGingoldViscosity):
+            cfg.eta_crit_frac, cfg.eta_fold_frac = h.Q.etaCritFrac, h.Q.etaFoldFrac
         cfg.cfl = h.opts["cfl"]
         cfg.use_velocity_magnitude_for_dt = int(h.opts["useVelocityMagnitudeForDt"])
         cfg.compatible_energy = 1
@@ -433,14 +494,21 @@ class SpheralController:
             raise NotImplementedError("restart files are not implemented on the GPU path")
         if vizBaseName is not None:
             raise NotImplementedError("visualisation dumps are not implemented on the GPU path")
-        if iterateInitialH:
-            raise NotImplementedError("iterateIdealH at start-up is not implemented (pass iterateInitialH=False)")
         if initialTime != 0.0:
             raise NotImplementedError("a non-zero initial time is not implemented")
         self.integrator = integrator
         self.kernel = kernel
         self.totalSteps = 0
         integrator.initializeProblemStartup()
+        if iterateInitialH:
+            self.iterateIdealH()
+
+    def iterateIdealH(self, maxIdealHIterations=50, idealHTolerance=1.0e-4):
+        """SpheralController.iterateIdealH (SpheralController.py:909-931) -> spx_run_iterate_ideal_h."""
+        k = self.integrator._run.iterate_ideal_h(maxIdealHIterations, idealHTolerance)
+        self.integrator._stale = True
+        self.integrator._download()
+        return k
 
     def time(self):
         return self.integrator.currentTime
@@ -479,6 +547,7 @@ def namespace(dim):
     ns = {
         "Vector": Vector, "Plane": Plane, "PeriodicBoundary": PeriodicBoundary,
         "ReflectingBoundary": ReflectingBoundary, "BSplineKernel": BSplineKernel, "WendlandC4Kernel": WendlandC4Kernel,
+        "NBSplineKernel": NBSplineKernel, "LimitedMonaghanGingoldViscosity": LimitedMonaghanGingoldViscosity,
         "TableKernel": TableKernel, "GammaLawGas": GammaLawGas, "GammaLawGasMKS": GammaLawGasMKS,
         "GammaLawGasCGS": GammaLawGasCGS, "makeFluidNodeList": mk_nl, "DataBase": mk_db,
         "MonaghanGingoldViscosity": MonaghanGingoldViscosity, "SPH": SPH, "TreeGravity": TreeGravity,
@@ -487,6 +556,7 @@ def namespace(dim):
         "IdealH": IdealH, "IntegrateH": IntegrateH,
     }
     for base in ("Vector", "Plane", "PeriodicBoundary", "ReflectingBoundary", "BSplineKernel", "WendlandC4Kernel",
+                 "NBSplineKernel", "LimitedMonaghanGingoldViscosity",
                  "TableKernel", "makeFluidNodeList", "DataBase", "MonaghanGingoldViscosity", "TreeGravity",
                  "CheapSynchronousRK2Integrator", "SPH"):
         ns[base + sfx] = ns[base]

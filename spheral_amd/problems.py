@@ -101,7 +101,7 @@ def noh_planar_1d(nx1=100, nperh=1.35, order=5, gamma=5.0 / 3.0, rho1=1.0, eps1=
     cfg.bc_lo[0] = abi.SPX_BC_REFLECTING
     cfg.xmin[0] = x0
     cfg.bc_hi[0] = abi.SPX_BC_NONE
-    cfg.xmax[0] = x1
+    cfg.xmax[0] = 1.0e100           # open side: no plane
     return cfg, dict(x=x.reshape(nx1, 1), v=v.reshape(nx1, 1), m=m, rho=np.full(nx1, rho1),
                      eps=np.full(nx1, eps1), H=H)
 

@@ -72,16 +72,13 @@ def test_reference_defaults_that_the_gpu_path_lacks_raise():
     q = S1.MonaghanGingoldViscosity(1.0, 1.0)
     with pytest.raises(NotImplementedError):
         S1.SPH(dataBase=db, W=WT, Q=q)                     # factory default XSPH=True
-    with pytest.raises(NotImplementedError):
-        S1.SPH(dataBase=db, W=WT, XSPH=False)              # default Q is LimitedMonaghanGingold
+    assert isinstance(S1.SPH(dataBase=db, W=WT, XSPH=False).Q, S1.LimitedMonaghanGingoldViscosity)
     with pytest.raises(NotImplementedError):
         S1.SPH(dataBase=db, W=WT, Q=q, XSPH=False, evolveTotalEnergy=True)
     with pytest.raises(NotImplementedError):
         S1.SPH(dataBase=db, W=WT, Q=q, XSPH=False, densityUpdate=S1.IntegrateDensity)
     with pytest.raises(NotImplementedError):
-        S1.MonaghanGingoldViscosity(1.0, 1.0, linearInExpansion=True)
-    with pytest.raises(NotImplementedError):
-        S1.SpheralController(integ)                         # iterateInitialH=True default
+        S1.MonaghanGingoldViscosity(1.0, 1.0, linearInExpansion=True)._check()
     with pytest.raises(NotImplementedError):
         S1.Plane(S1.Vector(0.0), S1.Vector(0.5)).axis_side(1)
 
@@ -119,3 +116,51 @@ def test_periodic_box_and_gravity_map_to_config():
     assert list(cfg.xmin) == [0.0]*3 and list(cfg.xmax) == [1.0]*3
     assert cfg.kernel == abi.SPX_KERNEL_WENDLANDC4 and cfg.neighbor_extent == 1.0 and cfg.nperh == 2.0
     assert cfg.gravity == 1 and cfg.G == 1.0 and cfg.softening == 0.01 and cfg.opening == 0.5
+
+
+def noh_script():
+    """Planar Noh set-up written as tests/functional/Hydro/Noh/Noh-planar-1d.py builds it with its
+    defaults: NBSplineKernel(5), the SPH() factory's default Q, a reflecting plane at x = 0."""
+    cfg_ref, st = problems.noh_planar_1d()
+    eos = S1.GammaLawGasMKS(5.0/3.0, 1.0)
+    WT = S1.TableKernel(S1.NBSplineKernel(5), 1000)
+    nodes1 = S1.makeFluidNodeList("nodes1", eos, numInternal=len(st["m"]), hmin=0.0001, hmax=0.1, nPerh=1.35,
+                                  kernelExtent=WT.kernelExtent)
+    for attr, key in (("positions", "x"), ("velocity", "v"), ("mass", "m"), ("massDensity", "rho"),
+                      ("specificThermalEnergy", "eps"), ("Hfield", "H")):
+        getattr(nodes1, attr)()[...] = st[key]
+    db = S1.DataBase()
+    db.appendNodeList(nodes1)
+    hydro = S1.SPH(dataBase=db, W=WT, cfl=0.5, useVelocityMagnitudeForDt=False, compatibleEnergyEvolution=True,
+                   evolveTotalEnergy=False, gradhCorrection=True, XSPH=False, correctVelocityGradient=True,
+                   densityUpdate=S1.RigorousSumDensity, epsTensile=0.0, nTensile=4.0, HUpdate=S1.IdealH, ASPH=False)
+    hydro.appendBoundary(S1.ReflectingBoundary(S1.Plane(S1.Vector(0.0), S1.Vector(1.0))))
+    integ = S1.CheapSynchronousRK2Integrator(db)
+    integ.appendPhysicsPackage(hydro)
+    integ.lastDt = 0.0001
+    integ.dtMin = 1.0e-5
+    integ.dtMax = 0.1
+    integ.dtGrowth = 2.0
+    return cfg_ref, st, integ, hydro, nodes1
+
+
+def test_noh_script_config_matches_problem_config():
+    """The factory's default Q is LimitedMonaghanGingold with Cl = extent, Cq = extent^2/2 (3, 4.5
+    for NBSpline(5)), and the script maps onto the same configuration as problems.noh_planar_1d."""
+    cfg_ref, _, integ, hydro, _ = noh_script()
+    assert isinstance(hydro.Q, S1.LimitedMonaghanGingoldViscosity)
+    assert (hydro.Q.Cl, hydro.Q.Cq) == (3.0, 4.5) and hydro.Q.requireVelocityGradient()
+    cfg = integ.config()
+    for name in FIELDS:
+        a, b = getattr(cfg, name), getattr(cfg_ref, name)
+        if hasattr(a, "__len__"):
+            assert list(a) == list(b), name
+        else:
+            assert a == b, name
+
+
+def test_unsupported_q_options_raise():
+    _, _, integ, hydro, _ = noh_script()
+    hydro.Q.balsaraShearCorrection = True
+    with pytest.raises(NotImplementedError):
+        integ.config()
