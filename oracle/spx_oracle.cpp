@@ -9,6 +9,8 @@
  * of the per-pair formulas and the pair-list (scatter) structure with OpenMP
  * per-thread copies:
  *   kernels/tables   Kernel/BSplineKernelInline.hh, Kernel/WendlandC4KernelInline.hh,
+ *                    Kernel/NBSplineKernel.cc:20-110, Kernel/VolumeIntegrationFunctions.cc,
+ *                    Utilities/simpsonsIntegration.hh,
  *                    Kernel/TableKernel.cc:18-66,169-209, Kernel/TableKernelViewInline.hh,
  *                    Utilities/QuadraticInterpolator.cc:59-100, QuadraticInterpolatorInline.hh:19-35,
  *                    Utilities/CubicHermiteInterpolator.cc, CubicHermiteInterpolatorViewInline.hh,
@@ -20,7 +22,10 @@
  *                    Neighbor/NodePairIdxType.hh:34-53
  *   sum density      SPH/computeSPHSumMassDensity.cc:15-87
  *   grad-h Omega     SPH/computeSPHOmegaGradhCorrection.cc:105-205
- *   SPH derivatives  SPH/SPH.cc:165-555; ArtificialViscosity/MonaghanGingoldViscosity.cc:41-101
+ *   SPH derivatives  SPH/SPH.cc:165-555; ArtificialViscosity/MonaghanGingoldViscosity.cc:41-101,
+ *                    ArtificialViscosity/LimitedMonaghanGingoldViscosity.cc:123-219,
+ *                    ArtificialViscosity/ArtificialViscosityHandle.cc:96-183 (the Q's DvDx copy)
+ *   start-up H       Utilities/iterateIdealH.cc:28-230, SimulationControl/SpheralController.py:231,909-931
  *   smoothing scale  SmoothingScale/SPHSmoothingScale.cc:101-245
  *   dt               Physics/GenericHydro.cc:112-381, Integrator/Integrator.cc:116-169
  *   compat. energy   Hydro/SpecificThermalEnergyPolicy.cc:50-175
@@ -36,7 +41,12 @@
  * O(N^2) neighbour sets, the perfect-lattice 32-neighbour count, momentum and
  * energy conservation, linear-field velocity gradients, an isolated-node Omega,
  * uniform-lattice sum density, closed-form dt on a uniform state, and direct
- * O(N^2) summation for TreeGravity at opening -> 0.
+ * O(N^2) summation for TreeGravity at opening -> 0; the reference's planar Noh
+ * acceptance test (tests/functional/Hydro/Noh/Noh-planar-1d.py: fifteen error norms
+ * at 1e-5 and energy at 1e-13) for the Limited Q, NBSpline, reflecting images and
+ * the integrator end to end; NBSpline against the quintic and cubic B-splines'
+ * closed forms; the Limited Q vanishing in a linear flow.  iterateIdealH is pinned
+ * by its fixed-point properties only (the Noh golden is insensitive to it).
  * Parity unpinned (no reference build or vectors exist): the bit-level table
  * coefficients (the reference inverts with Eigen 3x3), the CubicHermite gradient
  * knots (Eigen SparseLU; here a tridiagonal solve), OpenMP summation order.
