@@ -56,6 +56,13 @@ extern "C" {
 #define SPX_Q_MONAGHAN_GINGOLD          0
 #define SPX_Q_LIMITED_MONAGHAN_GINGOLD  1
 
+/* Time integrator.  CheapSynchronousRK2 (Integrator/CheapSynchronousRK2.cc:40-132): one
+   derivative evaluation per step, at the midpoint.  SynchronousRK2 (Integrator/SynchronousRK2.cc:
+   40-135): evaluations at the start and at the midpoint of every step; the engine runs it for
+   TreeGravity-only problems (gravity_only = 1). */
+#define SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2  0
+#define SPX_INTEGRATOR_SYNCHRONOUS_RK2        1
+
 /* Per-side boundary condition of one axis (Boundary/PeriodicBoundary.cc,
    Boundary/ReflectingBoundary.cc).  Periodic must be set on both sides. */
 #define SPX_BC_NONE        0
@@ -78,7 +85,9 @@ extern "C" {
      G .. ftimestep : TreeGravity ctor (Gravity/TreeGravity.hh:46-50)
      kernel_order : NBSplineKernel ctor (Kernel/NBSplineKernelInline.hh:19-29)
      viscosity, eta_crit_frac, eta_fold_frac : the Q and the Limited Q's ctor arguments
-       (PYB11/ArtificialViscosity/LimitedMonaghanGingoldViscosity.py:24-32; defaults 1.0, 0.2) */
+       (PYB11/ArtificialViscosity/LimitedMonaghanGingoldViscosity.py:24-32; defaults 1.0, 0.2)
+     integrator, gravity_only : the integrator class and its package list (an N-body script
+       appends TreeGravity alone, tests/functional/Gravity/CollisionlessSphereCollapse.py:178-186) */
 typedef struct spx_config {
   int32_t dim;                       /* 1, 2 or 3                                    */
   int32_t kernel;                    /* SPX_KERNEL_*                                 */
@@ -106,6 +115,8 @@ typedef struct spx_config {
   int32_t kernel_order;              /* NBSplineKernel order (SPX_KERNEL_NBSPLINE)   */
   int32_t viscosity;                 /* SPX_Q_*                                      */
   double  eta_crit_frac, eta_fold_frac; /* LimitedMonaghanGingoldViscosity ctor     */
+  int32_t integrator;                /* SPX_INTEGRATOR_*                             */
+  int32_t gravity_only;              /* 1: TreeGravity is the only package (no SPH)  */
 } spx_config;
 
 typedef struct spx_run spx_run;

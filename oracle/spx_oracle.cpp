@@ -1090,8 +1090,10 @@ template<int D> struct Sim {
   // Integrator::selectDt (Integrator.cc:116-169)
   double selectDt(double dtMin, double dtMax, double mult) const {
     double dt = dtMax;
-    const double hd = hydroDt();
-    if (hd > 0.0 && hd < dt) dt = hd;
+    if (!cfg.gravity_only) {
+      const double hd = hydroDt();
+      if (hd > 0.0 && hd < dt) dt = hd;
+    }
     if (cfg.gravity) { const double gd = gravityDt(); if (gd > 0.0 && gd < dt) dt = gd; }
     dt *= mult;
     dt = std::min(dt, cfg.dt_growth*lastDt);
@@ -1151,6 +1153,7 @@ template<int D> struct Sim {
 
   // ---- CheapSynchronousRK2::step via Integrator::step ---------------------------
   double step(double maxTime) {
+    if (cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) return stepSynchronousGravity(maxTime);
     updateConnectivity();                 // Integrator::setGhostNodes (every cycle)
     applyGhostBoundaries();
     // preStepInitialize: RigorousSumDensity
@@ -1187,6 +1190,29 @@ template<int D> struct Sim {
     ++cycle; lastDt = dt;
     return dt;
   }
+  // ---- SynchronousRK2::step with TreeGravity as the only package (Integrator/SynchronousRK2.cc:
+  // 40-135): dt from the previous evaluation's vote, derivatives at the start (TreeGravity::initialize
+  // builds the tree, evaluateDerivatives walks it and sets DxDt = v), a half step, derivatives at the
+  // midpoint, then the full step from the start state with the midpoint derivatives.  The state
+  // TreeGravity registers is position and velocity (TreeGravity.cc:240-277); nothing else changes.
+  double stepSynchronousGravity(double maxTime) {
+    const double dt = selectDt(std::min(cfg.dt_min, maxTime - t), std::min(cfg.dt_max, maxTime - t), 1.0);
+    const double hdt = 0.5*dt, t0 = t;
+    gravityInitialize(); zeroDerivs(); gravityEvaluate();
+    for (int64_t i = 0; i < nInt; ++i) DxDt[i] = v[i];
+    const std::vector<V> x0(x.begin(), x.begin() + nInt), v0(v.begin(), v.begin() + nInt);
+    for (int64_t i = 0; i < nInt; ++i)
+      for (int a = 0; a < D; ++a) { x[i].c[a] += hdt*DxDt[i].c[a]; v[i].c[a] += hdt*DvDtG[i].c[a]; }
+    t = t0 + hdt;
+    gravityInitialize(); zeroDerivs(); gravityEvaluate();
+    for (int64_t i = 0; i < nInt; ++i) DxDt[i] = v[i];
+    for (int64_t i = 0; i < nInt; ++i)
+      for (int a = 0; a < D; ++a) { x[i].c[a] = x0[i].c[a] + dt*DxDt[i].c[a]; v[i].c[a] = v0[i].c[a] + dt*DvDtG[i].c[a]; }
+    t = t0 + dt;
+    ++cycle; lastDt = dt;
+    return dt;
+  }
+
   // State::update with the registered policies. fullStep=false: timeAdvanceOnly (updateAsIncrement).
   void advanceIncrements(double mult, bool fullStep) {
     const double rmin = cfg.rho_min, rmax = cfg.rho_max;
@@ -1236,6 +1262,9 @@ int spxo_max_threads(void) { return omp_get_max_threads(); }
 
 void* spxo_create(const spx_config* cfg, int64_t n) {
   if (!cfg || n < 1 || cfg->dim < 1 || cfg->dim > 3) { fail(SPX_ERR_ARG, "bad config"); return nullptr; }
+  if ((cfg->integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) != (cfg->gravity_only != 0) || (cfg->gravity_only && !cfg->gravity)) {
+    fail(SPX_ERR_UNSUPPORTED, "SynchronousRK2 runs TreeGravity-only problems"); return nullptr;
+  }
   auto* h = new Handle{cfg->dim, nullptr};
   switch (cfg->dim) {
     case 1: h->sim = new Sim<1>(*cfg, n); break;

@@ -23,6 +23,9 @@ SPX_KERNEL_NBSPLINE = 2
 SPX_Q_MONAGHAN_GINGOLD = 0
 SPX_Q_LIMITED_MONAGHAN_GINGOLD = 1
 
+SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2 = 0
+SPX_INTEGRATOR_SYNCHRONOUS_RK2 = 1
+
 SPX_BC_NONE = 0
 SPX_BC_PERIODIC = 1
 SPX_BC_REFLECTING = 2
@@ -81,6 +84,8 @@ class SpxConfig(C.Structure):
         ("viscosity", C.c_int32),
         ("eta_crit_frac", C.c_double),
         ("eta_fold_frac", C.c_double),
+        ("integrator", C.c_int32),
+        ("gravity_only", C.c_int32),
     ]
 
     def copy(self):
@@ -141,6 +146,8 @@ def default_config(dim=3, kernel=SPX_KERNEL_BSPLINE, **overrides):
     c.viscosity = SPX_Q_MONAGHAN_GINGOLD
     c.eta_crit_frac = 1.0
     c.eta_fold_frac = 0.2
+    c.integrator = SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2
+    c.gravity_only = 0
     for k, v in overrides.items():
         if k in ("bc_lo", "bc_hi", "xmin", "xmax"):
             arr = getattr(c, k)

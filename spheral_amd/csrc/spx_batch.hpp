@@ -219,6 +219,10 @@ class Batch {
     if (cfg.gravity && (cfg.G <= 0.0 || cfg.softening <= 0.0 || cfg.opening <= 0.0 || cfg.ftimestep <= 0.0))
       raise(SPX_ERR_ARG, "TreeGravity needs G, softening, opening, ftimestep > 0");
     if (cfg.gravity && maxRuns != 1) raise(SPX_ERR_UNSUPPORTED, "a TreeGravity batch holds one run");
+    if (cfg.integrator != SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2 && cfg.integrator != SPX_INTEGRATOR_SYNCHRONOUS_RK2)
+      raise(SPX_ERR_ARG, "unknown integrator");
+    if ((cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) != (cfg.gravity_only != 0) || (cfg.gravity_only && !cfg.gravity))
+      raise(SPX_ERR_UNSUPPORTED, "SynchronousRK2 runs TreeGravity-only problems (gravity_only = 1, gravity = 1)");
   }
 
   template <typename T>
@@ -1251,9 +1255,48 @@ class Batch {
     return itr;
   }
 
+  // Integrator::step with SynchronousRK2::step when TreeGravity is the only package
+  // (Integrator/SynchronousRK2.cc:40-135): dt from the previous evaluation's vote, the tree and its
+  // walk at the start and at the midpoint, the full step with the midpoint derivatives.  No
+  // neighbours are needed: nothing but TreeGravity reads the state.
+  void stepSynchronousGravity() {
+    if (timing) {
+      evCount = 0;
+      mark("start");
+    }
+    if (nActive == 0) return;
+    const int nb = (int)xcd_grid(nActive, kBlock);
+    hipLaunchKernelGGL(k_reset_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
+    hipLaunchKernelGGL(k_select_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns, ph, cfg.dt_min,
+                       cfg.dt_max, cfg.dt_growth, 1);
+    HIP_CHECK(hipGetLastError());
+    mark("select_dt");
+    gravityPass(false);
+    mark("gravity");
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_grav_mid<DD>, dim3(nb), dim3(kBlock), 0, stream, f, (const Scalars*)dsc, (int)nActive);
+    });
+    HIP_CHECK(hipGetLastError());
+    mark("update");
+    gravityPass(true);
+    mark("gravity");
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_grav_full<DD>, dim3(nb), dim3(kBlock), 0, stream, f, (const Scalars*)dsc, (int)nActive);
+    });
+    HIP_CHECK(hipGetLastError());
+    mark("update");
+    hipLaunchKernelGGL(k_advance_time, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
+    HIP_CHECK(hipGetLastError());
+    download(hsc, dsc, sizeof(Scalars)*maxRuns);
+    collectTimes();
+  }
+
   // One Integrator::step of every active run.
   void step() {
     if (!retire && !initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called before stepping");
+    if (cfg.gravity_only) { stepSynchronousGravity(); return; }
     if (timing) {
       evCount = 0;
       mark("start");

@@ -102,6 +102,8 @@ int spx_config_default(int32_t dim, spx_config* c) {
   c->viscosity = SPX_Q_MONAGHAN_GINGOLD;
   c->eta_crit_frac = 1.0;
   c->eta_fold_frac = 0.2;
+  c->integrator = SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2;
+  c->gravity_only = 0;
   return SPX_OK;
 }
 

@@ -807,6 +807,35 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
   if (MID) write_mid_record<D>(f, i, om);
 }
 
+// SynchronousRK2 with TreeGravity as the only package (Integrator/SynchronousRK2.cc:40-135): the
+// half step with the start-of-step derivatives (DxDt = v, DvDt = the tree acceleration), then the
+// full step from the start state with the midpoint ones (TreeGravity.cc:303-316 sets DxDt = v).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_grav_mid(Fields f, const Scalars* sc, int n) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hdt = sc[f.run[i]].hdt;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const double v = f.v[a][i];
+    f.xm[a][i] = f.x[a][i] + hdt*v;
+    f.vm[a][i] = v + hdt*f.DvDtG[a][i];
+  }
+}
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_grav_full(Fields f, const Scalars* sc, int n) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dt = sc[f.run[i]].dt;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const double vm = f.vm[a][i];
+    f.x[a][i] = f.x[a][i] + dt*vm;
+    f.v[a][i] = f.v[a][i] + dt*f.DvDtG[a][i];
+    f.DxDt[a][i] = vm;
+  }
+}
+
 // One pass of iterateIdealH (Utilities/iterateIdealH.cc:118-176) with the SPHSmoothingScale
 // package alone (SmoothingScale/SPHSmoothingScale.cc:154-245) on the connectivity state: the
 // zeroth moment of kernelValueSPH over the node's neighbours (images included) gives the ideal h;

@@ -106,6 +106,54 @@ def noh_planar_1d(nx1=100, nperh=1.35, order=5, gamma=5.0 / 3.0, rho1=1.0, eps1=
                      eps=np.full(nx1, eps1), H=H)
 
 
+def collisionless_sphere_3d(nr=10, r0_au=1.0, M0_earth=1.0, plummer_au=1.0e-2, opening=1.0, fdt=0.1):
+    """The reference's N-body collapse test as CollisionlessSphereCollapse.py sets it up
+    (tests/functional/Gravity/CollisionlessSphereCollapse.py:40-190) in MKS units: a
+    (2 nr)^3 lattice on [-r0, r0]^3 clipped to r <= r0 (GenerateNodeDistribution3d lattice,
+    NodeGenerators/GenerateNodeDistribution3d.py:573-624), masses renormalised to M0,
+    OctTreeGravity(G, plummer softening, opening, ftimestep, AccelerationRatio) as the only
+    package, SynchronousRK2 with lastDt = dtMin = 0.1 dt, dtMax = 100 dt, dtGrowth 2, where
+    dt = collapse time / 100.  Returns (cfg, state, collapse_time)."""
+    AU = 149597870700.0
+    Mearth = 5.9722e24
+    G = 6.67430e-11                      # PhysicalConstants::GMKS (Material/PhysicalConstants.cc:48)
+    r0 = r0_au * AU
+    M0 = M0_earth * Mearth
+    plummer = plummer_au * AU
+    rho0 = M0 / (4.0 / 3.0 * math.pi * r0 * r0 * r0)
+    tdyn = math.sqrt(3.0 * math.pi / (16 * G * rho0))
+    collapse = tdyn / math.sqrt(2.0)
+    dt = collapse / 100
+    n = 2 * nr
+    dx = (r0 - (-r0)) / n
+    xs, ms = [], []
+    for ig in range(n * n * n):
+        i, j, k = ig % n, (ig // n) % n, ig // (n * n)
+        xx, yy, zz = -r0 + (i + 0.5) * dx, -r0 + (j + 0.5) * dx, -r0 + (k + 0.5) * dx
+        if (xx - 0.0) ** 2 + (yy - 0.0) ** 2 + (zz - 0.0) ** 2 <= r0 ** 2:
+            xs.append((xx, yy, zz))
+            ms.append(dx * dx * dx * rho0)
+    msum = 0.0
+    for mi in ms:
+        msum += mi
+    fmass = M0 / msum
+    m = np.array(ms) * fmass
+    x = np.array(xs)
+    N = len(m)
+    cfg = abi.default_config(3, kernel=abi.SPX_KERNEL_BSPLINE, nperh=2.01)
+    cfg.gravity = 1
+    cfg.gravity_only = 1
+    cfg.integrator = abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2
+    cfg.G = G
+    cfg.softening = plummer
+    cfg.opening = opening
+    cfg.ftimestep = fdt
+    cfg.dt_min, cfg.dt_max, cfg.dt_growth, cfg.last_dt = 0.1 * dt, 100.0 * dt, 2.0, 0.1 * dt
+    st = dict(x=x, v=np.zeros((N, 3)), m=m, rho=np.full(N, rho0), eps=np.zeros(N),
+              H=_iso_H(N, 3, 1.0 / (2.01 * dx)))
+    return cfg, st, collapse
+
+
 def kelvin_helmholtz_2d(n_target=4096, seed=7, nperh=2.0, gamma=5.0 / 3.0, P0=2.5,
                         kernel=abi.SPX_KERNEL_BSPLINE, precision=abi.SPX_PREC_MIXED):
     """2-D periodic KH: lattice of spacing sqrt(1/N) with the band |y-0.5|<0.25 at twice the
