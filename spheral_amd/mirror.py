@@ -333,6 +333,7 @@ class TreeGravity(_Package):
 
     def __init__(self, G, softeningLength, opening=0.5, ftimestep=0.1, timeStepChoice=None):
         super().__init__()
+        self.timeStepChoice = timeStepChoice
         self.G = float(G)
         self.softeningLength = float(softeningLength)
         self.opening = float(opening)
@@ -344,6 +345,7 @@ class TreeGravity(_Package):
 
 class CheapSynchronousRK2Integrator:
     """Integrator/CheapSynchronousRK2 driving the packages on one engine run."""
+    code = abi.SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2
 
     def __init__(self, dataBase):
         self.dataBase = dataBase
@@ -379,8 +381,13 @@ class CheapSynchronousRK2Integrator:
         hydro = [p for p in self.packages if isinstance(p, SPHPackage)]
         grav = [p for p in self.packages if isinstance(p, TreeGravity)]
         other = [p for p in self.packages if not isinstance(p, (SPHPackage, TreeGravity))]
+        if not hydro and len(grav) == 1 and not other:
+            return self._nbody_config(grav[0])
         if len(hydro) != 1 or other or len(grav) > 1:
-            raise NotImplementedError("the GPU path runs exactly one SPH package plus an optional TreeGravity")
+            raise NotImplementedError("the GPU path runs one SPH package plus an optional TreeGravity, or "
+                                      "TreeGravity alone under SynchronousRK2")
+        if self.code != abi.SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2:
+            raise NotImplementedError("SynchronousRK2 is implemented for TreeGravity-only problems")
         h = hydro[0]
         n0 = db.nodeLists[0]
         for nl in db.nodeLists[1:]:
@@ -442,6 +449,24 @@ class CheapSynchronousRK2Integrator:
             cfg.G, cfg.softening, cfg.opening, cfg.ftimestep = g.G, g.softeningLength, g.opening, g.ftimestep
         return cfg
 
+    def _nbody_config(self, g):
+        """TreeGravity as the only package (an N-body script): SynchronousRK2, no SPH."""
+        if self.code != abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2:
+            raise NotImplementedError("TreeGravity alone is implemented under SynchronousRK2Integrator")
+        if g.timeStepChoice not in (None, "AccelerationRatio"):
+            raise NotImplementedError("only the AccelerationRatio time step choice is implemented")
+        db = self.dataBase
+        n0 = db.nodeLists[0]
+        cfg = abi.default_config(db.nDim, nperh=n0.nPerh)
+        cfg.neighbor_extent = n0.kernelExtent
+        cfg.hmin, cfg.hmax = n0.hmin, n0.hmax
+        cfg.rho_min, cfg.rho_max = n0.rhoMin, n0.rhoMax
+        cfg.gamma = n0.eos.gamma
+        cfg.gravity, cfg.gravity_only, cfg.integrator = 1, 1, abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2
+        cfg.G, cfg.softening, cfg.opening, cfg.ftimestep = g.G, g.softeningLength, g.opening, g.ftimestep
+        cfg.dt_min, cfg.dt_max, cfg.dt_growth, cfg.last_dt = self.dtMin, self.dtMax, self.dtGrowth, self.lastDt
+        return cfg
+
     def _gather_state(self):
         nls = self.dataBase.nodeLists
         self._slices = []
@@ -483,6 +508,16 @@ class CheapSynchronousRK2Integrator:
     def conservation(self):
         """Mass, momentum (3), kinetic, thermal and gravitational energy of the state."""
         return self._run.conservation()
+
+
+class SynchronousRK2Integrator(CheapSynchronousRK2Integrator):
+    """Integrator/SynchronousRK2: derivatives at the start and at the midpoint of each step; the
+    engine runs it for TreeGravity-only problems."""
+    code = abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2
+
+
+AccelerationRatio = "AccelerationRatio"
+DynamicalTime = "DynamicalTime"
 
 
 class SpheralController:
@@ -552,13 +587,15 @@ def namespace(dim):
         "GammaLawGasCGS": GammaLawGasCGS, "makeFluidNodeList": mk_nl, "DataBase": mk_db,
         "MonaghanGingoldViscosity": MonaghanGingoldViscosity, "SPH": SPH, "TreeGravity": TreeGravity,
         "CheapSynchronousRK2Integrator": CheapSynchronousRK2Integrator, "SpheralController": SpheralController,
+        "SynchronousRK2Integrator": SynchronousRK2Integrator, "AccelerationRatio": AccelerationRatio,
+        "DynamicalTime": DynamicalTime,
         "RigorousSumDensity": RigorousSumDensity, "IntegrateDensity": IntegrateDensity, "SumDensity": SumDensity,
         "IdealH": IdealH, "IntegrateH": IntegrateH,
     }
     for base in ("Vector", "Plane", "PeriodicBoundary", "ReflectingBoundary", "BSplineKernel", "WendlandC4Kernel",
                  "NBSplineKernel", "LimitedMonaghanGingoldViscosity",
                  "TableKernel", "makeFluidNodeList", "DataBase", "MonaghanGingoldViscosity", "TreeGravity",
-                 "CheapSynchronousRK2Integrator", "SPH"):
+                 "CheapSynchronousRK2Integrator", "SynchronousRK2Integrator", "SPH"):
         ns[base + sfx] = ns[base]
     if dim == 3:
         ns["OctTreeGravity"] = TreeGravity

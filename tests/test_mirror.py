@@ -164,3 +164,30 @@ def test_unsupported_q_options_raise():
     hydro.Q.balsaraShearCorrection = True
     with pytest.raises(NotImplementedError):
         integ.config()
+
+
+def test_nbody_script_maps_to_gravity_only_config():
+    """CollisionlessSphereCollapse.py's package list (OctTreeGravity alone, SynchronousRK2Integrator)
+    maps onto the gravity-only configuration of problems.collisionless_sphere_3d."""
+    cfg_ref, st, _ = problems.collisionless_sphere_3d()
+    eos = S3.GammaLawGasMKS(5.0/3.0, 1.0)
+    nodes = S3.makeFluidNodeList("nodes", eos, numInternal=len(st["m"]))
+    for attr, key in (("positions", "x"), ("velocity", "v"), ("mass", "m"), ("massDensity", "rho"),
+                      ("specificThermalEnergy", "eps"), ("Hfield", "H")):
+        getattr(nodes, attr)()[...] = st[key]
+    db = S3.DataBase()
+    db.appendNodeList(nodes)
+    gravity = S3.OctTreeGravity(G=cfg_ref.G, softeningLength=cfg_ref.softening, opening=1.0, ftimestep=0.1,
+                                timeStepChoice=S3.AccelerationRatio)
+    integ = S3.SynchronousRK2Integrator(db)
+    integ.appendPhysicsPackage(gravity)
+    integ.lastDt, integ.dtMin, integ.dtMax, integ.dtGrowth = (cfg_ref.last_dt, cfg_ref.dt_min, cfg_ref.dt_max,
+                                                              cfg_ref.dt_growth)
+    cfg = integ.config()
+    for name in ("gravity", "gravity_only", "integrator", "G", "softening", "opening", "ftimestep", "dt_min",
+                 "dt_max", "dt_growth", "last_dt", "dim"):
+        assert getattr(cfg, name) == getattr(cfg_ref, name), name
+    cheap = S3.CheapSynchronousRK2Integrator(db)
+    cheap.appendPhysicsPackage(gravity)
+    with pytest.raises(NotImplementedError):
+        cheap.config()
