@@ -259,7 +259,9 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         dvdx[c*D + d] -= mv*(double)e.gs[d];
-        Mc[c*D + d] -= mr*(double)e.gs[d];
+        // with isotropic H the gradient is parallel to r, so M = sum m r (x) gradW is symmetric:
+        // the upper triangle is accumulated and mirrored after the loop
+        if (d >= c) Mc[c*D + d] -= mr*(double)e.gs[d];
       }
     }
     depsdt += mo*(double)(S.prho*vdg + e.workQs);
@@ -279,6 +281,10 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
     }
   }
 
+#pragma unroll
+  for (int c = 1; c < D; ++c)
+#pragma unroll
+    for (int d = 0; d < c; ++d) Mc[c*D + d] = Mc[d*D + c];
   // ---- per-node finalize (SPH.cc:480-551), double precision
   const double hdet = hdet_iso<D>(Sh);
   double det;
