@@ -1055,10 +1055,10 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (mid)
-        launchPrecTable(k_omega<DD, double, true>, k_omega<DD, float, true>, nb, f, geo, dtab, (int)nActive,
+        launchPrecTable(k_omega<DD, double, true, kGroupOmega>, k_omega<DD, float, true, kGroupOmega>, nb, f, geo, dtab, (int)nActive,
                         (const double*)f.xhm, f.omegam, runMask);
       else
-        launchPrecTable(k_omega<DD, double, false>, k_omega<DD, float, false>, nb, f, geo, dtab, (int)nActive,
+        launchPrecTable(k_omega<DD, double, false, kGroupOmega>, k_omega<DD, float, false, kGroupOmega>, nb, f, geo, dtab, (int)nActive,
                         (const double*)f.xh, f.omega, runMask);
     });
   }
