@@ -175,6 +175,9 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
 // pair loop makes three lookups per pair at scattered intervals).  sc != nullptr (the step):
 // the node's half-step velocity record for the compatible-energy update is written too (as
 // k_v12 would, with the run's dt/2).
+template <typename R>
+constexpr bool kSymM = sizeof(R) == sizeof(double);
+
 template <int D, bool FULL, typename R, bool LQ = false>
 __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable,
                                                    const Scalars* sc) {
@@ -260,8 +263,9 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
       for (int d = 0; d < D; ++d) {
         dvdx[c*D + d] -= mv*(double)e.gs[d];
         // with isotropic H the gradient is parallel to r, so M = sum m r (x) gradW is symmetric:
-        // the upper triangle is accumulated and mirrored after the loop
-        if (d >= c) Mc[c*D + d] -= mr*(double)e.gs[d];
+        // in FP64 the upper triangle is accumulated and mirrored after the loop (the FP32 pair
+        // terms round r_c g_d and r_d g_c differently, so mixed precision keeps all nine)
+        if (!kSymM<R> || d >= c) Mc[c*D + d] -= mr*(double)e.gs[d];
       }
     }
     depsdt += mo*(double)(S.prho*vdg + e.workQs);
@@ -281,10 +285,12 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
     }
   }
 
+  if (kSymM<R>) {
 #pragma unroll
-  for (int c = 1; c < D; ++c)
+    for (int c = 1; c < D; ++c)
 #pragma unroll
-    for (int d = 0; d < c; ++d) Mc[c*D + d] = Mc[d*D + c];
+      for (int d = 0; d < c; ++d) Mc[c*D + d] = Mc[d*D + c];
+  }
   // ---- per-node finalize (SPH.cc:480-551), double precision
   const double hdet = hdet_iso<D>(Sh);
   double det;
