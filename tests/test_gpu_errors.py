@@ -1,0 +1,40 @@
+"""Error paths of the entry points added for the Limited Q, NBSpline, iterateIdealH and the
+TreeGravity-only SynchronousRK2: each misuse returns its SPX_ERR_* code (no device work)."""
+import pytest
+
+from spheral_amd import abi, problems
+from spheral_amd.engine import SpxError
+
+pytestmark = pytest.mark.gpu
+
+
+UNSUP, ARG = -4, -1
+
+
+@pytest.mark.parametrize("change,code", [
+    (dict(integrator=abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2), UNSUP),               # needs gravity_only
+    (dict(gravity_only=1), UNSUP),                                              # needs gravity + SynchronousRK2
+    (dict(gravity_only=1, gravity=1), UNSUP),                                   # needs SynchronousRK2
+    (dict(kernel=abi.SPX_KERNEL_NBSPLINE, kernel_order=11), ARG),
+    (dict(viscosity=7), ARG),
+    (dict(viscosity=abi.SPX_Q_LIMITED_MONAGHAN_GINGOLD, eta_fold_frac=0.0), ARG),
+    (dict(integrator=5), ARG),
+])
+def test_bad_configurations_are_rejected(gpu, change, code):
+    cfg = abi.default_config(3)
+    for k, v in change.items():
+        setattr(cfg, k, v)
+    print(f"[case] {change}", flush=True)
+    with pytest.raises(SpxError) as err:
+        gpu.Run(cfg, 8, 0)
+    assert err.value.code == code
+
+
+def test_iterate_ideal_h_needs_initialize(gpu):
+    cfg, st = problems.noh_planar_1d()
+    g = gpu.make_run(cfg, st)
+    with pytest.raises(SpxError) as err:
+        g.iterate_ideal_h()
+    assert err.value.code == -3
+    g.initialize()
+    assert g.iterate_ideal_h(50, 1e-4) > 1
