@@ -1325,10 +1325,11 @@ class Batch {
     mark("omega");
     // derivatives at the midpoint
     if (cfg.gravity) { gravityPass(true); mark("gravity"); }
-    // the Limited Q's postStateUpdate copy: its gradient becomes the previous evaluation's DvDx,
-    // which this evaluation then overwrites
-    if (limitedQ()) std::swap(f.DvDx, f.DvDxQ);
+    // the Limited Q reads its gradient (the previous evaluation's DvDx, ArtificialViscosityHandle::
+    // postStateUpdate) while this evaluation writes DvDx; the end-of-step postStateUpdate then
+    // copies the new DvDx into it, as the reference's state holds it after the step
     sphPass(false, true, true);
+    copyQVelocityGradient();
     mark("sph");
     // full step from the start-of-step state, compatible energy first
     launchD([&](auto d) {

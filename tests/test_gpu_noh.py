@@ -154,3 +154,28 @@ def test_limited_q_steps_track_oracle(gpu, oracle_mod, name, cfg, st, gradh):
         a, b = g.field(f), o.field(f)
         sc = max(np.abs(b).max(), cs if f == "velocity" else 0.0)
         assert np.all(np.abs(a - b) <= 1e-9 * sc), f
+
+
+def test_limited_q_gradient_after_steps_is_the_references(gpu, oracle_mod):
+    """After a step the Q holds that step's DvDx (the end-of-step postStateUpdate copy): an
+    evaluation through the package calls right after stepping limits with it, on both sides."""
+    name, cfg, st = LQ[1]
+    print(f"[case] {name} limited Q: steps then evaluate", flush=True)
+    cfg = cfg.copy()
+    cfg.viscosity = abi.SPX_Q_LIMITED_MONAGHAN_GINGOLD
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    o = oracle_mod.OracleRun(cfg, len(st["m"]))
+    o.set_state(**st)
+    o.initialize()
+    for _ in range(3):
+        g.step()
+        o.step()
+    g.update_connectivity()
+    o.update_connectivity()
+    g.evaluate_derivatives()
+    o.evaluate_derivatives()
+    S = o.field("abs delta velocity hydro")
+    assert np.all(np.abs(g.field("delta velocity hydro") - o.field("delta velocity hydro")) <= 1e-9 * S + 1e-300)
+    a, b = g.field("max viscous pressure"), o.field("max viscous pressure")
+    assert np.all(np.abs(a - b) <= 1e-9 * np.maximum(np.abs(b), np.abs(b).max() * 1e-3 + 1e-300))
