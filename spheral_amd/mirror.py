@@ -195,6 +195,75 @@ class FluidNodeList:
         return self._H
 
 
+class GenerateNodeDistribution1d:
+    """NodeGenerators/GenerateNodeDistribution1d.py: n equally spaced nodes on [xmin, xmax],
+    x_i = xmin + (i + 1/2) dx, m = rho(x) dx, H = 1/(nNodePerh dx)."""
+
+    def __init__(self, n, rho, xmin, xmax, nNodePerh=2.01, offset=0.0, rejecter=None):
+        if rejecter is not None or offset != 0.0:
+            raise NotImplementedError("node rejecters and offsets are not implemented")
+        assert xmax > xmin and n > 0
+        rhof = (lambda x: float(rho)) if isinstance(rho, float) else rho
+        dx = (xmax - xmin)/n
+        h0 = nNodePerh*dx
+        self.x = [xmin + (i + 0.5)*dx for i in range(n)]
+        self.m = [rhof(xi)*dx for xi in self.x]
+        self.rho = [rhof(xi) for xi in self.x]
+        self.H = [np.array([[1.0/h0]]) for _ in range(n)]
+        self.dim = 1
+
+    def positions(self):
+        return np.array(self.x).reshape(-1, 1)
+
+
+class GenerateNodeDistribution3d:
+    """NodeGenerators/GenerateNodeDistribution3d.py, distributionType "lattice"
+    (latticeDistribution, lines 573-624): cell centres of an n1 x n2 x n3 lattice on [xmin, xmax],
+    kept when rmin^2 <= |x - origin|^2 <= rmax^2; m = dx dy dz rho(x); H = diag(1/(nNodePerh d))."""
+
+    def __init__(self, n1, n2, n3, rho, distributionType="lattice", xmin=(0.0, 0.0, 0.0), xmax=(1.0, 1.0, 1.0),
+                 rmin=None, rmax=None, origin=(0.0, 0.0, 0.0), nNodePerh=2.01, **kwargs):
+        if distributionType != "lattice" or kwargs:
+            raise NotImplementedError("GenerateNodeDistribution3d: only the lattice distribution is implemented")
+        rhof = (lambda x: float(rho)) if isinstance(rho, float) else rho
+        dx, dy, dz = ((xmax[a] - xmin[a])/n for a, n in zip(range(3), (n1, n2, n3)))
+        H0 = np.diag([1.0/(nNodePerh*dx), 1.0/(nNodePerh*dy), 1.0/(nNodePerh*dz)])
+        self.x, self.m, self.H, self.rho = [], [], [], []
+        for ig in range(n1*n2*n3):
+            i, j, k = ig % n1, (ig // n1) % n2, ig // (n1*n2)
+            xx, yy, zz = xmin[0] + (i + 0.5)*dx, xmin[1] + (j + 0.5)*dy, xmin[2] + (k + 0.5)*dz
+            r2 = (xx - origin[0])**2 + (yy - origin[1])**2 + (zz - origin[2])**2
+            if (rmin is None or r2 >= rmin**2) and (rmax is None or r2 <= rmax**2):
+                self.x.append((xx, yy, zz))
+                self.m.append(dx*dy*dz*rhof((xx, yy, zz)))
+                self.H.append(H0)
+                self.rho.append(rhof((xx, yy, zz)))
+        self.dim = 3
+
+    def positions(self):
+        return np.array(self.x).reshape(-1, 3)
+
+
+def _distribute(*pairs):
+    """SortAndDivideRedistributeNodes / PeanoHilbertDistributeNodes on one process: each NodeList
+    takes its generator's nodes (positions, masses, mass density, H) in generator order."""
+    for nodes, gen in pairs:
+        n = len(gen.m)
+        nodes.resize(n)
+        nodes._x[...] = gen.positions()
+        nodes._m[...] = np.asarray(gen.m)
+        nodes._rho[...] = np.asarray(gen.rho)
+        nodes._H[...] = np.stack(gen.H) if n else nodes._H
+
+
+def distributeNodes1d(*pairs):
+    _distribute(*pairs)
+
+
+def distributeNodes3d(*pairs, **kwargs):
+    _distribute(*pairs)
+
+
 def makeFluidNodeList(name, eos, numInternal=0, numGhost=0, hmin=1.0e-20, hmax=1.0e20, hminratio=0.1, nPerh=2.01,
                       maxNumNeighbors=500, rhoMin=1.0e-10, rhoMax=1e10, NeighborType=None, searchType=GatherScatter,
                       kernelExtent=2.0, xmin=None, xmax=None, dim=3):
@@ -591,6 +660,9 @@ def namespace(dim):
         "DynamicalTime": DynamicalTime,
         "RigorousSumDensity": RigorousSumDensity, "IntegrateDensity": IntegrateDensity, "SumDensity": SumDensity,
         "IdealH": IdealH, "IntegrateH": IntegrateH,
+        "GenerateNodeDistribution1d": GenerateNodeDistribution1d,
+        "GenerateNodeDistribution3d": GenerateNodeDistribution3d,
+        "distributeNodes1d": distributeNodes1d, "distributeNodes3d": distributeNodes3d,
     }
     for base in ("Vector", "Plane", "PeriodicBoundary", "ReflectingBoundary", "BSplineKernel", "WendlandC4Kernel",
                  "NBSplineKernel", "LimitedMonaghanGingoldViscosity",

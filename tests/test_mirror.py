@@ -191,3 +191,25 @@ def test_nbody_script_maps_to_gravity_only_config():
     cheap.appendPhysicsPackage(gravity)
     with pytest.raises(NotImplementedError):
         cheap.config()
+
+
+def test_node_generators_match_the_problem_setups():
+    """GenerateNodeDistribution1d as Noh-planar-1d.py calls it and the 3-D lattice clipped to a
+    sphere as CollisionlessSphereCollapse.py calls it give the problems' own node sets."""
+    _, st = problems.noh_planar_1d()
+    nodes = S1.makeFluidNodeList("nodes1", S1.GammaLawGasMKS(5.0/3.0, 1.0), nPerh=1.35)
+    gen = S1.GenerateNodeDistribution1d(n=100, rho=1.0, xmin=0.0, xmax=1.0, nNodePerh=1.35)
+    S1.distributeNodes1d((nodes, gen))
+    assert np.array_equal(nodes.positions(), st["x"]) and np.array_equal(nodes.mass(), st["m"])
+    assert np.array_equal(nodes.Hfield(), st["H"]) and np.array_equal(nodes.massDensity(), st["rho"])
+    cfg, st3, _ = problems.collisionless_sphere_3d()
+    r0 = 149597870700.0
+    rho0 = 5.9722e24/(4.0/3.0*np.pi*r0*r0*r0)
+    nodes3 = S3.makeFluidNodeList("nodes", S3.GammaLawGasMKS(5.0/3.0, 1.0))
+    gen3 = S3.GenerateNodeDistribution3d(20, 20, 20, rho0, distributionType="lattice", xmin=(-r0, -r0, -r0),
+                                         xmax=(r0, r0, r0), rmin=0.0, rmax=r0)
+    S3.distributeNodes3d((nodes3, gen3))
+    assert np.array_equal(nodes3.positions(), st3["x"])
+    m = nodes3.mass()
+    fmass = 5.9722e24/sum(m.tolist())        # the script's renormalisation (a sequential sum)
+    assert np.array_equal(m*fmass, st3["m"])
