@@ -58,8 +58,9 @@ extern "C" {
 
 /* Time integrator.  CheapSynchronousRK2 (Integrator/CheapSynchronousRK2.cc:40-132): one
    derivative evaluation per step, at the midpoint.  SynchronousRK2 (Integrator/SynchronousRK2.cc:
-   40-135): evaluations at the start and at the midpoint of every step; the engine runs it for
-   TreeGravity-only problems (gravity_only = 1). */
+   40-135): evaluations at the start and at the midpoint of every step, the trial half step with
+   the full policies; with the SPH package (plus an optional TreeGravity) or with TreeGravity as
+   the only package (gravity_only = 1). */
 #define SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2  0
 #define SPX_INTEGRATOR_SYNCHRONOUS_RK2        1
 
@@ -116,7 +117,8 @@ typedef struct spx_config {
   int32_t viscosity;                 /* SPX_Q_*                                      */
   double  eta_crit_frac, eta_fold_frac; /* LimitedMonaghanGingoldViscosity ctor     */
   int32_t integrator;                /* SPX_INTEGRATOR_*                             */
-  int32_t gravity_only;              /* 1: TreeGravity is the only package (no SPH)  */
+  int32_t gravity_only;              /* 1: TreeGravity is the only package (no SPH;  */
+                                     /*    needs gravity = 1 and SynchronousRK2)     */
 } spx_config;
 
 typedef struct spx_run spx_run;

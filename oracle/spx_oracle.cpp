@@ -30,7 +30,8 @@
  *   dt               Physics/GenericHydro.cc:112-381, Integrator/Integrator.cc:116-169
  *   compat. energy   Hydro/SpecificThermalEnergyPolicy.cc:50-175
  *   EOS              Material/GammaLawGas.cc:185-245, Material/EquationOfStateInline.hh:98-106
- *   integrator       Integrator/CheapSynchronousRK2.cc:40-132, Integrator/Integrator.cc:67-111,
+ *   integrator       Integrator/CheapSynchronousRK2.cc:40-132, Integrator/SynchronousRK2.cc:40-135,
+ *                    Integrator/Integrator.cc:67-111,
  *                    DataBase/State.cc:223-325, IncrementStateInline.hh, IncrementBoundedStateInline.hh,
  *                    ReplaceBoundedStateInline.hh
  *   TreeGravity      Gravity/TreeGravity.cc:30-90,143-328,362-477,483-497,818-950,
@@ -47,6 +48,8 @@
  * the integrator end to end; NBSpline against the quintic and cubic B-splines'
  * closed forms; the Limited Q vanishing in a linear flow.  iterateIdealH is pinned
  * by its fixed-point properties only (the Noh golden is insensitive to it).
+ * SynchronousRK2 with SPH: second order in time on a smooth wave and energy kept
+ * to round-off (the reference's tests hold no golden for it).
  * Parity unpinned (no reference build or vectors exist): the bit-level table
  * coefficients (the reference inverts with Eigen 3x3), the CubicHermite gradient
  * knots (Eigen SparseLU; here a tridiagonal solve), OpenMP summation order.
@@ -1151,9 +1154,53 @@ template<int D> struct Sim {
     initialized = true;
   }
 
+  // ---- SynchronousRK2::step with the SPH package (Integrator/SynchronousRK2.cc:40-135): the dt vote
+  // from the previous (midpoint) derivatives, derivatives at the start of the step, a half step with
+  // the full policies (compatible energy with multiplier dt/2, H replaced by the ideal H -- the
+  // state is not advanced time-only here), the midpoint postStateUpdate and derivatives, then the
+  // full step from the start state with the midpoint derivatives.
+  double stepSynchronousHydro(double maxTime) {
+    updateConnectivity();                 // Integrator::setGhostNodes (every cycle)
+    applyGhostBoundaries();
+    sumDensity();                         // preStepInitialize: RigorousSumDensity
+    applyGhostBoundaries();
+    const double dt = selectDt(std::min(cfg.dt_min, maxTime - t), std::min(cfg.dt_max, maxTime - t), 1.0);
+    const double hdt = 0.5*dt, t0 = t;
+    if (cfg.gravity) gravityInitialize();
+    evaluateDerivatives();
+    const std::vector<V> x0(x.begin(), x.begin() + nInt), v0(v.begin(), v.begin() + nInt);
+    const std::vector<double> rho0(rho.begin(), rho.begin() + nInt), eps0(eps.begin(), eps.begin() + nInt);
+    const std::vector<T> H0(H.begin(), H.begin() + nInt);
+    {
+      std::vector<V> vstart(v.begin(), v.end());
+      std::vector<double> DepsDt0(DepsDt.begin(), DepsDt.end());
+      compatibleEnergyUpdate(hdt, vstart, DepsDt0);
+      advanceIncrements(hdt, /*fullStep=*/true);
+    }
+    t = t0 + hdt;
+    applyGhostBoundaries();
+    postStateUpdate();
+    if (cfg.gravity) gravityInitialize();
+    evaluateDerivatives();
+    for (int64_t i = 0; i < nInt; ++i) { x[i] = x0[i]; v[i] = v0[i]; rho[i] = rho0[i]; eps[i] = eps0[i]; H[i] = H0[i]; }
+    applyGhostBoundaries();
+    std::vector<V> vstart(v.begin(), v.end());
+    std::vector<double> DepsDtMid(DepsDt.begin(), DepsDt.end());
+    compatibleEnergyUpdate(dt, vstart, DepsDtMid);
+    advanceIncrements(dt, /*fullStep=*/true);
+    t = t0 + dt;
+    applyGhostBoundaries();
+    postStateUpdate();
+    enforceBoundaries();
+    applyGhostBoundaries();
+    ++cycle; lastDt = dt;
+    return dt;
+  }
+
   // ---- CheapSynchronousRK2::step via Integrator::step ---------------------------
   double step(double maxTime) {
-    if (cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) return stepSynchronousGravity(maxTime);
+    if (cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2)
+      return cfg.gravity_only ? stepSynchronousGravity(maxTime) : stepSynchronousHydro(maxTime);
     updateConnectivity();                 // Integrator::setGhostNodes (every cycle)
     applyGhostBoundaries();
     // preStepInitialize: RigorousSumDensity
@@ -1262,8 +1309,8 @@ int spxo_max_threads(void) { return omp_get_max_threads(); }
 
 void* spxo_create(const spx_config* cfg, int64_t n) {
   if (!cfg || n < 1 || cfg->dim < 1 || cfg->dim > 3) { fail(SPX_ERR_ARG, "bad config"); return nullptr; }
-  if ((cfg->integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) != (cfg->gravity_only != 0) || (cfg->gravity_only && !cfg->gravity)) {
-    fail(SPX_ERR_UNSUPPORTED, "SynchronousRK2 runs TreeGravity-only problems"); return nullptr;
+  if ((cfg->gravity_only && (!cfg->gravity || cfg->integrator != SPX_INTEGRATOR_SYNCHRONOUS_RK2))) {
+    fail(SPX_ERR_UNSUPPORTED, "TreeGravity-only problems run under SynchronousRK2"); return nullptr;
   }
   auto* h = new Handle{cfg->dim, nullptr};
   switch (cfg->dim) {

@@ -221,8 +221,8 @@ class Batch {
     if (cfg.gravity && maxRuns != 1) raise(SPX_ERR_UNSUPPORTED, "a TreeGravity batch holds one run");
     if (cfg.integrator != SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2 && cfg.integrator != SPX_INTEGRATOR_SYNCHRONOUS_RK2)
       raise(SPX_ERR_ARG, "unknown integrator");
-    if ((cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) != (cfg.gravity_only != 0) || (cfg.gravity_only && !cfg.gravity))
-      raise(SPX_ERR_UNSUPPORTED, "SynchronousRK2 runs TreeGravity-only problems (gravity_only = 1, gravity = 1)");
+    if (cfg.gravity_only && (!cfg.gravity || cfg.integrator != SPX_INTEGRATOR_SYNCHRONOUS_RK2))
+      raise(SPX_ERR_UNSUPPORTED, "TreeGravity-only problems (gravity_only = 1) need gravity = 1 and SynchronousRK2");
   }
 
   template <typename T>
@@ -1297,6 +1297,7 @@ class Batch {
   void step() {
     if (!retire && !initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called before stepping");
     if (cfg.gravity_only) { stepSynchronousGravity(); return; }
+    if (cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) { stepSynchronousHydro(); return; }
     if (timing) {
       evCount = 0;
       mark("start");
@@ -1331,7 +1332,62 @@ class Batch {
     sphPass(false, true, true);
     copyQVelocityGradient();
     mark("sph");
-    // full step from the start-of-step state, compatible energy first
+    fullStep(nb);
+  }
+
+  // Integrator::step with SynchronousRK2::step and the SPH package (Integrator/SynchronousRK2.cc:
+  // 40-135): the connectivity pass (sum density, the hydro dt vote) and the dt selection as in
+  // step(); derivatives at the start of the step, whose half-step velocity records carry
+  // multiplier dt/2; the trial half step with the full policies (k_half_update) into the midpoint
+  // arrays; the midpoint grad-h Omega and the Q's gradient copy (postStateUpdate); derivatives at
+  // the midpoint; then the full step from the start-of-step state as in step().
+  void stepSynchronousHydro() {
+    if (timing) {
+      evCount = 0;
+      mark("start");
+    }
+    const bool fresh = prepareNewRuns();
+    buildConnectivity(true);
+    if (nActive == 0) return;
+    mark("connectivity");
+    if (fresh) finishNewRuns();
+    const int nb = (int)xcd_grid(nActive, kBlock);
+    hipLaunchKernelGGL(k_select_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns, ph, cfg.dt_min,
+                       cfg.dt_max, cfg.dt_growth, cfg.gravity);
+    HIP_CHECK(hipGetLastError());
+    mark("select_dt");
+    // derivatives at the start of the step (boundaries applied: the Q sees its gradient's images)
+    copyStateToMid();
+    if (cfg.gravity) { gravityPass(false); mark("gravity"); }
+    sphPass(false);
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)dsc, 0.25,
+                         (int)nActive);
+    });
+    HIP_CHECK(hipGetLastError());
+    copyQVelocityGradient();
+    mark("sph");
+    // trial half step, then the midpoint postStateUpdate (grad-h Omega and the records)
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      launchPrec(k_half_update<DD, double, kGroupUpdate>, k_half_update<DD, float, kGroupUpdate>, nb, f, geo, ph,
+                 (const Scalars*)dsc, (int)nActive);
+    });
+    mark("update");
+    omegaPass(true);
+    mark("omega");
+    // derivatives at the midpoint
+    if (cfg.gravity) { gravityPass(true); mark("gravity"); }
+    sphPass(false, true, true);
+    copyQVelocityGradient();
+    mark("sph");
+    fullStep(nb);
+  }
+
+  // The full step from the start-of-step state with the midpoint derivatives (compatible energy
+  // first), the end-of-step grad-h Omega and the time advance; shared by both integrators.
+  void fullStep(int nb) {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       launchPrec(k_full_update<DD, false, double, kGroupUpdate>, k_full_update<DD, false, float, kGroupUpdate>, nb, f, geo,
@@ -1414,7 +1470,7 @@ class Batch {
     requireConn();
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)dsc,
+      hipLaunchKernelGGL(k_v12<DD>, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const Scalars*)nullptr,
                          0.5*multiplier, (int)nActive);
       launchPrec(k_full_update<DD, true, double, kGroupUpdate>, k_full_update<DD, true, float, kGroupUpdate>,
                  (int)xcd_grid(nActive, kBlock), f, geo, ph, dsc, multiplier, (int)nActive);

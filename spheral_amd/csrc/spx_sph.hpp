@@ -373,24 +373,13 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 }
 
 // ---------------------------------------------------------------------------
-// Full step from the start-of-step state with the midpoint derivatives
-// (CheapSynchronousRK2.cc:106-109): SpecificThermalEnergyPolicy::update
-// (Hydro/SpecificThermalEnergyPolicy.cc:79-144) on the pair accelerations the SPH
-// pass stored per ELL entry (each already in canonical summation order), then the
-// increment and replace policies and the EOS.  v0 is read from f.v, the new velocity
-// goes to f.v1.  The step variant also measures, per run, how far nodes have moved and h
-// grown since the candidate list was built (see k_candidates).  ENERGY_ONLY: the
-// package-level policy call (eps only).
+// SpecificThermalEnergyPolicy::update's partition of the pair work (Hydro/
+// SpecificThermalEnergyPolicy.cc:79-144): node i's specific thermal energy rate from the pair
+// accelerations the SPH pass stored per ELL entry (each already in canonical summation order)
+// and the half-step velocity records (Fields::pkv).
 // ---------------------------------------------------------------------------
-template <int D, bool ENERGY_ONLY, typename R, int G>
-__global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, Scalars* sc, double multiplier,
-                                                        int n) {
-  const int i = xcd_block()*blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  int skr = 0;                       // candidate-list check (k_skin_check's measure, fused)
-  double skd = 0.0, skg = 0.0;
-  if (valid) {
-  const double dt = ENERGY_ONLY ? multiplier : sc[f.run[i]].dt;
+template <int D, typename R, int G>
+__device__ __forceinline__ double compatible_rate(const Fields& f, const Geo& g, int i) {
   const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
   const R* pdd = (const R*)f.pdd;       // pair terms in the pair arithmetic; the partition is FP64
   const double ms = f.m[i];
@@ -443,6 +432,27 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
       du += aFirst ? w*duab : (1.0 - w)*duab*ma/mb;
     }
   }
+  return du;
+}
+
+// ---------------------------------------------------------------------------
+// Full step from the start-of-step state with the midpoint derivatives
+// (CheapSynchronousRK2.cc:106-109, SynchronousRK2.cc:115-118): the compatible energy
+// partition (compatible_rate), then the increment and replace policies and the EOS.  v0 is
+// read from f.v, the new velocity goes to f.v1.  The step variant also measures, per run, how far nodes have moved and h
+// grown since the candidate list was built (see k_candidates).  ENERGY_ONLY: the
+// package-level policy call (eps only).
+// ---------------------------------------------------------------------------
+template <int D, bool ENERGY_ONLY, typename R, int G>
+__global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph, Scalars* sc, double multiplier,
+                                                        int n) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  int skr = 0;                       // candidate-list check (k_skin_check's measure, fused)
+  double skd = 0.0, skg = 0.0;
+  if (valid) {
+  const double dt = ENERGY_ONLY ? multiplier : sc[f.run[i]].dt;
+  const double du = compatible_rate<D, R, G>(f, g, i);
   const double eps1 = f.eps[i] + du*dt;
   f.eps[i] = eps1;
   if (ENERGY_ONLY) return;
@@ -489,14 +499,47 @@ __global__ void __launch_bounds__(kBlock) k_full_update(Fields f, Geo g, Phys ph
   }
 }
 
-// Half-step velocity of the compatible-energy partition, v^{1/2} = v + a dt/2 with the
-// midpoint hydro acceleration (SpecificThermalEnergyPolicy.cc:97-99), packed with DepsDt and
-// m (Fields::pkv).  hdt < 0: the run's dt/2.
+// SynchronousRK2's trial half step (SynchronousRK2.cc:67-68: state.update with multiplier dt/2):
+// the full policies, not timeAdvanceOnly -- the compatible energy partition of the start-of-step
+// pair accelerations, the increments, H replaced by the ideal H (ReplaceBoundedState) and the
+// EOS -- written to the midpoint arrays and position record; the start-of-step state stays.
+template <int D, typename R, int G>
+__global__ void __launch_bounds__(kBlock) k_half_update(Fields f, Geo g, Phys ph, const Scalars* sc, int n) {
+  const int i = xcd_block()*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hdt = sc[f.run[i]].hdt;
+  const double du = compatible_rate<D, R, G>(f, g, i);
+  const double eps = f.eps[i] + du*hdt;
+  double xm[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    xm[a] = f.x[a][i] + hdt*f.DxDt[a][i];
+    f.xm[a][i] = xm[a];
+    double va = f.v[a][i] + hdt*f.DvDt[a][i];
+    if (ph.gravity) va = va + hdt*f.DvDtG[a][i];
+    f.vm[a][i] = va;
+  }
+  const double rho = fmin(ph.rho_max, fmax(ph.rho_min, f.rho[i] + hdt*f.DrhoDt[i]));
+  const double h = fmin(ph.hminInv, fmax(ph.hmaxInv, f.Hideal[i]));
+  f.rhom[i] = rho;
+  f.epsm[i] = eps;
+  f.hinvm[i] = h;
+  f.Pm[i] = ph.g1*rho*eps;
+  f.csm[i] = sqrt(fmax(0.0, ph.gamma*ph.g1*eps));
+  double2* P = reinterpret_cast<double2*>(f.xhm) + 2*(size_t)i;
+  P[0] = make_double2(xm[0], xm[1]);
+  P[1] = make_double2(xm[2], h);
+}
+
+// Half-step velocity of the compatible-energy partition, v^{1/2} = v + a*multiplier/2 with the
+// evaluation's hydro acceleration (SpecificThermalEnergyPolicy.cc:97-99), packed with DepsDt and
+// m (Fields::pkv).  sc == nullptr: hdt is the fixed multiplier/2; otherwise hdt is the fraction
+// of each run's dt (1/4 for SynchronousRK2's trial half step, whose multiplier is dt/2).
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_v12(Fields f, const Scalars* sc, double hdtFixed, int n) {
+__global__ void __launch_bounds__(kBlock) k_v12(Fields f, const Scalars* sc, double hdtScale, int n) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double hdt = hdtFixed < 0.0 ? 0.5*sc[f.run[i]].dt : hdtFixed;
+  const double hdt = sc ? hdtScale*sc[f.run[i]].dt : hdtScale;
   double w[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < D; ++a) w[a] = f.v[a][i] + f.DvDt[a][i]*hdt;

@@ -455,8 +455,6 @@ class CheapSynchronousRK2Integrator:
         if len(hydro) != 1 or other or len(grav) > 1:
             raise NotImplementedError("the GPU path runs one SPH package plus an optional TreeGravity, or "
                                       "TreeGravity alone under SynchronousRK2")
-        if self.code != abi.SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2:
-            raise NotImplementedError("SynchronousRK2 is implemented for TreeGravity-only problems")
         h = hydro[0]
         n0 = db.nodeLists[0]
         for nl in db.nodeLists[1:]:
@@ -484,6 +482,7 @@ class CheapSynchronousRK2Integrator:
         cfg.gradh_correction = int(h.opts["gradhCorrection"])
         cfg.correct_velocity_gradient = int(h.opts["correctVelocityGradient"])
         cfg.dt_min, cfg.dt_max, cfg.dt_growth, cfg.last_dt = self.dtMin, self.dtMax, self.dtGrowth, self.lastDt
+        cfg.integrator = self.code
         bcs = []
         for p in self.packages:
             for bc in p.boundaries:
@@ -580,8 +579,8 @@ class CheapSynchronousRK2Integrator:
 
 
 class SynchronousRK2Integrator(CheapSynchronousRK2Integrator):
-    """Integrator/SynchronousRK2: derivatives at the start and at the midpoint of each step; the
-    engine runs it for TreeGravity-only problems."""
+    """Integrator/SynchronousRK2: derivatives at the start and at the midpoint of each step, the
+    trial half step with the full policies (SynchronousRK2.cc:40-135)."""
     code = abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2
 
 

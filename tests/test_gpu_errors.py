@@ -12,7 +12,7 @@ UNSUP, ARG = -4, -1
 
 
 @pytest.mark.parametrize("change,code", [
-    (dict(integrator=abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2), UNSUP),               # needs gravity_only
+    (dict(integrator=abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2, gravity_only=1), UNSUP),  # needs gravity
     (dict(gravity_only=1), UNSUP),                                              # needs gravity + SynchronousRK2
     (dict(gravity_only=1, gravity=1), UNSUP),                                   # needs SynchronousRK2
     (dict(kernel=abi.SPX_KERNEL_NBSPLINE, kernel_order=11), ARG),

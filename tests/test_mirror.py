@@ -57,6 +57,21 @@ def test_script_config_matches_problem_config():
             assert a == b, name
 
 
+def test_synchronous_rk2_script_selects_the_integrator():
+    """The same Sod script under SynchronousRK2Integrator (as Sod-spherical-1d.py selects it) maps
+    onto the problem's configuration with the SynchronousRK2 step and nothing else changed."""
+    cfg_ref, _, cheap, _ = sod_script()
+    integ = S1.SynchronousRK2Integrator(cheap.dataBase)
+    for p in cheap.physicsPackages():
+        integ.appendPhysicsPackage(p)
+    integ.lastDt, integ.dtMin = cheap.lastDt, cheap.dtMin
+    cfg = integ.config()
+    cfg_ref.integrator = abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2
+    for name in FIELDS:
+        a, b = getattr(cfg, name), getattr(cfg_ref, name)
+        assert (list(a) == list(b)) if hasattr(a, "__len__") else a == b, name
+
+
 def test_nodelists_concatenate_in_order():
     _, st, integ, lists = sod_script()
     g = integ._gather_state()
