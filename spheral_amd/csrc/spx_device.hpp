@@ -31,6 +31,8 @@ __host__ __device__ __forceinline__ size_t ell_at(int k, int i, size_t ld) {
 // (k0 + u >= nn) become the lane's own slot with no image, and are skipped by the caller.
 constexpr int kGroup = 4;
 // The compatible-energy update does little arithmetic per neighbour; it keeps 8 in flight.
+// FP64 Taylor-Green 128^3 per step: 8 in flight (184 VGPRs, 2 waves/SIMD) 0.96 ms; 4 in flight
+// (102 VGPRs, 4 waves) 1.16 ms; 8 capped at 3 waves (167 VGPRs, no spills) 1.04 ms.
 constexpr int kGroupUpdate = 8;
 // The grad-h passes read 32 bytes per neighbour and evaluate one kernel lookup: 8 in flight
 // (88 VGPRs, 5 waves/SIMD) measured 0.92 -> 0.906 ms FP64 per step on Taylor-Green 128^3.
