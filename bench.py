@@ -4,7 +4,8 @@
 One "step" is one full CheapSynchronousRK2 step of the whole box (neighbour
 build, sum density + dt, mid-step update, grad-h, SPH + smoothing-scale
 derivatives, compatible-energy full update, grad-h), with the state resident in
-HBM.  Prints ONE JSON line (the contract documented in DESIGN.md).
+HBM.  Prints ONE JSON line: metric, value, unit, ms_per_step, config, the dominant kernel's
+roofline (HIP-event timing, committed rocprofv3 traffic) and the CPU baseline.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 3] [--n1d 128] [--precision fp64|mixed]
 """
@@ -30,6 +31,11 @@ BYTES_NODE_3D = 520           # compulsory HBM bytes per particle-step, FP64 3-D
 PEAK_FP64 = 78.6e12           # MI355X FP64 vector/matrix peak, FLOP/s
 PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
+
+
+def sph_kernel(prec):
+    """rocprofv3's name of the step's SPH pair pass on this workload (Monaghan-Gingold Q)."""
+    return f"k_sph<3, false, {'double' if prec == abi.SPX_PREC_FP64 else 'float'}, false>"
 
 
 def measured_traffic(kernel):
@@ -416,7 +422,7 @@ def main():
     line["step_roofline"]["frac"] = value/line["step_roofline"]["ceiling_particle_steps_per_s"]
     # attainable HBM rate on this box: plain streaming copy kernel, 1 GiB read + 1 GiB write per launch
     line["hbm_copy_gbps"] = engine.bandwidth_probe(1 << 30, 10)
-    line["roofline"]["traffic"] = measured_traffic(f"k_sph<3, false, {'double' if prec == abi.SPX_PREC_FP64 else 'float'}>")
+    line["roofline"]["traffic"] = measured_traffic(sph_kernel(prec))
     if not args.no_cpu:
         print("[bench] timing the CPU baseline (oracle) on a bounded sample", file=sys.stderr, flush=True)
         cb = cpu_baseline(args.cpu_n1d, args.cpu_steps, prec)
