@@ -28,7 +28,7 @@ exactly as pairs within one.
 """
 import numpy as np
 
-from . import abi, engine
+from . import abi, connectivity, engine
 
 # Enumerations used as keyword values in the reference's scripts.
 RigorousSumDensity = "RigorousSumDensity"
@@ -441,6 +441,15 @@ class CheapSynchronousRK2Integrator:
         return package in self.packages
 
     # -- engine binding ------------------------------------------------------------
+    def _boundaries(self):
+        """Integrator::uniqueBoundaryConditions: each package's boundaries in append order."""
+        bcs = []
+        for p in self.packages:
+            for bc in p.boundaries:
+                if bc not in bcs:
+                    bcs.append(bc)
+        return bcs
+
     def config(self):
         """spx_config of the problem described by the DataBase and the packages."""
         db = self.dataBase
@@ -483,11 +492,7 @@ class CheapSynchronousRK2Integrator:
         cfg.correct_velocity_gradient = int(h.opts["correctVelocityGradient"])
         cfg.dt_min, cfg.dt_max, cfg.dt_growth, cfg.last_dt = self.dtMin, self.dtMax, self.dtGrowth, self.lastDt
         cfg.integrator = self.code
-        bcs = []
-        for p in self.packages:
-            for bc in p.boundaries:
-                if bc not in bcs:
-                    bcs.append(bc)
+        bcs = self._boundaries()
         for bc in bcs:
             if isinstance(bc, PeriodicBoundary):
                 (a1, s1), (a2, s2) = (pl.axis_side(dim) for pl in bc.planes)
@@ -576,6 +581,28 @@ class CheapSynchronousRK2Integrator:
     def conservation(self):
         """Mass, momentum (3), kinetic, thermal and gravitational energy of the state."""
         return self._run.conservation()
+
+    def connectivityMap(self):
+        """DataBase::connectivityMap of the current state in the reference's numbering: the
+        connectivity is rebuilt from the current positions (as the next Integrator::setGhostNodes
+        does) and its ghosts numbered as the appended boundaries create and cull them
+        (spheral_amd.connectivity).  connectivityForNode(nodeList, i) takes the NodeList."""
+        if self._run is None:
+            self.initializeProblemStartup()
+        dim = self.dataBase.nDim
+        seq = []
+        for bc in self._boundaries():
+            if isinstance(bc, PeriodicBoundary):
+                a, side = bc.planes[0].axis_side(dim)
+                seq.append(("periodic", a, 1 if side == 0 else 2))
+            elif isinstance(bc, ReflectingBoundary):
+                a, side = bc.plane.axis_side(dim)
+                seq.append(("reflecting", a, 1 if side == 0 else 2))
+        r = self._run
+        r.update_connectivity()
+        off, nbr, img = r.neighbors()
+        nls = [(nl.name, a, b - a) for nl, a, b in self._slices]
+        return connectivity.connectivity_map(r.cfg, r.field("position"), r.field("H"), off, nbr, img, nls, seq)
 
 
 class SynchronousRK2Integrator(CheapSynchronousRK2Integrator):

@@ -1,0 +1,146 @@
+"""The reference-numbered ConnectivityMap export (spheral_amd.connectivity) on the oracle's
+neighbour sets, which equal the engine's bit for bit.
+
+Pins: a 1-D reflecting lattice worked by hand (which wall images exist, which are culled, their
+numbers and node 0's list); on periodic 2-D and 3-D boxes, every list recomputed by brute force
+from the kept ghosts' positions (the neighbour criterion of the oracle's connectivity: each
+node's support, gather or scatter), every kept ghost's position the image of its control, its
+parent kept and numbered before it, and the cull leaving no ghost that nothing uses."""
+import io
+
+import numpy as np
+
+from spheral_amd import abi, connectivity, problems
+
+
+def _dump(oracle_mod, cfg, st, nodelists=None, boundaries=None):
+    o = oracle_mod.OracleRun(cfg, len(st["m"]))
+    o.set_state(**st)
+    o.initialize()
+    off, nbr, img = o.neighbors()
+    return o, connectivity.connectivity_map(cfg, o.field("position"), o.field("H"), off, nbr, img, nodelists,
+                                            boundaries)
+
+
+def test_reflecting_lattice_by_hand(oracle_mod):
+    # 10 nodes at 0.05 ... 0.95 between walls at 0 and 1, h = 0.09, support 2h = 0.18
+    n = 10
+    x = (np.arange(n) + 0.5)/n
+    cfg = abi.default_config(1, kernel=abi.SPX_KERNEL_BSPLINE)
+    cfg.neighbor_extent = 2.0
+    cfg.bc_lo[0] = cfg.bc_hi[0] = abi.SPX_BC_REFLECTING
+    cfg.xmin[0], cfg.xmax[0] = 0.0, 1.0
+    st = dict(x=x[:, None], v=np.zeros((n, 1)), m=np.full(n, 0.1), rho=np.ones(n), eps=np.ones(n),
+              H=np.full((n, 1, 1), 1.0/0.09))
+    _, cm = _dump(oracle_mod, cfg, st)
+    # created: low wall images of nodes 0, 1 (x <= 0.18), high wall images of 8, 9; the images of
+    # 1 (at -0.15) and 8 (at 1.15) lie 0.2 from the nearest node and are culled
+    g = cm.ghosts[0]
+    assert list(g["control"]) == [0, 9]
+    assert list(g["code"]) == [1, 2]
+    assert np.allclose(g["position"][:, 0], [-0.05, 1.05])
+    assert [list(v) for v in cm.connectivityForNode(0, 0)] == [[1, 10]]
+    assert [list(v) for v in cm.connectivityForNode(0, 9)] == [[8, 11]]
+    assert [list(v) for v in cm.connectivityForNode(0, 4)] == [[3, 5]]
+
+
+def _brute(cfg, x, H, cm, k=0):
+    """Neighbour lists from the node + kept-ghost positions: j is a neighbour of i when
+    min(|H_i r|, |H_j r|) < extent (either support reaches)."""
+    g = cm.ghosts[k]
+    xs = np.concatenate([x, g["position"]])
+    hs = np.concatenate([1.0/H[:, 0, 0], 1.0/H[g["control"], 0, 0]])
+    out = []
+    for i in range(len(x)):
+        r = np.linalg.norm(xs - x[i], axis=1)
+        ok = (np.minimum(r/hs[i], r/hs) < cfg.neighbor_extent)
+        ok[i] = False
+        out.append(np.nonzero(ok)[0])
+    return out
+
+
+def _check_invariants(cfg, cm, x, H, boundaries=None):
+    g = cm.ghosts[0]
+    n = cm.sizes[0]
+    for a in range(cfg.dim):
+        c = (g["code"] >> (2*a)) & 3
+        for side in (1, 2):
+            m = c == side
+            if m.any():
+                exp = connectivity._image_coord(cfg, a, side, x[g["control"][m], a])
+                assert np.array_equal(g["position"][m, a], exp)
+        assert np.array_equal(g["position"][c == 0, a], x[g["control"][c == 0], a])
+    par = g["parent"]
+    gi = np.arange(len(par)) + n
+    assert np.all(par < gi)                                   # parents numbered first
+    gpar = par[par >= n] - n
+    assert np.all(g["control"][gpar] == g["control"][par >= n])
+    assert np.all(np.diff(g["created"]) > 0)                  # culling keeps the creation order
+    # kept = the ghosts internal nodes use + every ghost that controls a ghost before culling
+    pre_parent = connectivity.ghost_nodes(cfg, x, H, boundaries)[0]
+    used = set()
+    for i in range(n):
+        used.update(int(g["created"][j - n]) for j in cm.connectivityForNode(0, i)[0] if j >= n)
+    used.update(int(p) - n for p in pre_parent[pre_parent >= n])
+    assert used == set(int(c) for c in g["created"])
+
+
+def test_periodic_2d_lists_equal_brute_force(oracle_mod):
+    cfg, st = problems.uniform_lattice(2, 12, kernel=abi.SPX_KERNEL_BSPLINE, nperh=1.51)
+    rng = np.random.default_rng(5)
+    st["x"] = (st["x"] + rng.uniform(-0.02, 0.02, st["x"].shape)) % 1.0
+    o, cm = _dump(oracle_mod, cfg, st)
+    x, H = o.field("position"), o.field("H")
+    ref = _brute(cfg, x, H, cm)
+    for i in range(len(x)):
+        assert list(cm.connectivityForNode(0, i)[0]) == list(ref[i]), i
+    _check_invariants(cfg, cm, x, H)
+    assert np.any(cm.ghosts[0]["code"] == (1 | (1 << 2)))    # corner images exist
+
+
+def test_boundary_order_changes_numbers_not_sets(oracle_mod):
+    """Appending the y boundary first, and x's periodic boundary with its high plane first,
+    renumbers the ghosts (creation order) but every list still names the same images."""
+    cfg, st = problems.uniform_lattice(2, 10, kernel=abi.SPX_KERNEL_BSPLINE, nperh=1.51)
+    seq = [("periodic", 1, 1), ("periodic", 0, 2)]
+    o, cm = _dump(oracle_mod, cfg, st, boundaries=seq)
+    _, cm0 = _dump(oracle_mod, cfg, st)
+    x, H = o.field("position"), o.field("H")
+    _check_invariants(cfg, cm, x, H, seq)
+    n = len(x)
+    c0 = cm0.ghosts[0]["code"]
+    assert (c0[0] >> 0) & 3 == 1 and (cm.ghosts[0]["code"][0] >> 2) & 3 == 1   # x-low first vs y-low first
+    for i in range(n):
+        a = [(int(j), 0) if j < n else (int(cm.ghosts[0]["control"][j - n]), int(cm.ghosts[0]["code"][j - n]))
+             for j in cm.connectivityForNode(0, i)[0]]
+        b = [(int(j), 0) if j < n else (int(cm0.ghosts[0]["control"][j - n]), int(cm0.ghosts[0]["code"][j - n]))
+             for j in cm0.connectivityForNode("nodes", i)[0]]
+        assert sorted(a) == sorted(b), i
+    assert list(cm.ghosts[0]["code"]) != list(c0)
+
+
+def test_periodic_3d_invariants(oracle_mod):
+    cfg, st = problems.taylor_green_3d(8)
+    o, cm = _dump(oracle_mod, cfg, st)
+    x, H = o.field("position"), o.field("H")
+    _check_invariants(cfg, cm, x, H)
+    ref = _brute(cfg, x, H, cm)
+    for i in range(0, len(x), 37):
+        assert list(cm.connectivityForNode(0, i)[0]) == list(ref[i]), i
+
+
+def test_two_nodelists_and_writer(oracle_mod):
+    cfg, st = problems.sod_1d(100)
+    nl = int(np.sum(st["x"][:, 0] < 0.0))
+    _, cm = _dump(oracle_mod, cfg, st, nodelists=[("nodes1", 0, nl), ("nodes2", nl, 100 - nl)])
+    # the interface node of nodes1 sees nodes2's first nodes by their local indices
+    i = nl - 1
+    own, other = cm.connectivityForNode(0, i)
+    assert list(other[:2]) == [0, 1] and list(own[-1:]) == [nl - 2]
+    # each NodeList numbers its own wall ghosts after its own internal nodes
+    assert len(cm.ghosts[0]["control"]) > 0 and len(cm.ghosts[1]["control"]) > 0
+    f = io.StringIO()
+    cm.write(f)
+    lines = f.getvalue().splitlines()
+    assert lines[0] == f"nodes1/numInternalNodes {nl}"
+    assert any(line.startswith(f"nodes1/{i}/nodes2 ") for line in lines)
