@@ -62,13 +62,14 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(n1d, steps, precision):
+def cpu_baseline(n1d, steps, precision, integrator=abi.SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2):
     """The oracle (FP64 C++ restatement, OpenMP) on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = min(os.cpu_count() or 1, int(os.environ.get("SPX_CPU_THREADS", "16")))
     O.set_threads(threads)
     cfg, st = problems.taylor_green_3d(n1d)
+    cfg.integrator = integrator
     n = len(st["m"])
     r = O.OracleRun(cfg, n)
     r.set_state(**st)
@@ -320,6 +321,8 @@ def main():
     ap.add_argument("--cpu-n1d", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--integrator", choices=["cheap", "synchronous"], default="cheap",
+                    help="taylor_green: CheapSynchronousRK2 (the headline) or SynchronousRK2")
     ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard", "serve", "accuracy"],
                     default="taylor_green")
     ap.add_argument("--requests", type=int, default=100, help="serve: number of requests in the trace")
@@ -349,6 +352,9 @@ def main():
         return
     prec = abi.SPX_PREC_FP64 if args.precision == "fp64" else abi.SPX_PREC_MIXED
     cfg, st = problems.taylor_green_3d(args.n1d, precision=prec)
+    sync = args.integrator == "synchronous"
+    if sync:
+        cfg.integrator = abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2
     n = len(st["m"])
     print(f"[bench] building Taylor-Green {args.n1d}^3 = {n} particles", file=sys.stderr, flush=True)
     run = engine.make_run(cfg, st)
@@ -373,7 +379,9 @@ def main():
     ppn = npairs/n
     cons = run.conservation()
     # dominant kernel: the SPH derivative pass
-    sph_ms = per_class.get("sph", float("nan"))/args.steps
+    # SynchronousRK2 evaluates the derivatives twice per step (its "sph" class also holds the
+    # start-of-step half-step velocity records, a small copy kernel)
+    sph_ms = per_class.get("sph", float("nan"))/(args.steps*(2 if sync else 1))
     sph_flops = FLOP_PAIR["sph"]*npairs + FLOP_PAIR["smoothing"]*npairs
     achieved = sph_flops/(sph_ms*1e-3)
     peak = PEAK_FP64 if prec == abi.SPX_PREC_FP64 else PEAK_FP32
@@ -390,7 +398,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == abi.SPX_PREC_FP64 else "f32 pair / f64 accumulate",
         "data": "synthetic",
-        "config": {"workload": f"taylor_green_3d_{args.n1d}^3_wendlandC4_nperh2_cheapRK2", "particles": n,
+        "config": {"workload": f"taylor_green_3d_{args.n1d}^3_wendlandC4_nperh2_{'synchronousRK2' if sync else 'cheapRK2'}",
+                   "particles": n,
                    "unique_pairs": npairs, "pairs_per_particle": ppn, "mean_neighbors": 2.0*ppn,
                    "kernel": "WendlandC4 TableKernel(1000)", "boundary": "periodic",
                    "t_end": t, "cycles": cycle},
@@ -425,7 +434,7 @@ def main():
     line["roofline"]["traffic"] = measured_traffic(sph_kernel(prec))
     if not args.no_cpu:
         print("[bench] timing the CPU baseline (oracle) on a bounded sample", file=sys.stderr, flush=True)
-        cb = cpu_baseline(args.cpu_n1d, args.cpu_steps, prec)
+        cb = cpu_baseline(args.cpu_n1d, args.cpu_steps, prec, cfg.integrator)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         line["speedup_vs_cpu"] = value/cb["value"]
     else:
