@@ -335,8 +335,8 @@ def main():
     ap.add_argument("--no-sequential", action="store_true", help="ensemble: skip the sequential FP64 leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), run with --gpus 1"}))
+        print(json.dumps({"error": "this engine runs on one GPU: sharding a run across GPUs is not "
+                                   "built; run with --gpus 1"}))
         sys.exit(2)
     if args.workload == "ensemble":
         ensemble(args)
