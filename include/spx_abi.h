@@ -157,6 +157,20 @@ int spx_run_set_state(spx_run* run, const double* x, const double* v,
                       const double* m, const double* rho, const double* eps,
                       const double* H);
 
+/* The integrator's State handed to a package hook (Physics::evaluateDerivatives / dt /
+   preStepInitialize / postStateUpdate take `const State&`, Physics/Physics.hh:49-73; the fields
+   SPHBase::registerState enrolls, SPH/SPHBase.cc:203-262): replaces position, velocity, mass,
+   mass density, specific thermal energy and H of an initialized run in place (same layouts and
+   node order as spx_run_set_state) and re-evaluates pressure and sound speed (the EOS policies
+   State::update fires, Hydro/PressurePolicy.cc, Hydro/SoundSpeedPolicy.cc).  Keeps the
+   derivatives, grad-h Omega, the pair accelerations of the last evaluation and the connectivity:
+   inside a step the reference evaluates the moved state on the connectivity of the step's start
+   (Integrator/CheapSynchronousRK2.cc:56-88).  A driver that keeps its State on the host calls
+   this at the top of each hook that reads State, so every hook sees the caller's current state.
+   Boundary enforcement of positions stays with the caller (Integrator::enforceBoundaries). */
+int spx_run_update_state(spx_run* run, const double* x, const double* v, const double* m,
+                         const double* rho, const double* eps, const double* H);
+
 /* Copy one field to host in the caller's original node order.  Names follow
    HydroFieldNames (Hydro/HydroFieldNames.cc:11-40): "position", "velocity",
    "mass", "mass density", "specific thermal energy", "H", "pressure",
@@ -297,7 +311,8 @@ int spx_batch_get_field(spx_batch* batch, int32_t run, const char* name, double*
                         int64_t capacity, int64_t* count);
 /* As spx_run_conservation for one run. */
 int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7);
-/* Occupancy for admission control: particles resident and run slots in use. */
+/* Occupancy for admission control: particles and run slots held by active and finished runs
+   (removed runs' slots and particles count as free: the next admission reclaims them). */
 int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs);
 /* As spx_run_neighbor_builds for one run of a batch. */
 int spx_batch_neighbor_builds(spx_batch* batch, int32_t run, int64_t* builds);

@@ -474,6 +474,16 @@ template<int D> struct Sim {
   // gravity
   std::vector<std::unordered_map<uint64_t, Cell<D>>> tree; V gxmin, gxmax; double boxLength = 0;
   double dtMinAcc = 0.0, extraEnergy = 0.0;
+  // The potential as the integrator's State holds it.  TreeGravity enrolls its potential FieldList
+  // in the State (TreeGravity.cc:137), so the state0 copy an integrator takes before its midpoint
+  // evaluation carries the potential of that moment, and state.assign(state0) puts it back after the
+  // midpoint evaluation (CheapSynchronousRK2.cc:66-68,107; SynchronousRK2.cc:68-69,101): after a
+  // step the "gravitational potential" field holds the start-of-step evaluation's values under
+  // SynchronousRK2, and under CheapSynchronousRK2 whatever it held before the step (the start-up
+  // dry run's values, or a standalone evaluation's).  extraEnergy is a member, not state: it keeps
+  // the last evaluation's value.
+  std::vector<double> potentialState;
+  void stickPotential() { potentialState.assign(potential.begin(), potential.begin() + nInt); }
 
   explicit Sim(const spx_config& c, int64_t n): cfg(c), nInt(n), nTot(n) {
     W.build(cfg.kernel, D, cfg.kernel_points, cfg.min_nperh, cfg.max_nperh, cfg.kernel_order);
@@ -1149,7 +1159,7 @@ template<int D> struct Sim {
     updateEOS();
     applyGhostBoundaries();
     gradhOmega();
-    if (cfg.gravity) { gravityInitialize(); gravityEvaluate(); }
+    if (cfg.gravity) { gravityInitialize(); gravityEvaluate(); stickPotential(); }
     applyGhostBoundaries();
     initialized = true;
   }
@@ -1168,6 +1178,7 @@ template<int D> struct Sim {
     const double hdt = 0.5*dt, t0 = t;
     if (cfg.gravity) gravityInitialize();
     evaluateDerivatives();
+    if (cfg.gravity) stickPotential();       // the state0 copy holds the start-of-step potential
     const std::vector<V> x0(x.begin(), x.begin() + nInt), v0(v.begin(), v.begin() + nInt);
     const std::vector<double> rho0(rho.begin(), rho.begin() + nInt), eps0(eps.begin(), eps.begin() + nInt);
     const std::vector<T> H0(H.begin(), H.begin() + nInt);
@@ -1246,6 +1257,7 @@ template<int D> struct Sim {
     const double dt = selectDt(std::min(cfg.dt_min, maxTime - t), std::min(cfg.dt_max, maxTime - t), 1.0);
     const double hdt = 0.5*dt, t0 = t;
     gravityInitialize(); zeroDerivs(); gravityEvaluate();
+    stickPotential();                        // the state0 copy holds the start-of-step potential
     for (int64_t i = 0; i < nInt; ++i) DxDt[i] = v[i];
     const std::vector<V> x0(x.begin(), x.begin() + nInt), v0(v.begin(), v.begin() + nInt);
     for (int64_t i = 0; i < nInt; ++i)
@@ -1349,7 +1361,11 @@ int spxo_update_connectivity(void* p) { auto* h = static_cast<Handle*>(p); DISPA
 int spxo_sum_density(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->sumDensity(); s->applyGhostBoundaries(); }); return 0; }
 int spxo_update_eos(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->updateEOS(); s->applyGhostBoundaries(); }); return 0; }
 int spxo_post_state_update(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->applyGhostBoundaries(); s->postStateUpdate(); }); return 0; }
-int spxo_evaluate_derivatives(void* p) { auto* h = static_cast<Handle*>(p); DISPATCH(h, { s->applyGhostBoundaries(); s->evaluateDerivatives(); }); return 0; }
+int spxo_evaluate_derivatives(void* p) {
+  auto* h = static_cast<Handle*>(p);
+  DISPATCH(h, { s->applyGhostBoundaries(); s->evaluateDerivatives(); if (s->cfg.gravity) s->stickPotential(); });
+  return 0;
+}
 int spxo_dt(void* p, double* hydro, double* grav) {
   auto* h = static_cast<Handle*>(p);
   DISPATCH(h, { *hydro = s->hydroDt(); *grav = s->cfg.gravity ? s->gravityDt() : 0.0; });
@@ -1407,7 +1423,7 @@ int64_t spxo_get_field(void* p, const char* name, double* out) {
     else if (n == "mass zeroth moment") sc(s->m0); else if (n == "mass first moment") ve(s->m1);
     else if (n == "M SPH correction") te(s->Mc); else if (n == "mass density gradient") ve(s->gradRho);
     else if (n == "normalization") sc(s->normalization); else if (n == "new mass density") sc(s->rhoSum);
-    else if (n == "gravitational potential") sc(s->potential);
+    else if (n == "gravitational potential") { std::vector<double> ps(s->potentialState); ps.resize(N, 0.0); sc(ps); }
     else if (n == "abs delta velocity hydro") ve(s->absDvDt); else if (n == "abs delta specific thermal energy") sc(s->absDepsDt);
   });
   if (cnt < 0) fail(SPX_ERR_ARG, "unknown field " + n);

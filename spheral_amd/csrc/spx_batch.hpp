@@ -128,6 +128,7 @@ class Batch {
   uint32_t *vals = nullptr, *valsAlt = nullptr;
   void* sortTmp = nullptr;
   size_t sortTmpBytes = 0;
+  double* stage = nullptr;    // caller-order staging of one state component (updateState)
   int* cellStart = nullptr;
   int* cellEnd = nullptr;
   int64_t cellsAlloc = 0;
@@ -313,7 +314,7 @@ class Batch {
   std::vector<double**> carried(Fields& F) {
     std::vector<double**> v;
     for (int a = 0; a < D; ++a) { v.push_back(&F.x[a]); v.push_back(&F.v[a]); }
-    for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.pot})
+    for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.pot, &F.potS})
       v.push_back(p);
     for (int a = 0; a < D; ++a) { v.push_back(&F.DxDt[a]); v.push_back(&F.DvDt[a]); v.push_back(&F.DvDtG[a]); }
     for (int a = 0; a < D; ++a) v.push_back(&F.xb[a]);
@@ -330,7 +331,7 @@ class Batch {
         all.push_back(p);
     for (double** p : {&F.m, &F.rho, &F.eps, &F.hinv, &F.P, &F.cs, &F.omega, &F.rhom, &F.epsm, &F.hinvm, &F.Pm,
                        &F.csm, &F.omegam, &F.prhom, &F.DepsDt, &F.DrhoDt, &F.DHDt, &F.maxQ, &F.Hideal, &F.m0,
-                       &F.rhoSum, &F.norm, &F.effQ, &F.ns, &F.pot, &F.hb})
+                       &F.rhoSum, &F.norm, &F.effQ, &F.ns, &F.pot, &F.potS, &F.hb})
       all.push_back(p);
     const size_t total = (all.size() + 2*(size_t)D*D)*(size_t)ld;   // + DvDx, Mcorr
     double* slab = dalloc<double>(total);
@@ -535,7 +536,7 @@ class Batch {
     for (int64_t i = 0; i < n; ++i) buf[i] = 1.0;
     up(f.omega, buf.data());
     // derivatives start at zero (SPHBase::initializeProblemStartup)
-    for (double* p : {f.DepsDt, f.DrhoDt, f.DHDt, f.maxQ, f.pot, f.P, f.cs}) zero(p);
+    for (double* p : {f.DepsDt, f.DrhoDt, f.DHDt, f.maxQ, f.pot, f.potS, f.P, f.cs}) zero(p);
     for (int c = 0; c < D*D; ++c) HIP_CHECK(hipMemsetAsync(f.DvDx + (size_t)c*ld + o, 0, n*sizeof(double), stream));
     if (limitedQ())
       for (int c = 0; c < D*D; ++c) HIP_CHECK(hipMemsetAsync(f.DvDxQ + (size_t)c*ld + o, 0, n*sizeof(double), stream));
@@ -1122,12 +1123,17 @@ class Batch {
     });
     HIP_CHECK(hipGetLastError());
   }
-  // TreeGravity::initialize (tree on the given positions) + evaluateDerivatives + dt vote
-  void gravityPass(bool mid) {
+  // TreeGravity::initialize (tree on the given positions) + evaluateDerivatives + dt vote.
+  // toState: the evaluation's potential also becomes the State's (potS), as it does for the
+  // start-up dry run, a standalone evaluation and SynchronousRK2's start-of-step evaluation; a
+  // midpoint evaluation's potential is replaced by the state0 copy's in the reference.
+  void gravityPass(bool mid, bool toState = false) {
     if (!cfg.gravity || nActive == 0) return;
     double* xs[3] = {mid ? f.xm[0] : f.x[0], mid ? f.xm[1] : f.x[1], mid ? f.xm[2] : f.x[2]};
     grav.build(xs, f.m, (int)nActive);
     grav.evaluate(f.DvDtG, f.pot, (int)nActive, cfg.G, cfg.softening, cfg.opening, dsc, cfg.ftimestep);
+    if (toState)
+      HIP_CHECK(hipMemcpyAsync(f.potS, f.pot, nActive*sizeof(double), hipMemcpyDeviceToDevice, stream));
   }
   void copyStateToMid() {
     auto cp = [&](double* d, const double* s) {
@@ -1191,7 +1197,7 @@ class Batch {
   }
   void finishNewRuns() {
     omegaPass(false, dinit);
-    if (cfg.gravity) gravityPass(false);
+    if (cfg.gravity) gravityPass(false, true);
     HIP_CHECK(hipStreamSynchronize(stream));
     for (int r = 0; r < maxRuns; ++r) if (hinit[r]) runs[r].needInit = false;
   }
@@ -1271,7 +1277,7 @@ class Batch {
                        cfg.dt_max, cfg.dt_growth, 1);
     HIP_CHECK(hipGetLastError());
     mark("select_dt");
-    gravityPass(false);
+    gravityPass(false, true);
     mark("gravity");
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
@@ -1358,7 +1364,7 @@ class Batch {
     mark("select_dt");
     // derivatives at the start of the step (boundaries applied: the Q sees its gradient's images)
     copyStateToMid();
-    if (cfg.gravity) { gravityPass(false); mark("gravity"); }
+    if (cfg.gravity) { gravityPass(false, true); mark("gravity"); }
     sphPass(false);
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
@@ -1444,10 +1450,58 @@ class Batch {
   void evaluateDerivatives() {
     requireConn();
     copyStateToMid();
-    if (cfg.gravity) gravityPass(true);
+    if (cfg.gravity) gravityPass(true, true);
     sphPass(true);
     HIP_CHECK(hipStreamSynchronize(stream));
     for (auto& r : runs) if (r.state == RUN_ACTIVE) r.stepOut = r.fullOut = true;
+  }
+  // State values from the caller (Physics::registerState's fields as the integrator's State holds
+  // them: position, velocity, mass, mass density, specific thermal energy, H) replace the run's
+  // resident ones in place, in the caller's node order; the EOS policies (pressure, sound speed)
+  // follow, as State::update fires them after any change of density or energy.  Derivatives,
+  // grad-h Omega, the pair accelerations and the connectivity are kept: within a step the
+  // reference evaluates the moved state on the connectivity of the step's start.  Candidate lists
+  // are marked stale, so the next connectivity build starts from the new positions.  Boundary
+  // enforcement of the positions is the caller's (Integrator::enforceBoundaries).
+  void updateState(const double* x, const double* v, const double* m, const double* rho, const double* eps,
+                   const double* H) {
+    if (retire || maxRuns != 1) raise(SPX_ERR_UNSUPPORTED, "state updates apply to single runs");
+    if (!initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called first");
+    if (!x || !v || !m || !rho || !eps || !H) raise(SPX_ERR_ARG, "null argument");
+    const int64_t n = runs[0].n;
+    std::vector<double> hbuf((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+      const double* Hi = H + i*D*D;
+      const double h = Hi[0];
+      for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+          if (Hi[a*D + b] != (a == b ? h : 0.0))
+            raise(SPX_ERR_UNSUPPORTED, "H must be isotropic (h^-1 I); ASPH smoothing tensors are not implemented");
+      if (!(h > 0.0) || !std::isfinite(h)) raise(SPX_ERR_ARG, "H must be positive definite");
+      if (!(m[i] > 0.0) || !(rho[i] > 0.0)) raise(SPX_ERR_ARG, "mass and mass density must be positive");
+      hbuf[i] = h;
+    }
+    if (!stage) stage = dalloc<double>(ld);
+    std::vector<double> buf((size_t)n);
+    auto put = [&](double* dst, const double* src) {
+      upload(stage, src, n*sizeof(double));
+      hipLaunchKernelGGL(k_from_local, dim3(nblocks(n)), dim3(kBlock), 0, stream, dst, (const double*)stage,
+                         (const int64_t*)f.oid, (int)n);
+      HIP_CHECK(hipGetLastError());
+    };
+    for (int a = 0; a < D; ++a) {
+      for (int64_t i = 0; i < n; ++i) buf[i] = x[i*D + a];
+      put(f.x[a], buf.data());
+      for (int64_t i = 0; i < n; ++i) buf[i] = v[i*D + a];
+      put(f.v[a], buf.data());
+    }
+    put(f.m, m);
+    put(f.rho, rho);
+    put(f.eps, eps);
+    put(f.hinv, hbuf.data());
+    eos();
+    HIP_CHECK(hipStreamSynchronize(stream));
+    runs[0].candValid = false;
   }
   void sumDensity() {
     requireConn();
@@ -1608,7 +1662,7 @@ class Batch {
     if (name == "velocity gradient") return ten(f.DvDx);
     if (name == "delta H") return iso(f.DHDt);
     if (name == "max viscous pressure") return scalar(f.maxQ);
-    if (name == "gravitational potential") return scalar(f.pot);
+    if (name == "gravitational potential") return scalar(f.potS);
     if (name == "new H") { needStep(); return iso(f.Hideal); }
     if (name == "mass zeroth moment") { needStep(); return scalar(f.m0); }
     if (name == "effective viscous pressure") { needFull(); return scalar(f.effQ); }
@@ -1624,11 +1678,18 @@ class Batch {
   void conservation(int slot, double* out) {
     checkSlot(slot);
     const int64_t n = runs[slot].n;
-    std::vector<double> m(n), e(n), v(n*D), pot(n);
+    std::vector<double> m(n), e(n), v(n*D), pot(n, 0.0);
     getField(slot, "mass", m.data(), n);
     getField(slot, "specific thermal energy", e.data(), n);
     getField(slot, "velocity", v.data(), n*D);
-    getField(slot, "gravitational potential", pot.data(), n);
+    if (cfg.gravity) {
+      // TreeGravity::extraEnergy: 0.5 sum m phi of the last evaluation (a member, not State)
+      std::vector<int64_t> local, slots;
+      runSlots(slot, local, slots);
+      std::vector<double> tmp(nTotal);
+      download(tmp.data(), f.pot, nTotal*sizeof(double));
+      for (size_t k = 0; k < slots.size(); ++k) pot[local[k]] = tmp[slots[k]];
+    }
     double M = 0, P[3] = {0, 0, 0}, KE = 0, TE = 0, PE = 0;
     for (int64_t i = 0; i < n; ++i) {
       M += m[i];

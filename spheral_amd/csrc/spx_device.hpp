@@ -155,7 +155,14 @@ struct Fields {
   //   pkm (8):         v0 v1 v2 m rho cs Prho 1/rho of the midpoint state (midpoint grad-h pass)
   //   pkv (6):         v^{1/2} (3) DepsDt m 0 for the compatible-energy partition (k_v12)
   double *xh, *vmr, *xhm, *pkm, *pkv;
-  double *pot;       // gravitational potential
+  double *pot;       // gravitational potential of the last TreeGravity evaluation
+  // The potential as the integrator's State holds it: TreeGravity enrolls its potential in the
+  // State (Gravity/TreeGravity.cc:137), so an integrator's state0 copy and the state.assign(state0)
+  // after the midpoint evaluation (Integrator/CheapSynchronousRK2.cc:66-68,107,
+  // Integrator/SynchronousRK2.cc:68-69,101) leave it at the start-of-step evaluation (SynchronousRK2)
+  // or at its value before the step (CheapSynchronousRK2: the start-up dry run or a standalone
+  // evaluation).
+  double *potS;
   uint32_t* nbr;
   // Candidate lists (same blocked layout as nbr): every j with |Hi rij| or |Hj rij| within
   // ext (1 + skin) when the run's list was built, from positions xb and h^-1 hb of that time.

@@ -232,6 +232,15 @@ int spx_run_set_state(spx_run* run, const double* x, const double* v, const doub
   });
 }
 
+int spx_run_update_state(spx_run* run, const double* x, const double* v, const double* m, const double* rho,
+                         const double* eps, const double* H) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    RUN_STATE(run);
+    run->impl->updateState(x, v, m, rho, eps, H);
+  });
+}
+
 int spx_run_get_field(spx_run* run, const char* name, double* out, int64_t capacity, int64_t* count) {
   RUN_OR_FAIL(run);
   return guarded([&] {
@@ -409,6 +418,9 @@ int spx_run_neighbor_builds(spx_run* run, int64_t* builds) {
 int spx_batch_create(const spx_config* cfg, int64_t capacity, int32_t max_runs, int32_t device, spx_batch** out) {
   return guarded([&] {
     if (!cfg || !out) spx::raise(SPX_ERR_ARG, "null argument");
+    // a TreeGravity-only problem is stepped as a single run (spx_run): the batch's start-up and
+    // end-time retirement ride on the hydro passes it does not have
+    if (cfg->gravity_only) spx::raise(SPX_ERR_UNSUPPORTED, "TreeGravity-only problems run as single runs (spx_run)");
     check_device(device);
     auto b = std::make_unique<spx_batch>();
     b->impl = std::make_unique<spx::Batch>(*cfg, capacity, max_runs, device, true);
@@ -475,10 +487,14 @@ int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7) {
 int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs) {
   BATCH_OR_FAIL(batch);
   return guarded([&] {
+    // runs that hold state (active or done); removed runs' slots and particles are reclaimed by
+    // the next admission or step, so they count as free
     auto& B = *batch->impl;
     int32_t k = 0;
-    for (auto& r : B.runs) k += r.state != spx::RUN_FREE;
-    if (particles) *particles = B.nTotal;
+    int64_t np = 0;
+    for (auto& r : B.runs)
+      if (r.state == spx::RUN_ACTIVE || r.state == spx::RUN_DONE) { ++k; np += r.n; }
+    if (particles) *particles = np;
     if (runs) *runs = k;
   });
 }

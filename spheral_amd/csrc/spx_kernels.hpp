@@ -105,6 +105,13 @@ __global__ void __launch_bounds__(kBlock) k_gather(GatherSet s, const uint32_t* 
   const uint32_t q = perm[p];
   for (int k = 0; k < s.count; ++k) s.dst[k][p] = s.src[k][q];
 }
+// Values given in a run's own node order into the engine's current particle order: slot p
+// holds node (oid[p] & low 40 bits) of its run.
+__global__ void __launch_bounds__(kBlock) k_from_local(double* __restrict__ dst, const double* __restrict__ src,
+                                                       const int64_t* __restrict__ oid, int n) {
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p < n) dst[p] = src[oid[p] & (((int64_t)1 << 40) - 1)];
+}
 __global__ void __launch_bounds__(kBlock) k_gather_ids(const int64_t* src, int64_t* dst, const int32_t* rsrc, int32_t* rdst,
                                                        const uint32_t* perm, int n) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;

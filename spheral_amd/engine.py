@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libspheral_amd.so")
 # Every C entry point declared by include/spx_abi.h (checked by the CPU test suite).
 EXPORTS = (
     "spx_last_error", "spx_config_default", "spx_device_count", "spx_table_eval", "spx_table_coefficients",
-    "spx_run_create", "spx_run_destroy", "spx_run_set_state", "spx_run_get_field", "spx_run_initialize",
+    "spx_run_create", "spx_run_destroy", "spx_run_set_state", "spx_run_update_state", "spx_run_get_field",
+    "spx_run_initialize",
     "spx_run_iterate_ideal_h",
     "spx_run_step", "spx_run_advance", "spx_run_time", "spx_run_update_connectivity", "spx_run_neighbor_counts",
     "spx_run_neighbors", "spx_run_sum_density", "spx_run_update_eos", "spx_run_post_state_update",
@@ -59,6 +60,7 @@ def lib():
             "spx_run_create": (C.c_int, [cfgp, C.c_int64, C.c_int32, C.POINTER(P)]),
             "spx_run_destroy": (None, [P]),
             "spx_run_set_state": (C.c_int, [P, D, D, D, D, D, D]),
+            "spx_run_update_state": (C.c_int, [P, D, D, D, D, D, D]),
             "spx_run_get_field": (C.c_int, [P, C.c_char_p, D, C.c_int64, I64]),
             "spx_run_initialize": (C.c_int, [P]),
             "spx_run_iterate_ideal_h": (C.c_int, [P, C.c_int32, C.c_double, C.POINTER(C.c_int32)]),
@@ -157,6 +159,12 @@ class Run:
     def set_state(self, x, v, m, rho, eps, H):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, v, m, rho, eps, H)]
         _check(lib().spx_run_set_state(self.h, *[_dp(a) for a in arrs]))
+
+    def update_state(self, x, v, m, rho, eps, H):
+        """Replace the run's state in place (spx_run_update_state): derivatives, Omega and the
+        connectivity are kept, pressure and sound speed follow from the new density and energy."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, v, m, rho, eps, H)]
+        _check(lib().spx_run_update_state(self.h, *[_dp(a) for a in arrs]))
 
     def field(self, name):
         k = FIELD_KIND[name]

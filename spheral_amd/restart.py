@@ -7,9 +7,9 @@ path (FlatFileIO.cc:486-545).  Geometric values print as ``( a b c )`` (Geometry
 unique elements in storage order (xx | xx xy yy | xx xy xz yy yz zz, GeomSymmetricTensorBase.hh).
 FlatFileIO writes with 20 significant digits, so doubles survive the round trip.
 
-A NodeList dumps ``<label>/numNodes``, ``mass``, ``positions``, ``velocity``, ``H`` (NodeList.cc:
-503-520) and a FluidNodeList adds ``massDensity`` and ``specificThermalEnergy`` (FluidNodeList.cc:
-160-170); the restart registrar prefixes each object's label (DataOutput/RestartRegistrar.cc:
+A NodeList dumps ``<label>/name``, ``numNodes``, ``mass``, ``positions``, ``velocity``, ``H``,
+``work`` (NodeList.cc:503-518; restoreState reads every one of them back, NodeList.cc:526-541) and a
+FluidNodeList adds ``massDensity`` and ``specificThermalEnergy`` (FluidNodeList.cc:160-186); the restart registrar prefixes each object's label (DataOutput/RestartRegistrar.cc:
 125-176), and older files put every label under a ``restart_cycle<N>/`` group.
 
 ``read_flat`` parses a file into {path: value}; ``nodelist_state`` extracts one NodeList as the
@@ -95,9 +95,12 @@ def nodelist_state(data, label, dim):
     for c, (a, b) in enumerate(idx):
         H[:, a, b] = Hs[:, c]
         H[:, b, a] = Hs[:, c]
-    return dict(x=vec("positions"), v=vec("velocity"), m=np.array(data[p + "mass"], np.float64),
-                rho=np.array(data[p + "massDensity"], np.float64),
-                eps=np.array(data[p + "specificThermalEnergy"], np.float64), H=H)
+    st = dict(x=vec("positions"), v=vec("velocity"), m=np.array(data[p + "mass"], np.float64),
+              rho=np.array(data[p + "massDensity"], np.float64),
+              eps=np.array(data[p + "specificThermalEnergy"], np.float64), H=H)
+    if p + "work" in data:
+        st["work"] = np.array(data[p + "work"], np.float64)
+    return st
 
 
 def _num(x):
@@ -119,6 +122,10 @@ def write_nodelist(f, label, state, prefix=""):
     idx = _SYM_INDEX[dim]
     H = np.asarray(state["H"])
     f.write(f"{p}H {n} " + " ".join(grp([h[a, b] for a, b in idx]) for h in H) + "\n")
+    # the NodeList's work field (a load-balancing measure the hydro does not evolve): carried through
+    # when the state has it, zeros otherwise
+    work = np.asarray(state.get("work", np.zeros(n)), dtype=np.float64)
+    f.write(f"{p}work {n} " + " ".join(_num(w) for w in work) + "\n")
     f.write(f"{p}massDensity {n} " + " ".join(_num(r) for r in state["rho"]) + "\n")
     f.write(f"{p}specificThermalEnergy {n} " + " ".join(_num(e) for e in state["eps"]) + "\n")
 

@@ -77,7 +77,9 @@ class _Lane:
             try:
                 rid = self.batch.add_run(req.state, req.t_end)
             except engine.SpxError as e:
-                if e.code == abi.SPX_ERR_CAPACITY:
+                # an empty batch admits any request within its capacity: failing then is an error,
+                # not a reason to wait (nothing on the GPU would free room)
+                if e.code == abi.SPX_ERR_CAPACITY and self.active:
                     return
                 raise
             self.queue.popleft()

@@ -38,3 +38,11 @@ def test_iterate_ideal_h_needs_initialize(gpu):
     assert err.value.code == -3
     g.initialize()
     assert g.iterate_ideal_h(50, 1e-4) > 1
+
+
+def test_gravity_only_batch_is_rejected(gpu):
+    """A TreeGravity-only problem is stepped as a single run; a batch of them is refused."""
+    cfg, _, _ = problems.collisionless_sphere_3d(nr=3)
+    with pytest.raises(SpxError) as err:
+        gpu.Batch(cfg, 1000, 1)
+    assert err.value.code == UNSUP

@@ -32,3 +32,28 @@ def test_served_results_equal_alone(gpu):
         assert h.steps == steps
         for f in serve.RESULT_FIELDS:
             assert np.array_equal(h.result[f], ref[f]), (k, n, f)
+
+
+@pytest.mark.parametrize("slots,trace", [
+    (1, [(900, 0.004), (700, 0.003)]),         # one slot: the second request waits for the first
+    (2, [(800, 0.004), (800, 0.004)]),         # identical requests finish on the same step
+])
+def test_lane_admits_after_every_run_finished(gpu, slots, trace):
+    """Every active run of a lane finishing in the same step leaves the batch empty with requests
+    queued: the removed runs' slots and particles count as free, so the lane admits the queue
+    instead of waiting on a step that never comes."""
+    print(f"[case] serve slots={slots} trace={trace}", flush=True)
+    cfg = problems.sod_tube(2, 1000)[0]
+    srv = serve.Server({"sod2d": (cfg, 2000, slots)})
+    handles = [srv.submit("sod2d", problems.sod_tube(2, n)[1], te) for n, te in trace]
+    handles.append(srv.submit("sod2d", problems.sod_tube(2, 600)[1], 0.002))
+    for h in handles:
+        assert h.event.wait(120)
+        assert h.error is None
+    lane = srv.lanes["sod2d"]
+    assert lane.batch.usage() == (0, 0)
+    srv.close()
+    for (n, te), h in zip(trace + [(600, 0.002)], handles):
+        ref, steps = serve.solve_alone(cfg, problems.sod_tube(2, n)[1], te)
+        assert h.steps == steps
+        assert np.array_equal(h.result["mass density"], ref["mass density"])

@@ -63,3 +63,25 @@ def test_value_layout_is_the_reference_stream_format():
     assert lines["n/positions"] == "n/positions 1 ( 1 2 3 )"
     assert lines["n/H"] == "n/H 1 ( 1 2 3 4 5 6 )"
     assert lines["n/numNodes"] == "n/numNodes 1"
+
+
+def test_written_paths_cover_what_the_reference_restores():
+    """Every path NodeList::restoreState and FluidNodeList::restoreState read (NodeList.cc:529-541,
+    FluidNodeList.cc:181-185; FlatFileIO fails on a missing path) is written, and a state's work
+    field is carried through."""
+    buf = io.StringIO()
+    st = dict(x=np.zeros((2, 2)), v=np.zeros((2, 2)), m=[1.0, 1.0], rho=[1.0, 1.0], eps=[1.0, 1.0],
+              H=np.stack([np.eye(2)] * 2), work=[0.5, 0.25])
+    restart.write_nodelist(buf, "fluid", st, prefix="restart_cycle0/")
+    paths = {l.split()[0] for l in buf.getvalue().splitlines()}
+    need = {"name", "numNodes", "mass", "positions", "velocity", "H", "work", "massDensity",
+            "specificThermalEnergy"}
+    assert {"restart_cycle0/fluid/" + k for k in need} <= paths
+    lines = {l.split()[0]: l for l in buf.getvalue().splitlines()}
+    assert lines["restart_cycle0/fluid/work"] == "restart_cycle0/fluid/work 2 0.5 0.25"
+    path = os.path.join(os.environ.get("TMPDIR", "/tmp"), "spx_restart_work")
+    with open(path, "w") as f:
+        f.write(buf.getvalue())
+    back = restart.read_flat(path)
+    os.remove(path)
+    assert np.array_equal(restart.nodelist_state(back, "fluid", 2)["work"], [0.5, 0.25])
