@@ -62,11 +62,37 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(n1d, steps, precision, integrator=abi.SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2):
-    """The oracle (FP64 C++ restatement, OpenMP) on a bounded sample of the same workload."""
+def cpu_share():
+    """Host cores this process may use: the affinity mask, bounded by a cgroup CPU quota when one
+    is set (cgroup v2 cpu.max), and the machine's nproc beside them."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, per = fh.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q)//int(per)))
+    except (OSError, ValueError):
+        pass
+    share = min(allowed, quota) if quota else allowed
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return {"nproc": nproc, "affinity": allowed, "cgroup_quota": quota, "omp_num_threads": omp, "threads": share}
+
+
+def cpu_baseline(n1d, steps, precision, integrator=abi.SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2, reps=3, warmup=2):
+    """The oracle (FP64 C++ restatement, OpenMP) on the same Taylor-Green workload: `warmup`
+    untimed steps, then `reps` timed segments of `steps` steps; the value is the median segment's
+    rate.  Threads: every host core this process may use (see cpu_share)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = min(os.cpu_count() or 1, int(os.environ.get("SPX_CPU_THREADS", "16")))
+    share = cpu_share()
+    threads = share["threads"]
     O.set_threads(threads)
     cfg, st = problems.taylor_green_3d(n1d)
     cfg.integrator = integrator
@@ -74,15 +100,23 @@ def cpu_baseline(n1d, steps, precision, integrator=abi.SPX_INTEGRATOR_CHEAP_SYNC
     r = O.OracleRun(cfg, n)
     r.set_state(**st)
     r.initialize()
-    r.step()                                       # warm-up step
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(warmup):
         r.step()
-    dt = time.perf_counter() - t0
-    return {"value": n*steps/dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+    seg = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.step()
+        seg.append(time.perf_counter() - t0)
+        print(f"[bench] cpu segment {len(seg)}/{reps}: {seg[-1]:.2f} s", file=sys.stderr, flush=True)
+    med = float(np.median(seg))
+    return {"value": n*steps/med, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle (FP64 C++/OpenMP restatement) on the same Taylor-Green workload at {n1d}^3="
-                      f"{n} particles, {steps} timed steps after 1 warm-up, {threads} threads, {cpu_model()}",
-            "seconds": dt}
+                      f"{n} particles: {warmup} warm-up steps, then the median of {reps} timed segments of "
+                      f"{steps} step(s); {threads} OpenMP threads = this process's CPU share "
+                      f"(nproc {share['nproc']}, affinity {share['affinity']}, cgroup quota {share['cgroup_quota']}, "
+                      f"OMP_NUM_THREADS {share['omp_num_threads']}); {cpu_model()}",
+            "segments_s": seg, "share": share}
 
 
 def ensemble(args):
@@ -311,6 +345,28 @@ def accuracy(args):
     print(json.dumps({"metric": "mixed-precision L1 vs FP64 and the roundoff floor", "cases": rows}), flush=True)
 
 
+def rebuild_amortised(run, n, cycles=2, max_steps=600):
+    """particle-steps/s over `cycles` whole candidate-list rebuild cycles (None when fewer rebuilds
+    happen within max_steps)."""
+    b = run.neighbor_builds()
+    for _ in range(max_steps):
+        run.step()
+        if run.neighbor_builds() > b:
+            break
+    else:
+        return None
+    b = run.neighbor_builds()
+    steps = 0
+    t0 = time.perf_counter()
+    while run.neighbor_builds() < b + cycles and steps < max_steps:
+        run.step()
+        steps += 1
+    dt = time.perf_counter() - t0
+    if run.neighbor_builds() < b + cycles:
+        return None
+    return {"value": n*steps/dt, "steps": steps, "builds": cycles, "ms_per_step": 1000.0*dt/steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,8 +374,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n1d", type=int, default=128)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--cpu-n1d", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-n1d", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps per timed CPU segment (median of 3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--integrator", choices=["cheap", "synchronous"], default="cheap",
                     help="taylor_green: CheapSynchronousRK2 (the headline) or SynchronousRK2")
@@ -405,7 +461,7 @@ def main():
                    "t_end": t, "cycles": cycle},
         "roofline": {
             "kernel": "k_sph (SPH + SPHSmoothingScale pair pass)",
-            "bound": "mfma",
+            "bound": "valu",
             "pipe": ("VALU FP64 (no MFMA: no dense contraction); peak = MI355X FP64 78.6 TFLOP/s"
                      if prec == abi.SPX_PREC_FP64 else
                      "VALU FP32 pair arithmetic, FP64 accumulation (no MFMA); peak = MI355X FP32 157.3 TFLOP/s"),
@@ -429,13 +485,19 @@ def main():
         "conservation": {"momentum": cons[1:4].tolist(), "energy": float(cons[4] + cons[5])},
     }
     line["step_roofline"]["frac"] = value/line["step_roofline"]["ceiling_particle_steps_per_s"]
+    # The timed window may hold few candidate-list rebuilds (none at 20 steps); the rate over whole
+    # rebuild cycles: from the step after one rebuild through the step of the second rebuild after it.
+    amort = rebuild_amortised(run, n)
+    if amort is not None:
+        line["value_rebuild_amortised"] = amort["value"]
+        line["rebuild_window"] = {k: amort[k] for k in ("steps", "builds", "ms_per_step")}
     # attainable HBM rate on this box: plain streaming copy kernel, 1 GiB read + 1 GiB write per launch
     line["hbm_copy_gbps"] = engine.bandwidth_probe(1 << 30, 10)
     line["roofline"]["traffic"] = measured_traffic(sph_kernel(prec))
     if not args.no_cpu:
         print("[bench] timing the CPU baseline (oracle) on a bounded sample", file=sys.stderr, flush=True)
         cb = cpu_baseline(args.cpu_n1d, args.cpu_steps, prec, cfg.integrator)
-        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "segments_s")}
         line["speedup_vs_cpu"] = value/cb["value"]
     else:
         line["cpu_baseline"] = None

@@ -64,6 +64,13 @@ extern "C" {
 #define SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2  0
 #define SPX_INTEGRATOR_SYNCHRONOUS_RK2        1
 
+/* TreeGravity's time step vote (Gravity/TreeGravity.cc:483-520; GravityTimeStepType):
+   AccelerationRatio ftimestep*min sqrt(softening/|a|), or DynamicalTime ftimestep/sqrt(G rhoMax)
+   with rhoMax the largest local effective density over the interaction buckets
+   (TreeGravity.cc:239-300). */
+#define SPX_GRAVITY_DT_ACCELERATION_RATIO  0
+#define SPX_GRAVITY_DT_DYNAMICAL_TIME      1
+
 /* Per-side boundary condition of one axis (Boundary/PeriodicBoundary.cc,
    Boundary/ReflectingBoundary.cc).  Periodic must be set on both sides. */
 #define SPX_BC_NONE        0
@@ -88,7 +95,8 @@ extern "C" {
      viscosity, eta_crit_frac, eta_fold_frac : the Q and the Limited Q's ctor arguments
        (PYB11/ArtificialViscosity/LimitedMonaghanGingoldViscosity.py:24-32; defaults 1.0, 0.2)
      integrator, gravity_only : the integrator class and its package list (an N-body script
-       appends TreeGravity alone, tests/functional/Gravity/CollisionlessSphereCollapse.py:178-186) */
+       appends TreeGravity alone, tests/functional/Gravity/CollisionlessSphereCollapse.py:178-186)
+     gravity_time_step : TreeGravity timeStepChoice (Gravity/TreeGravity.hh:46-50) */
 typedef struct spx_config {
   int32_t dim;                       /* 1, 2 or 3                                    */
   int32_t kernel;                    /* SPX_KERNEL_*                                 */
@@ -119,6 +127,7 @@ typedef struct spx_config {
   int32_t integrator;                /* SPX_INTEGRATOR_*                             */
   int32_t gravity_only;              /* 1: TreeGravity is the only package (no SPH;  */
                                      /*    needs gravity = 1 and SynchronousRK2)     */
+  int32_t gravity_time_step;         /* SPX_GRAVITY_DT_*: TreeGravity timeStepChoice */
 } spx_config;
 
 typedef struct spx_run spx_run;
@@ -260,8 +269,17 @@ int spx_run_conservation(spx_run* run, double* out7);
    clears the sums.  names: comma-separated labels written to buf. */
 int spx_run_kernel_times(spx_run* run, double* ms, int32_t max_entries,
                          int32_t* n_entries, char* names, int64_t names_cap);
-/* Unique pairs of the current connectivity (NodePairList size). */
+/* Unique pairs of the current connectivity (NodePairList size): ConnectivityMap::nodePairList()
+   (Neighbor/ConnectivityMap.hh:79; pairs admitted by calculatePairInteraction,
+   ConnectivityMapInline.hh:296-309), counted on the device into a library-allocated buffer. */
 int spx_run_num_pairs(spx_run* run, int64_t* npairs);
+/* spx_run_num_pairs without any device allocation.  No counterpart in the reference: for callers
+   that cannot allocate device memory (code under stream/graph capture, serving loops that forbid
+   hipMalloc).  Counts into a 512 KiB device array that the library holds per GPU for the life of
+   the process; runs of more than 16,777,216 active particles get SPX_ERR_CAPACITY.  Calls that
+   overlap on one GPU share that array and race; the array keeps the last call's counts.  Calls
+   no hipMalloc/hipFree.  Requires the run's connectivity (SPX_ERR_STATE otherwise). */
+int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs);
 /* Candidate-list builds of the run so far (no counterpart in the reference, which rebuilds
    its ConnectivityMap from scratch every step: a measurement aid).  The neighbour sets are
    the reference's every step either way; a build happens at start-up, after

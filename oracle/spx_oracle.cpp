@@ -34,7 +34,7 @@
  *                    Integrator/Integrator.cc:67-111,
  *                    DataBase/State.cc:223-325, IncrementStateInline.hh, IncrementBoundedStateInline.hh,
  *                    ReplaceBoundedStateInline.hh
- *   TreeGravity      Gravity/TreeGravity.cc:30-90,143-328,362-477,483-497,818-950,
+ *   TreeGravity      Gravity/TreeGravity.cc:30-90,143-328,362-477,483-520,818-950,
  *                    Gravity/TreeGravityInline.hh:37-196
  *
  * How each part is pinned (tests/test_oracle_*.py): closed forms of the base
@@ -473,7 +473,7 @@ template<int D> struct Sim {
   double t = 0, lastDt = 1e-5; int64_t cycle = 0; bool haveState = false, initialized = false;
   // gravity
   std::vector<std::unordered_map<uint64_t, Cell<D>>> tree; V gxmin, gxmax; double boxLength = 0;
-  double dtMinAcc = 0.0, extraEnergy = 0.0;
+  double dtMinAcc = 0.0, extraEnergy = 0.0, rhoMax = 0.0;
   // The potential as the integrator's State holds it.  TreeGravity enrolls its potential FieldList
   // in the State (TreeGravity.cc:137), so the state0 copy an integrator takes before its midpoint
   // evaluation carries the potential of that moment, and state.assign(state0) puts it back after the
@@ -1019,13 +1019,17 @@ template<int D> struct Sim {
   void gravityEvaluate() {                                      // TreeGravity.cc:143-328, 818-950
     const double boxLength2 = boxLength*boxLength, soft2 = cfg.softening*cfg.softening, open2 = cfg.opening*cfg.opening;
     const unsigned numLevels = (unsigned)tree.size();
+    const bool dynamical = cfg.gravity_time_step == SPX_GRAVITY_DT_DYNAMICAL_TIME;
     double result = std::numeric_limits<double>::max();
+    double rhoMaxAll = 0.0;
     std::fill(potential.begin(), potential.end(), 0.0);
     if (numLevels >= 1) {
       const Cell<D>& root = tree[0].begin()->second;
-#pragma omp parallel for schedule(dynamic, 64) reduction(min:result)
+#pragma omp parallel for schedule(dynamic, 64) reduction(min:result) reduction(max:rhoMaxAll)
       for (int64_t i = 0; i < nInt; ++i) {
         const V xi = x[i]; V acc = DvDtG[i]; double phi = 0.0;
+        // the interaction buckets of this point, in walk order (TreeGravity.cc:893-895, 921-923)
+        std::vector<double> mb; std::vector<V> xb;
         unsigned ilevel = 0;
         std::vector<const Cell<D>*> rem(root.dptr.begin(), root.dptr.end()), nd;
         while (!rem.empty() && ++ilevel < numLevels) {
@@ -1039,6 +1043,7 @@ template<int D> struct Sim {
               rji2 += soft2;
               acc = add(acc, scale(cell.Mglobal*forceLaw(rji2), nhat));
               phi -= cell.Mglobal*potentialLaw(rji2);
+              if (dynamical) { mb.push_back(cell.Mglobal); xb.push_back(cell.xcm); }
             } else if (cell.dptr.empty()) {
               for (size_t jj = 0; jj < cell.masses.size(); ++jj) {
                 const double mj = cell.masses[jj]; const V xj2 = sub(cell.positions[jj], xi); double r2 = mag2(xj2);
@@ -1049,6 +1054,7 @@ template<int D> struct Sim {
                   phi -= mj*potentialLaw(r2);
                 }
               }
+              if (dynamical) { mb.push_back(cell.Mglobal); xb.push_back(cell.xcm); }
             } else {
               nd.insert(nd.end(), cell.dptr.begin(), cell.dptr.end());
             }
@@ -1058,14 +1064,53 @@ template<int D> struct Sim {
         DvDtG[i] = acc; potential[i] = phi;
         const double amag = mag(acc);
         if (amag > 0.0) result = std::min(result, std::sqrt(cfg.softening/amag));
+        if (dynamical && !mb.empty()) rhoMaxAll = std::max(rhoMaxAll, dynamicalDensity(xi, mb, xb, soft2));
       }
     }
     dtMinAcc = std::min(std::numeric_limits<double>::max(), result/std::sqrt(cfg.G));
+    if (dynamical) rhoMax = rhoMaxAll;
     extraEnergy = 0.0;
     for (int64_t i = 0; i < nInt; ++i) {
       DvDtG[i] = scale(cfg.G, DvDtG[i]); potential[i] *= cfg.G;
       extraEnergy += 0.5*m[i]*potential[i];
     }
+  }
+
+  // The DynamicalTime choice's local effective density of one point (TreeGravity.cc:239-300, after
+  // Zemp et al. 2007): rhoenc_j = m_j/(|x_j - x_i|^2 + soft^2)^(D/2) per interaction bucket; the
+  // ntop = max(1, floor(0.05 nbuckets)) largest, found with std::partial_sort in decreasing order;
+  // for each of them the sum of rhoenc over the buckets inside its cone (|xpc|^2 <= 4|xpcmax|^2 and
+  // 0.75 |xpcmax||xpc| <= xpcmax.xpc).  As the reference writes it, that sum reads the partially
+  // sorted array by bucket index (rhoenc[j].first) while taking the cone test from bucket j's own
+  // position, so it depends on the order std::partial_sort leaves the array in; the same libstdc++
+  // algorithm is called here.  Returns the largest of the ntop sums.
+  static double dynamicalDensity(const V& xi, const std::vector<double>& mb, const std::vector<V>& xb, double soft2) {
+    const unsigned nb = (unsigned)mb.size();
+    std::vector<std::pair<double, unsigned>> rhoenc;
+    rhoenc.reserve(nb);
+    for (unsigned j = 0; j != nb; ++j) {
+      const double r = std::sqrt(mag2(sub(xb[j], xi)) + soft2);
+      double p = r;
+      for (int a = 1; a < D; ++a) p *= r;                                   // Dimension::pownu
+      rhoenc.push_back(std::make_pair(mb[j]/p, j));
+    }
+    const unsigned ntop = std::max(1U, unsigned(0.05*nb));
+    std::partial_sort(rhoenc.begin(), rhoenc.begin() + ntop, rhoenc.end(),
+                      [](const std::pair<double, unsigned>& l, const std::pair<double, unsigned>& r) { return l.first > r.first; });
+    double best = 0.0;
+    for (unsigned k = 0; k != ntop; ++k) {
+      const unsigned jmax = rhoenc[k].second;
+      const V xpcmax = sub(xb[jmax], xi);
+      const double rpcmax2 = mag2(xpcmax);
+      double rholocal = 0.0;
+      for (unsigned j = 0; j != nb; ++j) {
+        const V xpc = sub(xb[j], xi);
+        const double rpc2 = mag2(xpc);
+        if (rpc2 <= 4.0*rpcmax2 && 0.75*std::sqrt(rpcmax2*rpc2) <= dot(xpcmax, xpc)) rholocal += rhoenc[j].first;
+      }
+      if (best < rholocal) best = rholocal;
+    }
+    return best;
   }
 
   // ---- GenericHydro::dt (Physics/GenericHydro.cc:112-381) -----------------------
@@ -1098,7 +1143,12 @@ template<int D> struct Sim {
     }
     return minDt*cfg.cfl;
   }
-  double gravityDt() const { return cfg.ftimestep*dtMinAcc; }   // AccelerationRatio (TreeGravity.cc:491-496)
+  // TreeGravity::dt (TreeGravity.cc:483-520): AccelerationRatio ftimestep*min sqrt(soft/|a|)/sqrt(G),
+  // DynamicalTime ftimestep*sqrt(1/(G rhoMax)).
+  double gravityDt() const {
+    if (cfg.gravity_time_step == SPX_GRAVITY_DT_DYNAMICAL_TIME) return cfg.ftimestep*std::sqrt(1.0/(cfg.G*rhoMax));
+    return cfg.ftimestep*dtMinAcc;
+  }
 
   // Integrator::selectDt (Integrator.cc:116-169)
   double selectDt(double dtMin, double dtMax, double mult) const {

@@ -26,6 +26,9 @@ SPX_Q_LIMITED_MONAGHAN_GINGOLD = 1
 SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2 = 0
 SPX_INTEGRATOR_SYNCHRONOUS_RK2 = 1
 
+SPX_GRAVITY_DT_ACCELERATION_RATIO = 0
+SPX_GRAVITY_DT_DYNAMICAL_TIME = 1
+
 SPX_BC_NONE = 0
 SPX_BC_PERIODIC = 1
 SPX_BC_REFLECTING = 2
@@ -86,6 +89,7 @@ class SpxConfig(C.Structure):
         ("eta_fold_frac", C.c_double),
         ("integrator", C.c_int32),
         ("gravity_only", C.c_int32),
+        ("gravity_time_step", C.c_int32),
     ]
 
     def copy(self):
@@ -148,6 +152,7 @@ def default_config(dim=3, kernel=SPX_KERNEL_BSPLINE, **overrides):
     c.eta_fold_frac = 0.2
     c.integrator = SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2
     c.gravity_only = 0
+    c.gravity_time_step = SPX_GRAVITY_DT_ACCELERATION_RATIO
     for k, v in overrides.items():
         if k in ("bc_lo", "bc_hi", "xmin", "xmax"):
             arr = getattr(c, k)

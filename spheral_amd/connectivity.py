@@ -113,6 +113,20 @@ def ghost_nodes(cfg, x, H, boundaries=None):
     return parent[n:], control[n:], code[n:], pos[n:]
 
 
+def node_pair_hash(i_node, i_list, j_node, j_list):
+    """NodePairIdxType::hash (Neighbor/NodePairIdxType.hh:34-53) on 64-bit size_t: the pair is
+    flipped to (lower, higher) by (list, node), then (i_list << 59) + (i_node << 32) +
+    (j_list << 27) + j_node.  Requires lists < 32 and nodes < 2^27, as the reference does."""
+    i_node, i_list, j_node, j_list = (np.asarray(a, np.uint64) for a in (i_node, i_list, j_node, j_list))
+    if np.any(i_node >= 1 << 27) or np.any(j_node >= 1 << 27) or np.any(i_list >= 32) or np.any(j_list >= 32):
+        raise ValueError("NodePairIdxType needs node indices < 2^27 and NodeList indices < 32")
+    flip = (i_list > j_list) | ((i_list == j_list) & (i_node > j_node))
+    a_l, a_n = np.where(flip, j_list, i_list), np.where(flip, j_node, i_node)
+    b_l, b_n = np.where(flip, i_list, j_list), np.where(flip, i_node, j_node)
+    u = np.uint64
+    return (a_l << u(59)) + (a_n << u(32)) + (b_l << u(27)) + b_n
+
+
 class ConnectivityDump:
     """ConnectivityMap contents in the reference's numbering for NodeLists `labels`.
 
@@ -140,6 +154,35 @@ class ConnectivityDump:
 
     def numNeighborsForNode(self, k, i):
         return sum(len(v) for v in self.connectivityForNode(k, i))
+
+    def nodePairList(self):
+        """ConnectivityMap::nodePairList() (Neighbor/ConnectivityMap.hh:79): one NodePairIdxType per
+        (internal node i of NodeList k, neighbour j of NodeList l) that calculatePairInteraction
+        admits -- l > k, or l == k and j > i, or l < k and j a ghost of l
+        (ConnectivityMapInline.hh:296-309) -- stored as (i_node, i_list, j_node, j_list) in
+        ascending NodePairIdxType::hash order (ConnectivityMap.cc:925-927, 1028;
+        NodePairIdxType.hh:34-53)."""
+        parts = []
+        for k in range(len(self.labels)):
+            for l in range(len(self.labels)):
+                off, idx = self._csr[(k, l)]
+                i = np.repeat(np.arange(self.sizes[k], dtype=np.int64), np.diff(off))
+                j = np.asarray(idx, np.int64)
+                if l > k:
+                    keep = np.ones(len(j), bool)
+                elif l == k:
+                    keep = j > i
+                else:
+                    keep = j >= self.sizes[l]
+                parts.append((i[keep], np.full(keep.sum(), k, np.int64), j[keep], np.full(keep.sum(), l, np.int64)))
+        pl = np.zeros(sum(len(p[0]) for p in parts),
+                      dtype=[("i_node", np.int64), ("i_list", np.int64), ("j_node", np.int64), ("j_list", np.int64)])
+        if len(pl) == 0:
+            return pl
+        for f, a in zip(pl.dtype.names, map(np.concatenate, zip(*parts))):
+            pl[f] = a
+        order = np.argsort(node_pair_hash(pl["i_node"], pl["i_list"], pl["j_node"], pl["j_list"]), kind="stable")
+        return pl[order]
 
     def write(self, f):
         """Append the dump to the open text file `f`, one FlatFileIO-style line per path."""

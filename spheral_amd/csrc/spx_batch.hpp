@@ -156,6 +156,8 @@ class Batch {
   bool orderDirty = false;  // a run was added, finished or removed since the last reorder
   int64_t candBuilds = 0;   // candidate-list builds, summed over runs
   int64_t npairsCache = -1;
+  unsigned long long* pairScratch = nullptr;   // numPairs' partial counts (library-allocated)
+  int pairScratchBlocks = 0;
   bool mixed = false;
   // per-class kernel time (ms) summed over the steps since timing was enabled
   bool timing = false;
@@ -363,7 +365,7 @@ class Batch {
     f.vmr = dalloc<double>((size_t)4*ld);
     f.xhm = dalloc<double>((size_t)4*ld);
     f.pkm = dalloc<double>((size_t)8*ld);
-    f.pkv = dalloc<double>((size_t)6*ld);
+    f.pkv = dalloc<double>((size_t)8*ld);
     f.oid = dalloc<int64_t>(ld);
     oidAlt = dalloc<int64_t>(ld);
     f.run = dalloc<int32_t>(ld);
@@ -469,7 +471,7 @@ class Batch {
     f.nbr = nullptr;
     f.pdd = nullptr;
     f.nbr = dalloc<uint32_t>((size_t)c*ld);
-    f.pdd = mixed ? (void*)dalloc<float>((size_t)D*c*ld) : (void*)dalloc<double>((size_t)D*c*ld);
+    f.pdd = mixed ? (void*)dalloc<float>((size_t)c*ld) : (void*)dalloc<double>((size_t)c*ld);
     f.cap = c;
     npairsCache = -1;
   }
@@ -898,6 +900,7 @@ class Batch {
   // sum density and dt vote.
   void buildConnectivity(bool withDensityDt, bool force = false) {
     if (nTotal == 0) raise(SPX_ERR_STATE, "no state uploaded");
+    npairsCache = -1;                // every pass re-derives the neighbour lists
     bool any = false;
     for (int r = 0; r < maxRuns; ++r) {
       RunInfo& ri = runs[r];
@@ -1582,21 +1585,30 @@ class Batch {
   }
 
   // Unique pairs of the active runs: internal pairs once, image pairs once per (node, image) entry.
-  int64_t numPairs() {
+  // scratch: nblocks(nActive) + 1 counters of device memory for the per-block partial counts and
+  // the total (the library's own buffer when null).
+  int64_t numPairs(unsigned long long* scratch = nullptr) {
     requireConn();
-    if (npairsCache >= 0) return npairsCache;
-    std::vector<int32_t> cnt(nActive);
-    download(cnt.data(), f.cnt, nActive*sizeof(int32_t));
-    std::vector<uint32_t> ell((size_t)f.cap*ld);
-    download(ell.data(), f.nbr, ell.size()*sizeof(uint32_t));
-    int64_t np = 0;
-    for (int64_t p = 0; p < nActive; ++p)
-      for (int k = 0; k < cnt[p]; ++k) {
-        const uint32_t e = ell[ell_at(k, (int)p, (size_t)ld)];
-        if ((e >> kImgShift) != 0 || (int64_t)(e & kIdxMask) > p) ++np;
+    if (npairsCache >= 0 && !scratch) return npairsCache;
+    const int nb = std::max(1, nblocks(nActive));
+    if (!scratch) {
+      if (!pairScratch || pairScratchBlocks < nb) {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        dfree(pairScratch);
+        pairScratch = dalloc<unsigned long long>((size_t)nb + 1);
+        pairScratchBlocks = nb;
       }
-    npairsCache = np;
-    return np;
+      scratch = pairScratch;
+    }
+    hipLaunchKernelGGL(k_count_pairs, dim3(nb), dim3(kBlock), 0, stream, (const uint32_t*)f.nbr, (const int32_t*)f.cnt,
+                       (size_t)ld, (int)nActive, scratch);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, stream, (const unsigned long long*)scratch, nb, scratch + nb);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long np = 0;
+    download(&np, scratch + nb, sizeof(np));
+    npairsCache = (int64_t)np;
+    return npairsCache;
   }
 
   // ---------------------------------------------------------------- field access

@@ -153,7 +153,9 @@ struct Fields {
   //   vmr (4):         v0 v1 v2 m of the connectivity state (k_pack)
   //   xhm (4):         x0 x1 x2 h^-1 of the midpoint state (k_midstep)
   //   pkm (8):         v0 v1 v2 m rho cs Prho 1/rho of the midpoint state (midpoint grad-h pass)
-  //   pkv (6):         v^{1/2} (3) DepsDt m 0 for the compatible-energy partition (k_v12)
+  //   pkv (8):         v^{1/2} (3) DepsDt m x (3) for the compatible-energy partition: the
+  //                    half-step velocity and DepsDt (SPH pass in the step, else k_v12), m and
+  //                    the position the SPH pass evaluated at (SPH pass)
   double *xh, *vmr, *xhm, *pkm, *pkv;
   double *pot;       // gravitational potential of the last TreeGravity evaluation
   // The potential as the integrator's State holds it: TreeGravity enrolls its potential in the
@@ -169,9 +171,9 @@ struct Fields {
   uint32_t* cand;
   int32_t* ccnt;
   double *xb[3], *hb;
-  // Pairwise acceleration term of each ELL entry (the reference's pairAccelerations,
-  // without the partner mass), in the pair arithmetic type: component c of entry k of
-  // slot i at pdd[(c*cap + k)*ld + i], from the slot's side of the canonical pair.
+  // Pairwise acceleration coefficient of each ELL entry, in the pair arithmetic type: the
+  // reference's pairAccelerations of the pair (i, j) is -m_j C r_ij (isotropic H), C of entry k
+  // of slot i at pdd[k*ld + i] (the same bits from both sides of the pair).
   void* pdd;
   int32_t* cnt;
   int64_t* oid;      // (run slot << 40) | node index within the run

@@ -963,4 +963,48 @@ __global__ void __launch_bounds__(kBlock) k_finite(Fields f, int n, Scalars* sc)
   if (!ok) atomicOr(&sc[f.run[i]].nonfinite, 1);
 }
 
+// Unique-pair count of the neighbour lists (ConnectivityMap::nodePairList().size(),
+// Neighbor/ConnectivityMap.cc:925-927): an entry (i, j) is i's pair when j is an image or a later
+// slot (calculatePairInteraction, ConnectivityMapInline.hh:305-307).  One partial count per block
+// (no atomics), summed by k_sum_partials.
+__global__ void __launch_bounds__(kBlock) k_count_pairs(const uint32_t* nbr, const int32_t* cnt, size_t ld, int n,
+                                                        unsigned long long* partial) {
+  __shared__ unsigned long long red[kBlock/64];
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  unsigned long long c = 0;
+  if (i < n) {
+    const int nn = cnt[i];
+    for (int k = 0; k < nn; ++k) {
+      const uint32_t e = nbr[ell_at(k, i, ld)];
+      c += ((e >> kImgShift) != 0u || (int)(e & kIdxMask) > i) ? 1ull : 0ull;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// out[0] = sum of partial[0..nb); one block.
+__global__ void __launch_bounds__(kBlock) k_sum_partials(const unsigned long long* partial, int nb,
+                                                         unsigned long long* out) {
+  __shared__ unsigned long long red[kBlock/64];
+  unsigned long long c = 0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) c += partial[k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    out[0] = t;
+  }
+}
+
 }  // namespace spx

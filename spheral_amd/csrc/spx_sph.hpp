@@ -64,14 +64,22 @@ __device__ __forceinline__ void load_self(const Fields& f, int i, Nbr<D, R>& s) 
 // Pair quantities of SPH::evaluateDerivativesImpl (SPH/SPH.cc:362-430) and
 // MonaghanGingoldViscosity::QPiij (MonaghanGingoldViscosity.cc:64-100), evaluated
 // from the gathering node s toward its neighbour o (r = r_s - r_o).  The canonical
-// pair (a, b) of the reference has a = s when aFirst, else a = o; every term below
-// equals its canonical value up to an exact sign flip, and the pairwise acceleration
-// dd is summed in the canonical order so both lanes produce the same bits:
-//   dd_canonical = aFirst ? dd : -dd.
+// pair (a, b) of the reference has a = s when aFirst, else a = o.
+//
+// With isotropic H (H = h^-1 I) each side's kernel gradient H etaUnit dW is parallel to r:
+// gradW_s = ks r with ks = (dW_s h_s^-1)(h_s^-1 / |eta_s|), and the Q's tensor is a scalar times
+// the identity, so the pairwise acceleration (SPH.cc:428) is one scalar times r:
+//   pacc_ab / (-m_b) = C r_ab,   C = ((P_a k_a + P_b k_b) + Q_a k_a / 2) + Q_b k_b / 2
+// (P = safeInv(Omega) P / rho^2, Q = the Q's QPi).  C is formed in that canonical order from
+// quantities both lanes compute with the same operations on the same operands (r and the
+// velocity jump change sign exactly), so both lanes hold the same C bits and each lane's
+// C*r_lane is the exact negation of the other's: the pairwise acceleration is bitwise
+// antisymmetric.  C per neighbour entry is all the compatible-energy update needs besides the
+// positions, which it re-forms with the same operations.
 template <int D, typename R>
 struct PairS {
-  R r[D], vso[D], gs[D], go[D], es[D], dd[D];
-  R Ws, Wo, Qs, workQs, etaMagS, wsphS;
+  R r[D], vso[D], gs[D], es[D];
+  R C, Ws, Qs, workQs, vdg, etaMagS, wsphS;
 };
 
 // coef: the kernel table's coefficients in arithmetic R (global, or the block's LDS copy).
@@ -82,14 +90,14 @@ template <int D, typename R, bool LQ = false>
 __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, const Phys& ph, const Nbr<D, R>& S,
                                           const Nbr<D, R>& O, bool aFirst, PairS<D, R>& e,
                                           const R* TS = nullptr, const R* TO = nullptr) {
-  R eo[D], e2s = (R)0, e2o = (R)0;
+  R e2s = (R)0, e2o = (R)0;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
     e.r[c] = (R)(S.x[c] - O.x[c]);
     e.es[c] = S.h*e.r[c];
-    eo[c] = O.h*e.r[c];
+    const R eoc = O.h*e.r[c];
     e2s += e.es[c]*e.es[c];
-    e2o += eo[c]*eo[c];
+    e2o += eoc*eoc;
     e.vso[c] = (R)(S.v[c] - O.v[c]);
   }
   // |eta| and safeInvVar(|eta|) = 1/max(1e-30, |eta|) from one reciprocal square root each
@@ -97,15 +105,19 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
   const R is = e2s > tiny*tiny ? rsqrt(e2s) : (R)1/tiny, io = e2o > tiny*tiny ? rsqrt(e2o) : (R)1/tiny;
   e.etaMagS = e2s*is;
   const R etaMagO = e2o*io;
-  R gws, gwo, graws, grawo;
+  R gws, gwo, graws, grawo, Wo;
   kernel_wgw_c<R>(coef, tab, e.etaMagS, hdet_iso<D, R>(S.h), e.Ws, gws, graws);
-  kernel_wgw_c<R>(coef, tab, etaMagO, hdet_iso<D, R>(O.h), e.Wo, gwo, grawo);
+  kernel_wgw_c<R>(coef, tab, etaMagO, hdet_iso<D, R>(O.h), Wo, gwo, grawo);
   e.wsphS = fabs(graws);                                      // kernelValueSPH(etaMagS)
-  const R ss = gws*S.h, so = gwo*O.h;                        // (gW*H)*etaUnit
-  // velocity jump the Q sees: the plain difference, or the limited midpoint extrapolation
-  R vq[D];
+  const R ks = (gws*S.h)*(S.h*is), ko = (gwo*O.h)*(O.h*io);  // gradW = k r
+  R vsr = (R)0;
 #pragma unroll
-  for (int c = 0; c < D; ++c) vq[c] = e.vso[c];
+  for (int c = 0; c < D; ++c) {
+    e.gs[c] = ks*e.r[c];
+    vsr += e.vso[c]*e.r[c];
+  }
+  // velocity jump the Q sees: the plain difference, or the limited midpoint extrapolation
+  R vqr = vsr;
   if (LQ) {
     // van Leer limiter on the ratio of the gradients' projections along the half separation,
     // phased back toward the full Q when the pair is closer than etaCrit
@@ -128,41 +140,30 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
     const R etaij = fmin(e.etaMagS, etaMagO), ec = (R)ph.etaCrit;
     if (etaij < ec) { const R q = (etaij - ec)/(R)ph.etaFold; phi *= exp(-(q*q)); }
     // (phi*T).xij: the product with the scaled tensor, as the reference forms it
+    vqr = (R)0;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       R s1 = (phi*TS[a*D])*xij[0], s2 = (phi*TO[a*D])*xij[0];
 #pragma unroll
       for (int b = 1; b < D; ++b) { s1 += (phi*TS[a*D + b])*xij[b]; s2 += (phi*TO[a*D + b])*xij[b]; }
-      vq[a] = (R)((S.v[a] - (double)s1) - (O.v[a] + (double)s2));
+      const R vq = (R)((S.v[a] - (double)s1) - (O.v[a] + (double)s2));
+      vqr += vq*e.r[a];
     }
   }
-  R mus = (R)0, muo = (R)0;
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    e.gs[c] = ss*(e.es[c]*is);
-    e.go[c] = so*(eo[c]*io);
-    mus += vq[c]*e.es[c];
-    muo += vq[c]*eo[c];
-  }
-  mus = mus/(e2s + (R)1.0e-2);
-  muo = muo/(e2o + (R)1.0e-2);
+  // mu = (vq . eta)/(eta^2 + eps^2) with eta = h^-1 r on each side
+  const R mus = (S.h*vqr)/(e2s + (R)1.0e-2), muo = (O.h*vqr)/(e2o + (R)1.0e-2);
   const R mms = fmin((R)0, mus), mmo = fmin((R)0, muo);
   const R Cl = (R)ph.Cl, Cq = (R)ph.Cq;
   const R es_ = -Cl*S.cs*mms + Cq*(mms*mms);
   const R eo_ = -Cl*O.cs*mmo + Cq*(mmo*mmo);
   const R QPs = es_*S.irho, QPo = eo_*O.irho;
   e.Qs = S.rho*es_;
-  R wqs = (R)0;
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    const R qs = (R)0.5*(QPs*e.gs[c]), qo = (R)0.5*(QPo*e.go[c]);
-    wqs += e.vso[c]*qs;
-    const R p1 = aFirst ? S.prho : O.prho, p2 = aFirst ? O.prho : S.prho;
-    const R g1 = aFirst ? e.gs[c] : e.go[c], g2 = aFirst ? e.go[c] : e.gs[c];
-    const R q1 = aFirst ? qs : qo, q2 = aFirst ? qo : qs;
-    e.dd[c] = ((p1*g1 + p2*g2) + q1) + q2;
-  }
-  e.workQs = wqs;
+  const R p1 = aFirst ? S.prho : O.prho, p2 = aFirst ? O.prho : S.prho;
+  const R k1 = aFirst ? ks : ko, k2 = aFirst ? ko : ks;
+  const R q1 = aFirst ? QPs : QPo, q2 = aFirst ? QPo : QPs;
+  e.C = ((p1*k1 + p2*k2) + (R)0.5*(q1*k1)) + (R)0.5*(q2*k2);
+  e.vdg = ks*vsr;                                            // v_so . gradW_s
+  e.workQs = (R)0.5*(QPs*e.vdg);                             // v_so . (QPi_s gradW_s / 2)
 }
 
 // ---------------------------------------------------------------------------
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
   const R* coef = stage_table<R>(tab, ldsTable);
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
+  const size_t ld = (size_t)f.ld;
   R* pdd = (R*)f.pdd;
   Nbr<D, R> S;
   load_self<D, R>(f, i, S);
@@ -250,14 +251,11 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
     } else {
       pair_eval<D, R>(tab, coef, ph, S, O, code != 0 || i < j, e);
     }
-#pragma unroll
-    for (int c = 0; c < D; ++c) pdd[((size_t)c*cap + k)*ld + i] = e.dd[c];
+    pdd[(size_t)k*ld + i] = e.C;
     const double mo = rc.m;
-    R vdg = (R)0;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      dvdt[c] -= mo*(double)e.dd[c];
-      vdg += e.vso[c]*e.gs[c];
+      dvdt[c] -= mo*(double)(e.C*e.r[c]);
       const double mv = mo*(double)e.vso[c], mr = mo*(double)e.r[c];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -268,7 +266,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
         if (!kSymM<R> || d >= c) Mc[c*D + d] -= mr*(double)e.gs[d];
       }
     }
-    depsdt += mo*(double)(S.prho*vdg + e.workQs);
+    depsdt += mo*(double)(S.prho*e.vdg + e.workQs);
     maxq = fmax(maxq, (double)e.Qs);
     const R wsph = e.wsphS;
     m0 += (double)wsph;
@@ -339,15 +337,23 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 #pragma unroll
   for (int a = 0; a < D; ++a) { f.DvDt[a][i] = dvdt[a]; f.DxDt[a][i] = f.vm[a][i]; }
   f.DepsDt[i] = depsdt;
-  if (sc) {
-    const double hdt = 0.5*sc[f.run[i]].dt;
-    double w[3] = {0.0, 0.0, 0.0};
+  {
+    // the partition record: this evaluation's positions (the pair coefficients' r) always; the
+    // half-step velocities and DepsDt in the step (otherwise k_v12 writes them)
+    double2* PV = reinterpret_cast<double2*>(f.pkv) + 4*(size_t)i;
+    double xs[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < D; ++a) w[a] = f.v[a][i] + dvdt[a]*hdt;
-    double2* PV = reinterpret_cast<double2*>(f.pkv) + 3*(size_t)i;
-    PV[0] = make_double2(w[0], w[1]);
-    PV[1] = make_double2(w[2], depsdt);
-    PV[2] = make_double2(Sm, 0.0);
+    for (int a = 0; a < D; ++a) xs[a] = S.x[a];
+    if (sc) {
+      const double hdt = 0.5*sc[f.run[i]].dt;
+      double w[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < D; ++a) w[a] = f.v[a][i] + dvdt[a]*hdt;
+      PV[0] = make_double2(w[0], w[1]);
+      PV[1] = make_double2(w[2], depsdt);
+    }
+    PV[2] = make_double2(Sm, xs[0]);
+    PV[3] = make_double2(xs[1], xs[2]);
   }
   f.DrhoDt[i] = -Srho*tr;
   f.maxQ[i] = maxq;
@@ -375,21 +381,26 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 // ---------------------------------------------------------------------------
 // SpecificThermalEnergyPolicy::update's partition of the pair work (Hydro/
 // SpecificThermalEnergyPolicy.cc:79-144): node i's specific thermal energy rate from the pair
-// accelerations the SPH pass stored per ELL entry (each already in canonical summation order)
-// and the half-step velocity records (Fields::pkv).
+// coefficients C the SPH pass stored per ELL entry and the partition records (Fields::pkv: the
+// half-step velocity, DepsDt, m and the evaluation's position).  The pairwise acceleration is
+// re-formed as C*r with r computed from the evaluation's positions by the SPH pass's own
+// operations, so it is the same bits the SPH pass summed into DvDt.
 // ---------------------------------------------------------------------------
 template <int D, typename R, int G>
 __device__ __forceinline__ double compatible_rate(const Fields& f, const Geo& g, int i) {
-  const size_t ld = (size_t)f.ld, cap = (size_t)f.cap;
-  const R* pdd = (const R*)f.pdd;       // pair terms in the pair arithmetic; the partition is FP64
+  const size_t ld = (size_t)f.ld;
+  const R* pdd = (const R*)f.pdd;       // pair coefficients in the pair arithmetic; the partition is FP64
   const double ms = f.m[i];
-  double v12s[D];
+  double v12s[D], xs[D];
   const double2* PV = reinterpret_cast<const double2*>(f.pkv);
   {
-    const double2 a = PV[3*(size_t)i], b = PV[3*(size_t)i + 1];
+    const double2 a = PV[4*(size_t)i], b = PV[4*(size_t)i + 1], c = PV[4*(size_t)i + 2], d = PV[4*(size_t)i + 3];
     v12s[0] = a.x;
     if (D >= 2) v12s[1 % D] = a.y;
     if (D >= 3) v12s[2 % D] = b.x;
+    xs[0] = c.y;
+    if (D >= 2) xs[1 % D] = d.x;
+    if (D >= 3) xs[2 % D] = d.y;
   }
   const double de0s = f.DepsDt[i];
   const double epsm = 2.220446049250313e-16;
@@ -398,32 +409,37 @@ __device__ __forceinline__ double compatible_rate(const Fields& f, const Geo& g,
   for (int k0 = 0; k0 < nn; k0 += G) {
     int js[G], codes[G];
     ell_group<G>(f.nbr, ld, k0, i, nn, js, codes);
-    double vo[G][D], mo[G], deo[G], dd[G][D];
+    double vo[G][D], xo[G][D], mo[G], deo[G];
+    R C[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       const int j = js[u];
-      const double2 a = PV[3*(size_t)j], b = PV[3*(size_t)j + 1], c = PV[3*(size_t)j + 2];
+      const double2 a = PV[4*(size_t)j], b = PV[4*(size_t)j + 1], c = PV[4*(size_t)j + 2], d = PV[4*(size_t)j + 3];
       vo[u][0] = a.x;
       if (D >= 2) vo[u][1 % D] = a.y;
       if (D >= 3) vo[u][2 % D] = b.x;
       deo[u] = b.y;
       mo[u] = c.x;
-#pragma unroll
-      for (int q = 0; q < D; ++q) dd[u][q] = (double)pdd[((size_t)q*cap + k0 + u)*ld + i];
+      xo[u][0] = c.y;
+      if (D >= 2) xo[u][1 % D] = d.x;
+      if (D >= 3) xo[u][2 % D] = d.y;
+      C[u] = pdd[(size_t)(k0 + u)*ld + i];
     }
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (k0 + u >= nn) continue;
       const int j = js[u], code = codes[u];
       const bool aFirst = code != 0 || i < j;
-      // canonical: pacc_ab = (-m_b) dd_ab ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
+      if (code) image_pos<D>(g, code, xo[u]);
+      // canonical: pacc_ab = (-m_b) C r_ab ; du_ab = (v_b^{1/2} - v_a^{1/2}) . pacc_ab
       const double mb = aFirst ? mo[u] : ms, ma = aFirst ? ms : mo[u];
       double duab = 0.0;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
+        const double dd = (double)(C[u]*(R)(xs[c] - xo[u][c]));     // the SPH pass's C*r, this lane's side
         const double v12o = refl_sign<D>(g, code, c)*vo[u][c];
         const double va = aFirst ? v12s[c] : v12o, vb = aFirst ? v12o : v12s[c];
-        const double ddc = aFirst ? dd[u][c] : -dd[u][c];
+        const double ddc = aFirst ? dd : -dd;
         duab += (vb - va)*((-mb)*ddc);
       }
       const double sgn = duab < 0.0 ? -1.0 : 1.0;
@@ -543,10 +559,9 @@ __global__ void __launch_bounds__(kBlock) k_v12(Fields f, const Scalars* sc, dou
   double w[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < D; ++a) w[a] = f.v[a][i] + f.DvDt[a][i]*hdt;
-  double2* P = reinterpret_cast<double2*>(f.pkv) + 3*(size_t)i;
+  double2* P = reinterpret_cast<double2*>(f.pkv) + 4*(size_t)i;
   P[0] = make_double2(w[0], w[1]);
   P[1] = make_double2(w[2], f.DepsDt[i]);
-  P[2] = make_double2(f.m[i], 0.0);
 }
 
 

@@ -411,6 +411,14 @@ class TreeGravity(_Package):
     def label(self):
         return "TreeGravity"
 
+    def time_step_code(self):
+        """GravityTimeStepType (Gravity/TreeGravity.hh) as the engine's SPX_GRAVITY_DT_* code."""
+        if self.timeStepChoice in (None, "AccelerationRatio"):
+            return abi.SPX_GRAVITY_DT_ACCELERATION_RATIO
+        if self.timeStepChoice == "DynamicalTime":
+            return abi.SPX_GRAVITY_DT_DYNAMICAL_TIME
+        raise NotImplementedError(f"TreeGravity timeStepChoice {self.timeStepChoice!r}")
+
 
 class CheapSynchronousRK2Integrator:
     """Integrator/CheapSynchronousRK2 driving the packages on one engine run."""
@@ -520,14 +528,13 @@ class CheapSynchronousRK2Integrator:
             g = grav[0]
             cfg.gravity = 1
             cfg.G, cfg.softening, cfg.opening, cfg.ftimestep = g.G, g.softeningLength, g.opening, g.ftimestep
+            cfg.gravity_time_step = g.time_step_code()
         return cfg
 
     def _nbody_config(self, g):
         """TreeGravity as the only package (an N-body script): SynchronousRK2, no SPH."""
         if self.code != abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2:
             raise NotImplementedError("TreeGravity alone is implemented under SynchronousRK2Integrator")
-        if g.timeStepChoice not in (None, "AccelerationRatio"):
-            raise NotImplementedError("only the AccelerationRatio time step choice is implemented")
         db = self.dataBase
         n0 = db.nodeLists[0]
         cfg = abi.default_config(db.nDim, nperh=n0.nPerh)
@@ -537,6 +544,7 @@ class CheapSynchronousRK2Integrator:
         cfg.gamma = n0.eos.gamma
         cfg.gravity, cfg.gravity_only, cfg.integrator = 1, 1, abi.SPX_INTEGRATOR_SYNCHRONOUS_RK2
         cfg.G, cfg.softening, cfg.opening, cfg.ftimestep = g.G, g.softeningLength, g.opening, g.ftimestep
+        cfg.gravity_time_step = g.time_step_code()
         cfg.dt_min, cfg.dt_max, cfg.dt_growth, cfg.last_dt = self.dtMin, self.dtMax, self.dtGrowth, self.lastDt
         return cfg
 

@@ -9,6 +9,7 @@ parent kept and numbered before it, and the cull leaving no ghost that nothing u
 import io
 
 import numpy as np
+import pytest
 
 from spheral_amd import abi, connectivity, problems
 
@@ -144,3 +145,64 @@ def test_two_nodelists_and_writer(oracle_mod):
     lines = f.getvalue().splitlines()
     assert lines[0] == f"nodes1/numInternalNodes {nl}"
     assert any(line.startswith(f"nodes1/{i}/nodes2 ") for line in lines)
+
+
+def test_node_pair_hash_known_values():
+    """NodePairIdxType::hash (NodePairIdxType.hh:34-53): flipped to (lower, higher), then
+    (i_list << 59) + (i_node << 32) + (j_list << 27) + j_node."""
+    h = connectivity.node_pair_hash
+    assert int(h(3, 0, 5, 0)) == (3 << 32) + 5
+    assert int(h(5, 0, 3, 0)) == (3 << 32) + 5                       # flipped: same pair, same hash
+    assert int(h(2, 1, 7, 0)) == (7 << 32) + (1 << 27) + 2           # list 0 first
+    assert int(h(4, 2, 9, 3)) == (2 << 59) + (4 << 32) + (3 << 27) + 9
+    assert int(h((1 << 27) - 1, 31, 0, 0)) == ((1 << 27) - 1) + (31 << 27)
+    with pytest.raises(ValueError):
+        h(1 << 27, 0, 0, 0)
+
+
+def test_node_pair_list_equals_brute_force_pairs(oracle_mod):
+    """Pairs from the brute-force lists under calculatePairInteraction (one NodeList: j > i,
+    ghosts included), each once, ascending in hash; as many as the oracle's own pair list."""
+    cfg, st = problems.uniform_lattice(2, 12, kernel=abi.SPX_KERNEL_BSPLINE, nperh=1.51)
+    rng = np.random.default_rng(5)
+    st["x"] = (st["x"] + rng.uniform(-0.02, 0.02, st["x"].shape)) % 1.0
+    o, cm = _dump(oracle_mod, cfg, st)
+    x, H = o.field("position"), o.field("H")
+    ref = _brute(cfg, x, H, cm)
+    want = sorted((i, int(j)) for i in range(len(x)) for j in ref[i] if j > i)
+    pl = cm.nodePairList()
+    got = list(zip(pl["i_node"].tolist(), pl["j_node"].tolist()))
+    assert got == want                     # (i, j) with i < j: hash order is (i, j) order
+    assert np.all(pl["i_list"] == 0) and np.all(pl["j_list"] == 0)
+    assert len(pl) == o.num_pairs()
+    hs = connectivity.node_pair_hash(pl["i_node"], pl["i_list"], pl["j_node"], pl["j_list"])
+    assert np.all(np.diff(hs.astype(np.float64)) > 0) and len(np.unique(hs)) == len(hs)
+
+
+def test_node_pair_list_two_nodelists(oracle_mod):
+    """Across NodeLists the lower list's internal node owns the pair with every node of the higher
+    list; the higher list's node owns a pair with the lower list's ghosts only."""
+    cfg, st = problems.sod_1d(100)
+    nl = int(np.sum(st["x"][:, 0] < 0.0))
+    o, cm = _dump(oracle_mod, cfg, st, nodelists=[("nodes1", 0, nl), ("nodes2", nl, 100 - nl)])
+    pl = cm.nodePairList()
+    sizes = cm.sizes
+    for p in pl:
+        i, k, j, l = (int(p[f]) for f in ("i_node", "i_list", "j_node", "j_list"))
+        assert i < sizes[k]
+        assert l > k or (l == k and j > i) or (l < k and j >= sizes[l])
+    cross = pl[(pl["i_list"] == 0) & (pl["j_list"] == 1)]
+    assert len(cross) > 0
+    # every internal-internal neighbour relation appears exactly once over the two orderings
+    links = set()
+    for k in range(2):
+        for i in range(sizes[k]):
+            for l in range(2):
+                for j in cm.connectivityForNode(k, i)[l]:
+                    if j < sizes[l]:
+                        links.add(frozenset([(k, i), (l, int(j))]))
+    internal = [frozenset([(int(p["i_list"]), int(p["i_node"])), (int(p["j_list"]), int(p["j_node"]))])
+                for p in pl if p["j_node"] < sizes[int(p["j_list"])]]
+    assert len(internal) == len(set(internal)) == len(links)
+    hs = connectivity.node_pair_hash(pl["i_node"], pl["i_list"], pl["j_node"], pl["j_list"])
+    assert np.all(hs[1:] > hs[:-1])

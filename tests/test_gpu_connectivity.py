@@ -44,7 +44,12 @@ def test_engine_dump_equals_oracle_dump(gpu, oracle_mod, name):
     g.initialize()
     off, nbr, img = g.neighbors()
     cm = connectivity.connectivity_map(cfg, g.field("position"), g.field("H"), off, nbr, img, nls)
-    _same(cm, _oracle_dump(oracle_mod, cfg, st, nls))
+    co = _oracle_dump(oracle_mod, cfg, st, nls)
+    _same(cm, co)
+    # ConnectivityMap::nodePairList in hash order: the same pairs, as many as the engine counts
+    pl, po = cm.nodePairList(), co.nodePairList()
+    assert np.array_equal(pl, po)
+    assert len(pl) == g.num_pairs()
 
 
 def test_script_connectivity_map_after_steps(gpu, oracle_mod):

@@ -78,6 +78,29 @@ def test_collapse_potential_fit_matches_the_reference(oracle_mod):
     assert abs(sig - SIGPHI_REF) / SIGPHI_REF < 5e-5
 
 
+# CollisionlessSphereCollapse.py:74-76 (DynamicalTime), checked with the same tol 5e-5
+COEFS_REF_DYN = np.array([8.29009280e+00, 1.20192051e-12, -2.60731358e-23])
+SIGPHI_REF_DYN = 8.968145544554178
+
+
+def test_collapse_dynamical_time_matches_the_reference(oracle_mod):
+    """The reference's second collapse check (ATS --timeStepChoice DynamicalTime --steps=40): the
+    dt vote ftimestep/sqrt(G rhoMax) from the interaction buckets' local effective densities
+    (TreeGravity.cc:239-300, 498-518) drives the 40 steps."""
+    cfg, st, _ = problems.collisionless_sphere_3d()
+    cfg.gravity_time_step = abi.SPX_GRAVITY_DT_DYNAMICAL_TIME
+    o = oracle_mod.OracleRun(cfg, len(st["m"]))
+    o.set_state(**st)
+    o.initialize()
+    dts = [o.step(1.0e40) for _ in range(40)]
+    coefs, sig = collapse_fit(o)
+    print("[collapse dynamical] coefs", coefs, "sigphi", sig)
+    assert np.abs(coefs - COEFS_REF_DYN).max() < 5e-5
+    assert abs(sig - SIGPHI_REF_DYN) / SIGPHI_REF_DYN < 5e-5
+    # the vote differs from AccelerationRatio's: the first steps double from lastDt up to it
+    assert dts[1] == 2.0*dts[0] and dts[3] < dts[2]
+
+
 def test_potential_field_is_the_states(oracle_mod):
     """After a SynchronousRK2 step the potential field equals a standalone evaluation at the
     start-of-step state; under CheapSynchronousRK2 (one midpoint evaluation per step, restored by
