@@ -10,7 +10,6 @@
 #include <cmath>
 #include <limits>
 #include <memory>
-#include <mutex>
 #include <string>
 
 #include "../../include/spx_abi.h"
@@ -26,30 +25,6 @@ __global__ void __launch_bounds__(kBlock) k_copy(const double2* __restrict__ src
   for (int64_t i = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-// Device-resident scratch of spx_run_num_pairs_noalloc: per-block partial counts plus the total.
-// Each GPU the library runs on holds one copy (512 KiB) until the process ends.  Host code never
-// passes `scratch` itself to a kernel: it passes the address hipGetSymbolAddress returns for the
-// current device (noalloc_scratch).
-constexpr int kNoAllocBlocks = 65536;            // up to 65536*256 = 16,777,216 particles
-__device__ __align__(256) unsigned long long scratch[kNoAllocBlocks + 1];
-
-constexpr int kMaxDevices = 64;
-static std::mutex g_scratch_mutex;
-static unsigned long long* g_scratch_ptr[kMaxDevices] = {};
-
-// The device address of `scratch` on `device`, looked up once per device with that device current.
-static unsigned long long* noalloc_scratch(int device) {
-  if (device < 0 || device >= kMaxDevices) raise(SPX_ERR_ARG, "device index out of range");
-  std::lock_guard<std::mutex> lock(g_scratch_mutex);
-  if (!g_scratch_ptr[device]) {
-    HIP_CHECK(hipSetDevice(device));
-    void* p = nullptr;
-    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
-    if (!p) raise(SPX_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
-    g_scratch_ptr[device] = static_cast<unsigned long long*>(p);
-  }
-  return g_scratch_ptr[device];
-}
 }  // namespace spx
 
 struct spx_run {
@@ -424,21 +399,6 @@ int spx_run_num_pairs(spx_run* run, int64_t* npairs) {
     if (!npairs) spx::raise(SPX_ERR_ARG, "null argument");
     RUN_STATE(run);
     *npairs = run->impl->numPairs();
-  });
-}
-
-int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs) {
-  RUN_OR_FAIL(run);
-  return guarded([&] {
-    if (!npairs) spx::raise(SPX_ERR_ARG, "null argument");
-    RUN_STATE(run);
-    auto& B = *run->impl;
-    const int64_t nb = (B.nActive + spx::kBlock - 1)/spx::kBlock;
-    if (nb > spx::kNoAllocBlocks)
-      spx::raise(SPX_ERR_CAPACITY, "spx_run_num_pairs_noalloc: more than 16,777,216 active particles "
-                                   "(the scratch array's limit); use spx_run_num_pairs");
-    unsigned long long* p = spx::noalloc_scratch(B.device);
-    *npairs = B.numPairs(p);
   });
 }
 
