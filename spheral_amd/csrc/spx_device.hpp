@@ -289,36 +289,52 @@ __device__ __forceinline__ void kernel_wgw(const DevTable& t, R eta, R hdet, R& 
   }
 }
 
-// The same lookup on a given copy of the coefficients (global or staged in LDS); graw is the
-// gradient polynomial before the Hdet factor, so |graw| is kernelValueSPH at the same eta.
+// A pointer into the block's LDS, typed as such (address space 3) so that loads through it are
+// LDS reads wherever the optimiser moves them.
 template <typename R>
-__device__ __forceinline__ void kernel_wgw_c(const R* coef, const DevTable& t, R eta, R hdet, R& w, R& gw, R& graw) {
-  if (eta < (R)t.extent) {
-    int b = (int)(eta*(R)t.inv_xstep);
-    b = b < t.n1 ? b : t.n1;
-    const R* c = coef + 6*b;
-    w = hdet*(c[0] + (c[1] + c[2]*eta)*eta);
-    graw = c[3] + (c[4] + c[5]*eta)*eta;
-    gw = hdet*graw;
-  } else {
-    w = (R)0;
-    gw = (R)0;
-    graw = (R)0;
-  }
+using lds_ptr = const R __attribute__((address_space(3)))*;
+
+// The same lookup on a given copy of the coefficients (a global pointer, or an lds_ptr to the
+// block's staged copy); graw is the gradient polynomial before the Hdet factor, so |graw| is
+// kernelValueSPH at the same eta.  Branch-free: the interval index is clamped to the last one (in
+// bounds for any eta >= 0, the float-to-int conversion saturates), and the values beyond the
+// extent are selected to zero.
+template <typename R, typename CP>
+__device__ __forceinline__ void kernel_wgw_c(CP coef, const DevTable& t, R eta, R hdet, R& w, R& gw, R& graw) {
+  int b = (int)(eta*(R)t.inv_xstep);
+  b = b < t.n1 ? b : t.n1;
+  const CP c = coef + 6*b;
+  const bool in = eta < (R)t.extent;
+  const R wv = hdet*(c[0] + (c[1] + c[2]*eta)*eta);
+  const R gr = c[3] + (c[4] + c[5]*eta)*eta;
+  w = in ? wv : (R)0;
+  graw = in ? gr : (R)0;
+  gw = in ? hdet*gr : (R)0;
 }
 
-// The kernel table's coefficients for a pair pass: staged in the block's dynamic LDS when
-// lds != 0 (every thread of the block must call this before any returns), else global.
+// The kernel table's coefficients for a pair pass, staged in the block's dynamic LDS (every
+// thread of the block must call this before any returns).  Pair passes call their node body
+// through with_table: once with this LDS pointer and once with the global table, never with a
+// generic pointer that may be either -- a load through such a pointer is a flat load, and waiting
+// for its result also waits for every global load issued before it (e.g. the next neighbour's
+// prefetched record).
 template <typename R>
-__device__ __forceinline__ const R* stage_table(const DevTable& tab, int lds) {
+__device__ __forceinline__ lds_ptr<R> stage_table_lds(const DevTable& tab) {
   extern __shared__ __align__(16) unsigned char spx_lds[];
   const R* coef = table_coef<R>(tab);
-  if (!lds) return coef;
   R* t = reinterpret_cast<R*>(spx_lds);
   const int nc = 6*(tab.n1 + 1);
   for (int k = threadIdx.x; k < nc; k += blockDim.x) t[k] = coef[k];
   __syncthreads();
-  return t;
+  return (lds_ptr<R>)t;
+}
+
+// body(coef) with the table staged in LDS when lds != 0 (a uniform branch), else in global
+// memory; body is a generic lambda, instantiated once per pointer type.
+template <typename R, typename F>
+__device__ __forceinline__ void with_table(const DevTable& tab, int lds, F&& body) {
+  if (lds) body(stage_table_lds<R>(tab));
+  else body(table_coef<R>(tab));
 }
 
 // kernelValueSPH: |dW/deta| at Hdet = 1 (Kernel/TableKernelViewInline.hh:112-120)

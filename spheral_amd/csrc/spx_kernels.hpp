@@ -512,10 +512,9 @@ __global__ void __launch_bounds__(kBlock) k_remap_candidates(const uint32_t* src
 // k_neighbor_filter to each candidate and writes the ELL neighbour list as it goes, so the
 // connectivity filter and the first pair pass of the step read the neighbours once.
 // ---------------------------------------------------------------------------
-template <int D, bool DENS, bool DT, typename R, int G, bool FILT>
-__device__ __forceinline__ void density_dt_node(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc,
-                                                int* overflow, int ldsTable) {
-  const R* coef = stage_table<R>(tab, DENS ? ldsTable : 0);
+template <int D, bool DENS, bool DT, typename R, int G, bool FILT, typename CP>
+__device__ __forceinline__ void density_dt_node(const Fields& f, const Geo& g, const Phys& ph, const DevTable& tab,
+                                                CP coef, int n, Scalars* sc, int* overflow) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   double dtmin = 1.7976931348623157e308;
   const bool valid = i < n;
@@ -659,7 +658,9 @@ __device__ __forceinline__ void density_dt_node(Fields f, Geo g, Phys ph, DevTab
 template <int D, bool DENS, bool DT, typename R, int G, bool FILT = false>
 __global__ void __launch_bounds__(kBlock) k_density_dt(Fields f, Geo g, Phys ph, DevTable tab, int n, Scalars* sc,
                                                        int* overflow, int ldsTable) {
-  density_dt_node<D, DENS, DT, R, G, FILT>(f, g, ph, tab, n, sc, overflow, ldsTable);
+  with_table<R>(tab, DENS ? ldsTable : 0, [&](auto coef) {
+    density_dt_node<D, DENS, DT, R, G, FILT>(f, g, ph, tab, coef, n, sc, overflow);
+  });
 }
 
 // Integrator::selectDt (Integrator/Integrator.cc:116-169), one thread per run.
@@ -778,10 +779,9 @@ __global__ void __launch_bounds__(kBlock) k_eos(Fields f, Phys ph, int n) {
 // also writes node i's midpoint record for the SPH pass (write_mid_record).  runMask
 // (optional): only nodes of runs with a non-zero entry are updated.
 // ---------------------------------------------------------------------------
-template <int D, typename R, bool MID, int G = kGroup>
-__global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, const double* xh,
-                                                  double* omega, const int* runMask, int ldsTable) {
-  const R* coef = stage_table<R>(tab, ldsTable);
+template <int D, typename R, bool MID, int G, typename CP>
+__device__ __forceinline__ void omega_node(const Fields& f, const Geo& g, const DevTable& tab, CP coef, int n,
+                                           const double* xh, double* omega, const int* runMask) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (runMask && !runMask[f.run[i]]) return;
@@ -821,6 +821,11 @@ __global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab,
   }
   omega[i] = om;
   if (MID) write_mid_record<D>(f, i, om);
+}
+template <int D, typename R, bool MID, int G = kGroup>
+__global__ void __launch_bounds__(kBlock) k_omega(Fields f, Geo g, DevTable tab, int n, const double* xh,
+                                                  double* omega, const int* runMask, int ldsTable) {
+  with_table<R>(tab, ldsTable, [&](auto coef) { omega_node<D, R, MID, G>(f, g, tab, coef, n, xh, omega, runMask); });
 }
 
 // SynchronousRK2 with TreeGravity as the only package (Integrator/SynchronousRK2.cc:40-135): the

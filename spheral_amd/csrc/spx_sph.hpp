@@ -86,8 +86,8 @@ struct PairS {
 // LQ: LimitedMonaghanGingoldViscosity::QPiij (LimitedMonaghanGingoldViscosity.cc:160-195) in
 // place of the plain Q; TS, TO are the two nodes' velocity gradients as the Q holds them
 // (row-major D x D, the partner's already image-transformed).
-template <int D, typename R, bool LQ = false>
-__device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, const Phys& ph, const Nbr<D, R>& S,
+template <int D, typename R, bool LQ = false, typename CP>
+__device__ __forceinline__ void pair_eval(const DevTable& tab, CP coef, const Phys& ph, const Nbr<D, R>& S,
                                           const Nbr<D, R>& O, bool aFirst, PairS<D, R>& e,
                                           const R* TS = nullptr, const R* TO = nullptr) {
   R e2s = (R)0, e2o = (R)0;
@@ -179,10 +179,9 @@ __device__ __forceinline__ void pair_eval(const DevTable& tab, const R* coef, co
 template <typename R>
 constexpr bool kSymM = sizeof(R) == sizeof(double);
 
-template <int D, bool FULL, typename R, bool LQ = false>
-__global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable,
-                                                   const Scalars* sc) {
-  const R* coef = stage_table<R>(tab, ldsTable);
+template <int D, bool FULL, typename R, bool LQ, typename CP>
+__device__ __forceinline__ void sph_node(const Fields& f, const Geo& g, const Phys& ph, const DevTable& tab,
+                                         CP coef, int n, const Scalars* sc) {
   const int i = xcd_block()*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t ld = (size_t)f.ld;
@@ -376,6 +375,12 @@ __global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, Dev
 #pragma unroll
     for (int a = 0; a < D*D; ++a) f.Mcorr[a*ld + i] = Mc[a];
   }
+}
+
+template <int D, bool FULL, typename R, bool LQ = false>
+__global__ void __launch_bounds__(kBlock, 3) k_sph(Fields f, Geo g, Phys ph, DevTable tab, int n, int ldsTable,
+                                                   const Scalars* sc) {
+  with_table<R>(tab, ldsTable, [&](auto coef) { sph_node<D, FULL, R, LQ>(f, g, ph, tab, coef, n, sc); });
 }
 
 // ---------------------------------------------------------------------------

@@ -99,7 +99,8 @@ def test_host_driven_hooks_equal_step_and_oracle(gpu, oracle_mod, name, cfg, st,
     o.initialize()
     S = {k: np.array(v, dtype=np.float64, copy=True) for k, v in st.items()}
     S["H"] = h.field("H")
-    D = {f: np.zeros_like(h.field(f)) for f in DERIVS}
+    # the derivatives before the first evaluation: zero (the ideal H exists only after a step)
+    D = {f: np.zeros_like(h.field(f)) for f in DERIVS if f != "new H"}
     D["new H"] = S["H"].copy()
     last_dt = cfg.last_dt
     for k in range(steps):

@@ -3,7 +3,8 @@
 * The served problem kinds (2-D and 3-D Sod tubes with reflecting walls, the periodic Gresho
   vortex; tests/functional/Hydro/Sod, tests/functional/Hydro/GreshoVortex/GreshoVortex.py), each
   as a plain FP64 run: every step's dt within 1e-10 of the oracle's, the state after 8 steps
-  within 1e-9 of the state's scale.
+  within 1e-9 of the state's scale or twice the problem's one-ulp roundoff floor, whichever is
+  larger (the floor decides only the vortex's specific energy: see roundoff_floor).
 * A batch of 8 Kelvin-Helmholtz runs (tests/functional/Hydro/KelvinHelmholtz/
   KelvinHelmholtz-2d.py) of 4k-16k particles: in FP64 every run tracks its oracle run at the
   same bars; in mixed precision the density and specific-energy L1 against the FP64 oracle stay
@@ -29,12 +30,30 @@ def oracle_run(oracle_mod, cfg, st):
     return o
 
 
-def assert_state_close(get_g, o, tol=1e-9):
+def assert_state_close(get_g, o, tol=1e-9, floor=None):
+    """Every field within tol of its scale, or within twice the problem's own roundoff floor
+    (floor[f]: how far the oracle moves when the initial positions move by one ulp) where that
+    is larger."""
     cs = o.field("sound speed").max()
     for f in FIELDS:
         a, b = get_g(f), o.field(f)
         sc = max(np.abs(b).max(), cs if f == "velocity" else 0.0)
-        assert np.all(np.abs(a - b) <= tol*sc), f
+        bar = max(tol*sc, 2.0*floor[f]) if floor else tol*sc
+        assert np.all(np.abs(a - b) <= bar), (f, float(np.abs(a - b).max()), bar)
+
+
+def roundoff_floor(oracle_mod, cfg, st, steps):
+    """Largest change of each field after `steps` oracle steps when every initial position moves
+    by one ulp.  The compatible-energy partition weighs a pair by max(eps, DepsDt sgn du): where
+    DepsDt is zero up to roundoff (the Gresho vortex's rigidly rotating core, div v = 0, on the
+    first step) the weights, and so the specific energies, follow the roundoff of DepsDt.  Any two
+    summation orders -- the engine's and the oracle's, or the oracle on a one-ulp move -- differ
+    there by the same amount (3e-8 of an energy of 8.7 on the 4000-node vortex)."""
+    o, p = oracle_run(oracle_mod, cfg, st), oracle_run(oracle_mod, cfg, dict(st, x=np.nextafter(st["x"], np.inf)))
+    for _ in range(steps):
+        o.step()
+        p.step()
+    return {f: float(np.abs(o.field(f) - p.field(f)).max()) for f in FIELDS}
 
 
 SERVED = [
@@ -54,7 +73,7 @@ def test_served_problem_kinds_track_oracle(gpu, oracle_mod, name, make):
     for k in range(8):
         dg, do = g.step(), o.step()
         assert abs(dg - do) <= 1e-10*do, k
-    assert_state_close(g.field, o)
+    assert_state_close(g.field, o, floor=roundoff_floor(oracle_mod, cfg, st, 8))
     cg, co = g.conservation(), o.conservation()
     assert abs((cg[4] + cg[5]) - (co[4] + co[5])) <= 1e-12*abs(co[4] + co[5])
 
