@@ -273,6 +273,13 @@ int spx_run_kernel_times(spx_run* run, double* ms, int32_t max_entries,
    (Neighbor/ConnectivityMap.hh:79; pairs admitted by calculatePairInteraction,
    ConnectivityMapInline.hh:296-309), counted on the device into a library-allocated buffer. */
 int spx_run_num_pairs(spx_run* run, int64_t* npairs);
+/* spx_run_num_pairs without any device allocation.  No counterpart in the reference: for callers
+   that cannot allocate device memory (code under stream/graph capture, serving loops that forbid
+   hipMalloc).  Counts into a 512 KiB device array that the library holds per GPU for the life of
+   the process; runs of more than 16,777,216 active particles get SPX_ERR_CAPACITY.  Calls that
+   overlap on one GPU share that array and race; the array keeps the last call's counts.  Calls
+   no hipMalloc/hipFree.  Requires the run's connectivity (SPX_ERR_STATE otherwise). */
+int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs);
 /* Candidate-list builds of the run so far (no counterpart in the reference, which rebuilds
    its ConnectivityMap from scratch every step: a measurement aid).  The neighbour sets are
    the reference's every step either way; a build happens at start-up, after

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "../../include/spx_abi.h"
@@ -23,6 +24,37 @@ static thread_local std::string g_last_error;
 __global__ void __launch_bounds__(kBlock) k_copy(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x*blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x*blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Device-resident scratch of spx_run_num_pairs_noalloc: per-block partial counts plus the total.
+// Each GPU the library runs on holds one copy (512 KiB) until the process ends.  Host code never
+// passes `scratch` itself to a kernel: it passes the address hipGetSymbolAddress returns for the
+// current device (noalloc_scratch).  Every partial a call reads is written by the same call, so
+// nothing needs clearing between calls.
+constexpr int kNoAllocBlocks = 65536;            // up to 65536*256 = 16,777,216 particles
+__device__ __align__(256) unsigned long long scratch[kNoAllocBlocks + 1];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with that device
+// current (the caller's current device is restored afterwards).
+static unsigned long long* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) raise(SPX_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    int prev = 0;
+    HIP_CHECK(hipGetDevice(&prev));
+    HIP_CHECK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    HIP_CHECK(hipSetDevice(prev));
+    HIP_CHECK(e);
+    if (!p) raise(SPX_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = static_cast<unsigned long long*>(p);
+  }
+  return g_scratch_ptr[device];
 }
 
 }  // namespace spx
@@ -399,6 +431,21 @@ int spx_run_num_pairs(spx_run* run, int64_t* npairs) {
     if (!npairs) spx::raise(SPX_ERR_ARG, "null argument");
     RUN_STATE(run);
     *npairs = run->impl->numPairs();
+  });
+}
+
+int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    if (!npairs) spx::raise(SPX_ERR_ARG, "null argument");
+    RUN_STATE(run);
+    auto& B = *run->impl;
+    const int64_t nb = (B.nActive + spx::kBlock - 1)/spx::kBlock;
+    if (nb > spx::kNoAllocBlocks)
+      spx::raise(SPX_ERR_CAPACITY, "spx_run_num_pairs_noalloc: more than 16,777,216 active particles "
+                                   "(the scratch array's limit); use spx_run_num_pairs");
+    unsigned long long* p = spx::noalloc_scratch(B.device);
+    *npairs = B.numPairs(p);
   });
 }
 

@@ -22,7 +22,7 @@ EXPORTS = (
     "spx_run_step", "spx_run_advance", "spx_run_time", "spx_run_update_connectivity", "spx_run_neighbor_counts",
     "spx_run_neighbors", "spx_run_sum_density", "spx_run_update_eos", "spx_run_post_state_update",
     "spx_run_evaluate_derivatives", "spx_run_dt", "spx_run_compatible_energy", "spx_run_conservation",
-    "spx_run_kernel_times", "spx_run_num_pairs", "spx_run_neighbor_builds",
+    "spx_run_kernel_times", "spx_run_num_pairs", "spx_run_num_pairs_noalloc", "spx_run_neighbor_builds",
     "spx_batch_create", "spx_batch_destroy", "spx_batch_add_run", "spx_batch_remove_run", "spx_batch_step",
     "spx_batch_run_status", "spx_batch_get_field", "spx_batch_conservation", "spx_batch_usage",
     "spx_batch_kernel_times", "spx_batch_neighbor_builds", "spx_bandwidth_probe",
@@ -79,6 +79,7 @@ def lib():
             "spx_run_conservation": (C.c_int, [P, D]),
             "spx_run_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
             "spx_run_num_pairs": (C.c_int, [P, I64]),
+            "spx_run_num_pairs_noalloc": (C.c_int, [P, I64]),
             "spx_run_neighbor_builds": (C.c_int, [P, I64]),
             "spx_batch_create": (C.c_int, [cfgp, C.c_int64, C.c_int32, C.c_int32, C.POINTER(P)]),
             "spx_batch_destroy": (None, [P]),
@@ -246,6 +247,12 @@ class Run:
     def num_pairs(self):
         n = C.c_int64()
         _check(lib().spx_run_num_pairs(self.h, C.byref(n)))
+        return n.value
+
+    def num_pairs_noalloc(self):
+        """num_pairs() through the entry point that allocates no device memory."""
+        n = C.c_int64()
+        _check(lib().spx_run_num_pairs_noalloc(self.h, C.byref(n)))
         return n.value
 
     def neighbor_builds(self):
