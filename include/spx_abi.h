@@ -286,6 +286,24 @@ int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs);
    spx_run_update_connectivity, and when the run's nodes have moved or grown enough that
    the candidate list could miss a pair. */
 int spx_run_neighbor_builds(spx_run* run, int64_t* builds);
+/* Step execution (no counterpart in the reference, whose Integrator::step is plain host
+   calls).  A CheapSynchronousRK2 step in which no candidate list is rebuilt, no run is
+   admitted, finished or removed and no field was read since the previous step is replayed
+   from a HIP graph captured on its first occurrence (one launch instead of about fifteen);
+   its results are the eager step's bit for bit.  Enabled by default; enable = 0 runs every
+   step eagerly (and drops the captured graphs).  TreeGravity, the Limited Q and
+   SynchronousRK2 steps always run eagerly. */
+int spx_run_set_step_graphs(spx_run* run, int32_t enable);
+/* Neighbour-group sizes of the filtering sum-density pass, the grad-h passes and the
+   compatible-energy update (sizes[0..3), each 4 or 8: how many neighbours' loads a lane
+   issues before using them).  Every size gives the same result bits.  By default (sizes null
+   or sizes[0] <= 0) the engine measures: over the first four steps it alternates 4 and 8
+   per pass, timing each, keeps the faster, and measures again once the active particle count
+   has changed by more than 2x. */
+int spx_run_set_group_sizes(spx_run* run, const int32_t* sizes);
+/* out7: graph captures, graph replays, eager steps, group-size measurements, current sizes of
+   the three passes. */
+int spx_run_step_stats(spx_run* run, int64_t* out7);
 /* HBM copy probe (no counterpart in the reference: a measurement aid).  Copies `bytes`
    between two device buffers `reps` times with a plain streaming kernel and returns the
    achieved rate, read + write bytes per second, in GB/s. */
@@ -334,6 +352,11 @@ int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7);
 int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs);
 /* As spx_run_neighbor_builds for one run of a batch. */
 int spx_batch_neighbor_builds(spx_batch* batch, int32_t run, int64_t* builds);
+/* As spx_run_set_step_graphs, spx_run_set_group_sizes and spx_run_step_stats for the batch's
+   steps (one measurement and one graph pair per batch: the runs share the launches). */
+int spx_batch_set_step_graphs(spx_batch* batch, int32_t enable);
+int spx_batch_set_group_sizes(spx_batch* batch, const int32_t* sizes);
+int spx_batch_step_stats(spx_batch* batch, int64_t* out7);
 /* As spx_run_kernel_times for the batch's steps. */
 int spx_batch_kernel_times(spx_batch* batch, double* ms, int32_t max_entries,
                            int32_t* n_entries, char* names, int64_t names_cap);

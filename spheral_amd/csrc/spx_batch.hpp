@@ -159,6 +159,36 @@ class Batch {
   unsigned long long* pairScratch = nullptr;   // numPairs' partial counts (library-allocated)
   int pairScratchBlocks = 0;
   bool mixed = false;
+  // Neighbour-group sizes of the three per-step gather passes (how many neighbours' loads a lane
+  // issues before it uses any of them): 0 the filtering sum-density + dt pass, 1 the grad-h
+  // passes, 2 the compatible-energy update.  Every size performs the same arithmetic in the same
+  // order, so the choice never changes a result bit; it trades latency hiding against registers
+  // (waves per SIMD), and which size wins depends on the dimension, the precision, the neighbour
+  // count and the particle count.  The batch measures instead of guessing: over the first four
+  // steps of a composition it alternates sizes 4 and 8 of every pass (HIP events around each
+  // pass), keeps the faster per pass, and measures again once the active particle count has
+  // changed by more than 2x.  Callers may fix the sizes (setGroupSizes).
+  struct GroupTuner {
+    static constexpr int kPasses = 3;
+    int size[kPasses] = {kGroup, kGroupOmega, kGroupUpdate};
+    bool fixed = false;
+    int phase = -1;                 // -1: not measuring; else the measurement step (0..3)
+    int64_t tunedFor = 0;           // active particles when the sizes were last measured
+    float best[kPasses][2];         // fastest time seen per pass and size (4, 8)
+    int64_t tunings = 0;
+  } tuner;
+  hipEvent_t tuneEv[8] = {};
+  // Steps replayed from a captured HIP graph (see graphStep) and the captured graphs.
+  struct StepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
+  };
+  StepGraph graphs[2];
+  int graphNext = 0;
+  bool useGraphs = true;
+  int64_t graphCaptures = 0, graphReplays = 0, eagerSteps = 0;
+  int* dOverflowG = nullptr;  // the filter's overflow flag in graph-replayed steps
   // per-class kernel time (ms) summed over the steps since timing was enabled
   bool timing = false;
   std::vector<std::string> tnames;
@@ -187,6 +217,8 @@ class Batch {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto e : evPool) (void)hipEventDestroy(e);
+    for (auto e : tuneEv) if (e) (void)hipEventDestroy(e);
+    dropGraphs();
     for (void* p : allocs) (void)hipFree(p);
     for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow,
                     (void*)htiles, (void*)hbuild, (void*)hdelta, (void*)hsub})
@@ -393,6 +425,7 @@ class Batch {
     dstate = dalloc<int>(maxRuns);
     dinit = dalloc<int>(maxRuns);
     dOverflow = dalloc<int>(2);
+    dOverflowG = dalloc<int>(1);
     dbuild = dalloc<int>(maxRuns);
     dsub = dalloc<int>(3*(size_t)maxRuns);
     skeys = dalloc<uint64_t>(ld);
@@ -899,6 +932,14 @@ class Batch {
   // neighbour list is filtered from its candidates.  withDensityDt: follow with the step's
   // sum density and dt vote.
   void buildConnectivity(bool withDensityDt, bool force = false) {
+    connectivityPrep(force);
+    if (nActive == 0) return;
+    connectivityFilter(withDensityDt);
+  }
+  // The part of the connectivity pass that decides and does candidate-list builds: boundary
+  // enforcement, the reorders, the candidate search (with its own capacity checks, which may
+  // resize buffers).  Leaves connValid false.
+  void connectivityPrep(bool force) {
     if (nTotal == 0) raise(SPX_ERR_STATE, "no state uploaded");
     npairsCache = -1;                // every pass re-derives the neighbour lists
     bool any = false;
@@ -978,6 +1019,10 @@ class Batch {
         hsc[r].skinD = hsc[r].skinG = 0ull;
       }
     }
+  }
+  // The step's exact neighbour lists from the candidate lists, with the connectivity records
+  // and, withDensityDt, the sum density and the dt vote; grows the lists on overflow.
+  void connectivityFilter(bool withDensityDt) {
     pack();
     bool ok = false;
     for (int attempt = 0; attempt < 3 && !ok; ++attempt) {
@@ -986,12 +1031,9 @@ class Batch {
       if (withDensityDt) {
         // the step's sum density and dt vote ride on the filter
         hipLaunchKernelGGL(k_reset_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
-        launchD([&](auto d) {
-          constexpr int DD = decltype(d)::value;
-          launchPrecTable(k_density_dt<DD, true, true, double, kGroup, true>,
-                          k_density_dt<DD, true, true, float, kGroup, true>, (int)nbx, f, geo, ph, dtab, (int)nActive, dsc,
-                          dOverflow);
-        });
+        tuneMark(0);
+        launchFilter(nbx, dOverflow);
+        tuneMark(1);
       } else {
         launchD([&](auto d) {
           constexpr int DD = decltype(d)::value;
@@ -1056,15 +1098,95 @@ class Batch {
     const int nb = (int)xcd_grid(nActive, kBlock);
     // midpoint: the record k_midstep wrote; state: the connectivity record (k_pack), or the
     // end-of-step position k_full_update left in it
+    tuneMark(mid ? 2 : 4);
+    if (groupSize(1) == 8) launchOmega<8>(mid, nb, runMask);
+    else launchOmega<4>(mid, nb, runMask);
+    tuneMark(mid ? 3 : 5);
+  }
+  template <int G>
+  void launchOmega(bool mid, int nb, const int* runMask) {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       if (mid)
-        launchPrecTable(k_omega<DD, double, true, kGroupOmega>, k_omega<DD, float, true, kGroupOmega>, nb, f, geo, dtab, (int)nActive,
+        launchPrecTable(k_omega<DD, double, true, G>, k_omega<DD, float, true, G>, nb, f, geo, dtab, (int)nActive,
                         (const double*)f.xhm, f.omegam, runMask);
       else
-        launchPrecTable(k_omega<DD, double, false, kGroupOmega>, k_omega<DD, float, false, kGroupOmega>, nb, f, geo, dtab, (int)nActive,
+        launchPrecTable(k_omega<DD, double, false, G>, k_omega<DD, float, false, G>, nb, f, geo, dtab, (int)nActive,
                         (const double*)f.xh, f.omega, runMask);
     });
+  }
+  // The step's filtering sum-density + dt pass (see buildConnectivity).
+  void launchFilter(unsigned nbx, int* overflow) {
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      if (groupSize(0) == 8)
+        launchPrecTable(k_density_dt<DD, true, true, double, 8, true>, k_density_dt<DD, true, true, float, 8, true>,
+                        (int)nbx, f, geo, ph, dtab, (int)nActive, dsc, overflow);
+      else
+        launchPrecTable(k_density_dt<DD, true, true, double, 4, true>, k_density_dt<DD, true, true, float, 4, true>,
+                        (int)nbx, f, geo, ph, dtab, (int)nActive, dsc, overflow);
+    });
+  }
+
+  // ---------------------------------------------------------------- neighbour-group sizes
+  int groupSize(int pass) const {
+    if (tuner.phase >= 0) return (tuner.phase & 1) ? 8 : 4;
+    return tuner.size[pass];
+  }
+  // Event k of the measurement step (0/1 filter, 2/3 and 4/5 the two grad-h passes, 6/7 the
+  // energy update).
+  void tuneMark(int k) {
+    if (tuner.phase < 0) return;
+    if (!tuneEv[k]) HIP_CHECK(hipEventCreate(&tuneEv[k]));
+    HIP_CHECK(hipEventRecord(tuneEv[k], stream));
+  }
+  void maybeStartTuning() {
+    if (tuner.fixed || tuner.phase >= 0 || nActive == 0) return;
+    if (tuner.tunedFor && nActive <= 2*tuner.tunedFor && 2*nActive >= tuner.tunedFor) return;
+    for (auto& b : tuner.best) b[0] = b[1] = 3.0e38f;
+    tuner.phase = 0;
+  }
+  // After a measured step (the run scalars' download has synchronised the stream).
+  void finishTuningStep() {
+    if (tuner.phase < 0) return;
+    const int v = tuner.phase & 1;
+    float ms[3] = {0.f, 0.f, 0.f};
+    auto el = [&](int a, int b) {
+      float t = 0.f;
+      if (tuneEv[a] && tuneEv[b]) HIP_CHECK(hipEventElapsedTime(&t, tuneEv[a], tuneEv[b]));
+      return t;
+    };
+    ms[0] = el(0, 1);
+    ms[1] = el(2, 3) + el(4, 5);
+    ms[2] = el(6, 7);
+    for (int p = 0; p < 3; ++p) tuner.best[p][v] = std::min(tuner.best[p][v], ms[p]);
+    if (++tuner.phase < 4) return;
+    for (int p = 0; p < 3; ++p) tuner.size[p] = tuner.best[p][1] < tuner.best[p][0] ? 8 : 4;
+    tuner.phase = -1;
+    tuner.tunedFor = nActive;
+    ++tuner.tunings;
+  }
+  // Counters: graph captures, graph replays, eager steps, group-size measurements, and the
+  // current sizes of the three passes.
+  void stepStats(int64_t* out) const {
+    out[0] = graphCaptures;
+    out[1] = graphReplays;
+    out[2] = eagerSteps;
+    out[3] = tuner.tunings;
+    for (int p = 0; p < 3; ++p) out[4 + p] = tuner.size[p];
+  }
+  // Fixed sizes (each 4 or 8), or sizes <= 0: measure.
+  void setGroupSizes(const int* sizes) {
+    if (tuner.phase >= 0) tuner.phase = -1;
+    if (!sizes || sizes[0] <= 0) {
+      tuner.fixed = false;
+      tuner.tunedFor = 0;
+      return;
+    }
+    for (int p = 0; p < 3; ++p)
+      if (sizes[p] != 4 && sizes[p] != 8) raise(SPX_ERR_ARG, "neighbour-group sizes must be 4 or 8");
+    for (int p = 0; p < 3; ++p) tuner.size[p] = sizes[p];
+    tuner.fixed = true;
   }
   // Packed neighbour records and nodeScale of the current state (active runs).
   void pack() {
@@ -1307,6 +1429,9 @@ class Batch {
     if (!retire && !initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called before stepping");
     if (cfg.gravity_only) { stepSynchronousGravity(); return; }
     if (cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) { stepSynchronousHydro(); return; }
+    maybeStartTuning();
+    if (graphsAllowed()) { graphStep(); return; }
+    ++eagerSteps;
     if (timing) {
       evCount = 0;
       mark("start");
@@ -1397,17 +1522,34 @@ class Batch {
   // The full step from the start-of-step state with the midpoint derivatives (compatible energy
   // first), the end-of-step grad-h Omega and the time advance; shared by both integrators.
   void fullStep(int nb) {
+    fullStepLaunches(nb);
+    download(hsc, dsc, sizeof(Scalars)*maxRuns);
+    collectTimes();
+    finishTuningStep();
+    endOfStep();
+  }
+  // The full step's launches: update (compatible energy first), velocity buffer swap, end-of-step
+  // grad-h Omega, time advance.
+  void fullStepLaunches(int nb) {
+    tuneMark(6);
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
-      launchPrec(k_full_update<DD, false, double, kGroupUpdate>, k_full_update<DD, false, float, kGroupUpdate>, nb, f, geo,
-                 ph, dsc, 0.0, (int)nActive);
+      if (groupSize(2) == 8)
+        launchPrec(k_full_update<DD, false, double, 8>, k_full_update<DD, false, float, 8>, nb, f, geo, ph, dsc, 0.0,
+                   (int)nActive);
+      else
+        launchPrec(k_full_update<DD, false, double, 4>, k_full_update<DD, false, float, 4>, nb, f, geo, ph, dsc, 0.0,
+                   (int)nActive);
     });
+    tuneMark(7);
     for (int a = 0; a < D; ++a) {
       std::swap(f.v[a], f.v1[a]);
       // the update wrote only the active runs: the finished runs behind them keep their velocity
-      if (nTotal > nActive)
-        HIP_CHECK(hipMemcpyAsync(f.v[a] + nActive, f.v1[a] + nActive, (nTotal - nActive)*sizeof(double),
-                                 hipMemcpyDeviceToDevice, stream));
+      if (nTotal > nActive) {
+        hipLaunchKernelGGL(k_copy_doubles, dim3(nblocks(nTotal - nActive)), dim3(kBlock), 0, stream,
+                           (const double*)(f.v1[a] + nActive), f.v[a] + nActive, nTotal - nActive);
+        HIP_CHECK(hipGetLastError());
+      }
     }
     mark("energy_update");
     omegaPass(false);
@@ -1415,13 +1557,127 @@ class Batch {
     wrapPending = true;
     hipLaunchKernelGGL(k_advance_time, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
     HIP_CHECK(hipGetLastError());
-    download(hsc, dsc, sizeof(Scalars)*maxRuns);
-    collectTimes();
+  }
+  // Host bookkeeping once the step's run scalars are downloaded.
+  void endOfStep() {
     for (int r = 0; r < maxRuns; ++r) {
       if (runs[r].state != RUN_ACTIVE) continue;
       runs[r].stepOut = true;
       if (retire && reached_end(hsc[r].t, hsc[r].tEnd)) markDone(r);
     }
+  }
+
+  // ---------------------------------------------------------------- graph-replayed steps
+  // After the connectivity pass's build decisions (candidate-list builds and reorders, which
+  // may resize buffers and wait on the device, run eagerly), the rest of a CheapSynchronousRK2
+  // step -- connectivity records, the filtering sum-density + dt pass, dt selection, half step,
+  // midpoint grad-h, derivatives, full step -- launches the same kernels with the same arguments
+  // as the step two before it whenever no buffer was reallocated and the active particle count
+  // is unchanged (the velocity arrays alternate between two buffers).  That part is captured as
+  // a HIP graph per buffer parity and replayed: one launch instead of about fifteen, which is
+  // most of a small run's step time.  The graph holds no memset and no host synchronisation: the
+  // overflow flag is cleared by a kernel, and with the neighbour lists sized to the candidate
+  // lists' capacity the filter cannot overflow (a node's neighbours are a subset of its
+  // candidates).  Results are the eager step's bit for bit.  Steps with newly admitted runs,
+  // TreeGravity (host-side tree sizing), the Limited Q (its gradient copy), SynchronousRK2,
+  // timing or a group-size measurement run eagerly.
+  bool graphsAllowed() const {
+    if (!useGraphs || timing || tuner.phase >= 0 || cfg.gravity || limitedQ()) return false;
+    for (int r = 0; r < maxRuns; ++r)
+      if (runs[r].state == RUN_ACTIVE && runs[r].needInit) return false;
+    return true;
+  }
+  std::vector<unsigned char> graphKey() const {
+    std::vector<unsigned char> k;
+    auto put = [&](const void* p, size_t n) {
+      const unsigned char* c = static_cast<const unsigned char*>(p);
+      k.insert(k.end(), c, c + n);
+    };
+    put(&f, sizeof(f));
+    put(&nActive, sizeof(nActive));
+    put(&nTotal, sizeof(nTotal));
+    put(tuner.size, sizeof(tuner.size));
+    put(&ph, sizeof(ph));
+    put(&geo, sizeof(geo));
+    put(&dtab, sizeof(dtab));
+    const void* bufs[] = {keys, vals, dsc, dOverflowG};
+    put(bufs, sizeof(bufs));
+    return k;
+  }
+  void dropGraphs() {
+    for (auto& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+      g.exec = nullptr;
+      g.graph = nullptr;
+      g.key.clear();
+    }
+  }
+  // The eager step's launches after the connectivity build decisions (connectivity records,
+  // filter + sum density + dt vote, dt selection, half step, midpoint grad-h, derivatives, full
+  // step), with the overflow flag cleared by a kernel.
+  void launchSteadyStep() {
+    pack();
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, dOverflowG, 1, 0);
+    hipLaunchKernelGGL(k_reset_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
+    launchFilter(xcd_grid(nActive, kBlock), dOverflowG);
+    const int nb = (int)xcd_grid(nActive, kBlock);
+    hipLaunchKernelGGL(k_select_dt, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns, ph, cfg.dt_min,
+                       cfg.dt_max, cfg.dt_growth, cfg.gravity);
+    HIP_CHECK(hipGetLastError());
+    launchD([&](auto d) {
+      constexpr int DD = decltype(d)::value;
+      hipLaunchKernelGGL(k_midstep<DD>, dim3(nb), dim3(kBlock), 0, stream, f, ph, (const Scalars*)dsc, (int)nActive);
+    });
+    HIP_CHECK(hipGetLastError());
+    omegaPass(true);
+    sphPass(false, true, true);
+    fullStepLaunches(nb);
+  }
+  void graphStep() {
+    connectivityPrep(false);
+    if (nActive == 0) return;
+    ensureNbrCapacity(f.ccap);          // every candidate fits: the filter cannot overflow
+    const Fields f0 = f;
+    std::vector<unsigned char> key = graphKey();
+    StepGraph* g = nullptr;
+    for (auto& c : graphs)
+      if (c.exec && c.key == key) g = &c;
+    if (!g) {
+      StepGraph& c = graphs[graphNext];
+      graphNext ^= 1;
+      if (c.exec) HIP_CHECK(hipGraphExecDestroy(c.exec));
+      if (c.graph) HIP_CHECK(hipGraphDestroy(c.graph));
+      c.exec = nullptr;
+      c.graph = nullptr;
+      c.key.clear();
+      HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        launchSteadyStep();
+      } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(stream, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        f = f0;
+        throw;
+      }
+      HIP_CHECK(hipStreamEndCapture(stream, &c.graph));
+      HIP_CHECK(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+      c.key = std::move(key);
+      ++graphCaptures;
+      g = &c;
+    }
+    f = f0;
+    HIP_CHECK(hipGraphLaunch(g->exec, stream));
+    for (int a = 0; a < D; ++a) std::swap(f.v[a], f.v1[a]);
+    ++graphReplays;
+    connValid = true;
+    npairsCache = -1;
+    wrapPending = true;
+    HIP_CHECK(hipMemcpyAsync(hOverflow, dOverflowG, sizeof(int), hipMemcpyDeviceToHost, stream));
+    download(hsc, dsc, sizeof(Scalars)*maxRuns);
+    if (*hOverflow != 0) raise(SPX_ERR_STATE, "neighbour list overflow in a replayed step");
+    endOfStep();
   }
 
   int activeRuns() const {
@@ -1537,15 +1793,19 @@ class Batch {
   }
 
   // Slots of one run in the current order and the run-local index of each.
+  // The slots of run `slot` (its contiguous segment [offset, offset + n)) and the run-local
+  // node index held in each: one download of the segment's ids, not of the whole batch.
   void runSlots(int slot, std::vector<int64_t>& local, std::vector<int64_t>& slots) {
-    std::vector<int64_t> oid(nTotal);
-    std::vector<int32_t> rr(nTotal);
-    download(oid.data(), f.oid, nTotal*sizeof(int64_t));
-    download(rr.data(), f.run, nTotal*sizeof(int32_t));
-    local.clear();
-    slots.clear();
-    for (int64_t p = 0; p < nTotal; ++p)
-      if (rr[p] == slot) { slots.push_back(p); local.push_back(oid[p] & (((int64_t)1 << 40) - 1)); }
+    const int64_t o = runs[slot].offset, n = runs[slot].n;
+    std::vector<int64_t> oid(n);
+    if (n) download(oid.data(), f.oid + o, n*sizeof(int64_t));
+    local.resize(n);
+    slots.resize(n);
+    for (int64_t k = 0; k < n; ++k) {
+      if ((oid[k] >> 40) != slot) raise(SPX_ERR_STATE, "internal: a run's particles are not one segment");
+      slots[k] = o + k;
+      local[k] = oid[k] & (((int64_t)1 << 40) - 1);
+    }
   }
 
   void neighbors(int slot, int64_t* offsets, int64_t* nbr, int32_t* img, int64_t capacity, int32_t* countsOnly) {
@@ -1625,19 +1885,20 @@ class Batch {
     std::vector<int64_t> local, slots;
     runSlots(slot, local, slots);
     const int64_t n = ri.n;
-    std::vector<double> tmp(nTotal);
-    auto dl = [&](const double* src) { download(tmp.data(), src, nTotal*sizeof(double)); };
+    // the run's segment of an array: tmp[k] is slot slots[k] = offset + k
+    std::vector<double> tmp(n);
+    auto dl = [&](const double* src) { if (n) download(tmp.data(), src + ri.offset, n*sizeof(double)); };
     auto scalar = [&](const double* src) -> int64_t {
       if (capacity < n) raise(SPX_ERR_ARG, "output capacity too small");
       dl(src);
-      for (size_t k = 0; k < slots.size(); ++k) out[local[k]] = tmp[slots[k]];
+      for (size_t k = 0; k < slots.size(); ++k) out[local[k]] = tmp[k];
       return n;
     };
     auto vec = [&](double* const* src) -> int64_t {
       if (capacity < n*D) raise(SPX_ERR_ARG, "output capacity too small");
       for (int a = 0; a < D; ++a) {
         dl(src[a]);
-        for (size_t k = 0; k < slots.size(); ++k) out[local[k]*D + a] = tmp[slots[k]];
+        for (size_t k = 0; k < slots.size(); ++k) out[local[k]*D + a] = tmp[k];
       }
       return n*D;
     };
@@ -1645,7 +1906,7 @@ class Batch {
       if (capacity < n*D*D) raise(SPX_ERR_ARG, "output capacity too small");
       for (int c = 0; c < D*D; ++c) {
         dl(src + (size_t)c*ld);
-        for (size_t k = 0; k < slots.size(); ++k) out[local[k]*D*D + c] = tmp[slots[k]];
+        for (size_t k = 0; k < slots.size(); ++k) out[local[k]*D*D + c] = tmp[k];
       }
       return n*D*D;
     };
@@ -1654,7 +1915,7 @@ class Batch {
       dl(src);
       for (size_t k = 0; k < slots.size(); ++k)
         for (int a = 0; a < D; ++a)
-          for (int b = 0; b < D; ++b) out[local[k]*D*D + a*D + b] = a == b ? tmp[slots[k]] : 0.0;
+          for (int b = 0; b < D; ++b) out[local[k]*D*D + a*D + b] = a == b ? tmp[k] : 0.0;
       return n*D*D;
     };
     if (name == "position") return vec(f.x);
@@ -1698,9 +1959,9 @@ class Batch {
       // TreeGravity::extraEnergy: 0.5 sum m phi of the last evaluation (a member, not State)
       std::vector<int64_t> local, slots;
       runSlots(slot, local, slots);
-      std::vector<double> tmp(nTotal);
-      download(tmp.data(), f.pot, nTotal*sizeof(double));
-      for (size_t k = 0; k < slots.size(); ++k) pot[local[k]] = tmp[slots[k]];
+      std::vector<double> tmp(n);
+      if (n) download(tmp.data(), f.pot + runs[slot].offset, n*sizeof(double));
+      for (size_t k = 0; k < slots.size(); ++k) pot[local[k]] = tmp[k];
     }
     double M = 0, P[3] = {0, 0, 0}, KE = 0, TE = 0, PE = 0;
     for (int64_t i = 0; i < n; ++i) {

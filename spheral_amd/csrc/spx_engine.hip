@@ -449,6 +449,31 @@ int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs) {
   });
 }
 
+int spx_run_set_step_graphs(spx_run* run, int32_t enable) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    run->impl->useGraphs = enable != 0;
+    if (!enable) run->impl->dropGraphs();
+  });
+}
+
+int spx_run_set_group_sizes(spx_run* run, const int32_t* sizes) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    int s[3] = {0, 0, 0};
+    if (sizes) for (int p = 0; p < 3; ++p) s[p] = sizes[p];
+    run->impl->setGroupSizes(s);
+  });
+}
+
+int spx_run_step_stats(spx_run* run, int64_t* out7) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    if (!out7) spx::raise(SPX_ERR_ARG, "null argument");
+    run->impl->stepStats(out7);
+  });
+}
+
 int spx_run_neighbor_builds(spx_run* run, int64_t* builds) {
   RUN_OR_FAIL(run);
   return guarded([&] {
@@ -545,6 +570,31 @@ int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs) {
       if (r.state == spx::RUN_ACTIVE || r.state == spx::RUN_DONE) { ++k; np += r.n; }
     if (particles) *particles = np;
     if (runs) *runs = k;
+  });
+}
+
+int spx_batch_set_step_graphs(spx_batch* batch, int32_t enable) {
+  BATCH_OR_FAIL(batch);
+  return guarded([&] {
+    batch->impl->useGraphs = enable != 0;
+    if (!enable) batch->impl->dropGraphs();
+  });
+}
+
+int spx_batch_set_group_sizes(spx_batch* batch, const int32_t* sizes) {
+  BATCH_OR_FAIL(batch);
+  return guarded([&] {
+    int s[3] = {0, 0, 0};
+    if (sizes) for (int p = 0; p < 3; ++p) s[p] = sizes[p];
+    batch->impl->setGroupSizes(s);
+  });
+}
+
+int spx_batch_step_stats(spx_batch* batch, int64_t* out7) {
+  BATCH_OR_FAIL(batch);
+  return guarded([&] {
+    if (!out7) spx::raise(SPX_ERR_ARG, "null argument");
+    batch->impl->stepStats(out7);
   });
 }
 

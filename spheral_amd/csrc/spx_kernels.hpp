@@ -968,6 +968,20 @@ __global__ void __launch_bounds__(kBlock) k_finite(Fields f, int n, Scalars* sc)
   if (!ok) atomicOr(&sc[f.run[i]].nonfinite, 1);
 }
 
+// dst[0..n) = value.  Clears flags inside captured step graphs, which hold no memset nodes.
+__global__ void k_fill_int(int* dst, int n, int value) {
+  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = value;
+}
+
+// dst[0..n) = src[0..n) (doubles): the velocities of the finished runs behind the active ones,
+// which the update does not write, carried into the other velocity buffer.
+__global__ void __launch_bounds__(kBlock) k_copy_doubles(const double* __restrict__ src, double* __restrict__ dst,
+                                                         int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 // Unique-pair count of the neighbour lists (ConnectivityMap::nodePairList().size(),
 // Neighbor/ConnectivityMap.cc:925-927): an entry (i, j) is i's pair when j is an image or a later
 // slot (calculatePairInteraction, ConnectivityMapInline.hh:305-307).  One partial count per block

@@ -26,6 +26,8 @@ EXPORTS = (
     "spx_batch_create", "spx_batch_destroy", "spx_batch_add_run", "spx_batch_remove_run", "spx_batch_step",
     "spx_batch_run_status", "spx_batch_get_field", "spx_batch_conservation", "spx_batch_usage",
     "spx_batch_kernel_times", "spx_batch_neighbor_builds", "spx_bandwidth_probe",
+    "spx_run_set_step_graphs", "spx_run_set_group_sizes", "spx_run_step_stats",
+    "spx_batch_set_step_graphs", "spx_batch_set_group_sizes", "spx_batch_step_stats",
 )
 
 RUN_ACTIVE = 1
@@ -91,6 +93,12 @@ def lib():
             "spx_batch_conservation": (C.c_int, [P, C.c_int32, D]),
             "spx_batch_usage": (C.c_int, [P, I64, I32]),
             "spx_batch_neighbor_builds": (C.c_int, [P, C.c_int32, I64]),
+            "spx_run_set_step_graphs": (C.c_int, [P, C.c_int32]),
+            "spx_run_set_group_sizes": (C.c_int, [P, I32]),
+            "spx_run_step_stats": (C.c_int, [P, I64]),
+            "spx_batch_set_step_graphs": (C.c_int, [P, C.c_int32]),
+            "spx_batch_set_group_sizes": (C.c_int, [P, I32]),
+            "spx_batch_step_stats": (C.c_int, [P, I64]),
             "spx_batch_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
             "spx_bandwidth_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, D]),
         }
@@ -133,6 +141,24 @@ FIELD_KIND = {
     "mass first moment": "v", "M SPH correction": "t", "mass density gradient": "v",
     "normalization": "s", "new mass density": "s", "gravitational potential": "s",
 }
+
+
+
+STEP_STATS = ("graph_captures", "graph_replays", "eager_steps", "group_tunings", "group_filter", "group_gradh",
+              "group_update")
+
+
+def _sizes(sizes):
+    if sizes is None:
+        return None
+    arr = (C.c_int32*3)(*[int(v) for v in sizes])
+    return C.cast(arr, C.POINTER(C.c_int32))
+
+
+def _stats(fn, h):
+    out = (C.c_int64*7)()
+    _check(fn(h, out))
+    return dict(zip(STEP_STATS, list(out)))
 
 
 class Run:
@@ -255,6 +281,18 @@ class Run:
         _check(lib().spx_run_num_pairs_noalloc(self.h, C.byref(n)))
         return n.value
 
+    def set_step_graphs(self, enable):
+        """Replay steady CheapSynchronousRK2 steps from captured HIP graphs (default on)."""
+        _check(lib().spx_run_set_step_graphs(self.h, 1 if enable else 0))
+
+    def set_group_sizes(self, sizes=None):
+        """Fix the neighbour-group sizes (filter, grad-h, update; each 4 or 8), or None: measure."""
+        _check(lib().spx_run_set_group_sizes(self.h, _sizes(sizes)))
+
+    def step_stats(self):
+        """Graph captures/replays, eager steps, group-size measurements and the current sizes."""
+        return _stats(lib().spx_run_step_stats, self.h)
+
     def neighbor_builds(self):
         """Candidate-list builds so far (start-up, forced rebuilds, and skin exhaustion)."""
         n = C.c_int64()
@@ -336,6 +374,15 @@ class Batch:
         p, r = C.c_int64(), C.c_int32()
         _check(lib().spx_batch_usage(self.h, C.byref(p), C.byref(r)))
         return p.value, r.value
+
+    def set_step_graphs(self, enable):
+        _check(lib().spx_batch_set_step_graphs(self.h, 1 if enable else 0))
+
+    def set_group_sizes(self, sizes=None):
+        _check(lib().spx_batch_set_group_sizes(self.h, _sizes(sizes)))
+
+    def step_stats(self):
+        return _stats(lib().spx_batch_step_stats, self.h)
 
     def neighbor_builds(self, rid):
         n = C.c_int64()
