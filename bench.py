@@ -195,10 +195,16 @@ def ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+FLOP_TREE_INTERACTION = 27   # one monopole or point-point term of the walk (+,-,*,/,sqrt = 1)
+
+
 def evrard(args):
     """Evrard collapse with TreeGravity: args.particles particles to t = args.t_end (default
     0.8) in mixed precision, then the same run in plain FP64; reports the mixed run's
-    particle-steps/s and its L1 distance from the FP64 run (density, specific energy)."""
+    particle-steps/s and its L1 distance from the FP64 run (density, specific energy), the tree
+    walk's interactions per walk and interactions/s (counting copy of the walk on the final
+    tree; walk time from the run's HIP events), and the oracle (FP64 CPU restatement) on the
+    same problem as the CPU leg (one warm-up step, then the median of three single steps)."""
     t_end = args.t_end if args.t_end != 0.1 else 0.8
     out = {}
     for prec, name in ((abi.SPX_PREC_MIXED, "mixed"), (abi.SPX_PREC_FP64, "fp64")):
@@ -214,13 +220,25 @@ def evrard(args):
             k += k1
             print(f"[bench] evrard {name}: step {k} t {t:.4f}", file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
-        out[name] = {"seconds": wall, "steps": k, "t": t, "particle_steps_per_s": n*k/wall,
-                     "builds": r.neighbor_builds(),
-                     "kernel_ms_per_step": {c: v/max(k, 1) for c, v in r.kernel_times().items()},
-                     "fields": {f: r.field(f) for f in ("mass density", "specific thermal energy")}}
+        kt = r.kernel_times()
+        fields = {f: r.field(f) for f in ("mass density", "specific thermal energy")}
+        builds = r.neighbor_builds()
+        # the walk's throughput at the end state: interactions of one walk over the tree of the
+        # last step, and the walk's time over 5 more steps
+        cells, points = r.gravity_interactions()
+        r.enable_timing()
+        for _ in range(5):
+            r.step()
+        walk_ms = r.kernel_times().get("gravity", float("nan"))/5
+        out[name] = {"seconds": wall, "steps": k, "t": t, "particle_steps_per_s": n*k/wall, "builds": builds,
+                     "kernel_ms_per_step": {c: v/max(k, 1) for c, v in kt.items()},
+                     "walk_interactions": {"cell": cells, "point": points}, "walk_ms_end": walk_ms,
+                     "fields": fields}
         r.close()
     m, d = out["mixed"], out["fp64"]
     l1 = {f: float(np.abs(m["fields"][f] - d["fields"][f]).sum()/np.abs(d["fields"][f]).sum()) for f in m["fields"]}
+    walk_ms = d["walk_ms_end"]
+    inter = d["walk_interactions"]["cell"] + d["walk_interactions"]["point"]
     line = {
         "metric": "SPH + TreeGravity particle-steps/s",
         "value": m["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": m["steps"], "warmup": 0,
@@ -229,9 +247,43 @@ def evrard(args):
         "config": {"workload": f"evrard_3d_{args.particles}_to_t{t_end}", "particles": args.particles, "t_end": t_end,
                    "opening": 0.5, "softening": 0.01, "kernel": "BSpline TableKernel(1000), nPerh 1.51"},
         "t_reached": m["t"], "fp64_particle_steps_per_s": d["particle_steps_per_s"], "fp64_steps": d["steps"],
+        "fp64_t_reached": d["t"],
         "l1_mixed_vs_fp64": l1, "neighbor_list_builds": {"mixed": m["builds"], "fp64": d["builds"]},
         "kernel_ms_per_step": m["kernel_ms_per_step"], "fp64_kernel_ms_per_step": d["kernel_ms_per_step"],
+        # the walk (FP64 in both modes), at the end state of the FP64 run: one walk per step
+        "tree_walk": {"interactions_per_walk": inter, "cell_interactions": d["walk_interactions"]["cell"],
+                      "point_interactions": d["walk_interactions"]["point"], "walk_ms": walk_ms,
+                      "interactions_per_s": inter/(walk_ms*1e-3),
+                      "flop_per_interaction": FLOP_TREE_INTERACTION,
+                      "achieved_tflops": inter*FLOP_TREE_INTERACTION/(walk_ms*1e-3)/1e12,
+                      "peak_tflops": PEAK_FP64/1e12,
+                      "frac": inter*FLOP_TREE_INTERACTION/(walk_ms*1e-3)/PEAK_FP64},
     }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        share = cpu_share()
+        O.set_threads(share["threads"])
+        cfg, st = problems.evrard_3d(args.particles, precision=abi.SPX_PREC_FP64)
+        n = len(st["m"])
+        o = O.OracleRun(cfg, n)
+        o.set_state(**st)
+        o.initialize()
+        o.step()
+        seg = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            o.step()
+            seg.append(time.perf_counter() - t0)
+            print(f"[bench] evrard cpu step {len(seg)}/3: {seg[-1]:.2f} s", file=sys.stderr, flush=True)
+        med = float(np.median(seg))
+        line["cpu_baseline"] = {"value": n/med, "unit": "particle-steps/s", "cores": share["threads"], "kind": "port",
+                                "sample": f"oracle (FP64 C++/OpenMP restatement, TreeGravity walk per point in "
+                                          f"parallel) on the same Evrard problem at {n} particles: one warm-up step "
+                                          f"after start-up, then the median of 3 single steps; {share['threads']} "
+                                          f"OpenMP threads; {cpu_model()}",
+                                "segments_s": seg}
+        line["speedup_vs_cpu_fp64"] = d["particle_steps_per_s"]/(n/med)
     print(json.dumps(line), flush=True)
 
 
@@ -485,6 +537,8 @@ def main():
         "conservation": {"momentum": cons[1:4].tolist(), "energy": float(cons[4] + cons[5])},
     }
     line["step_roofline"]["frac"] = value/line["step_roofline"]["ceiling_particle_steps_per_s"]
+    # graph replays (none here: timed steps run eagerly) and the measured neighbour-group sizes
+    line["step_stats"] = run.step_stats()
     # The timed window may hold few candidate-list rebuilds (none at 20 steps); the rate over whole
     # rebuild cycles: from the step after one rebuild through the step of the second rebuild after it.
     amort = rebuild_amortised(run, n)

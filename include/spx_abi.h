@@ -286,6 +286,11 @@ int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs);
    spx_run_update_connectivity, and when the run's nodes have moved or grown enough that
    the candidate list could miss a pair. */
 int spx_run_neighbor_builds(spx_run* run, int64_t* builds);
+/* TreeGravity interactions of one walk over the run's current tree (no counterpart in the
+   reference: a measurement aid for the walk's throughput).  out2[0]: accepted-cell (monopole)
+   interactions, out2[1]: point-point interactions in opened leaves, summed over the points.
+   Runs a counting copy of the walk; allocates and waits on the device. */
+int spx_run_gravity_interactions(spx_run* run, int64_t* out2);
 /* Step execution (no counterpart in the reference, whose Integrator::step is plain host
    calls).  A CheapSynchronousRK2 step in which no candidate list is rebuilt, no run is
    admitted, finished or removed and no field was read since the previous step is replayed

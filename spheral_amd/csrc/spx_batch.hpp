@@ -254,6 +254,12 @@ class Batch {
     if (cfg.gravity && (cfg.G <= 0.0 || cfg.softening <= 0.0 || cfg.opening <= 0.0 || cfg.ftimestep <= 0.0))
       raise(SPX_ERR_ARG, "TreeGravity needs G, softening, opening, ftimestep > 0");
     if (cfg.gravity && maxRuns != 1) raise(SPX_ERR_UNSUPPORTED, "a TreeGravity batch holds one run");
+    // TreeGravity's DynamicalTime vote (TreeGravity.cc:239-300, 497-518) needs every point's
+    // interaction buckets and the reference's partial-sort order; the engine votes
+    // AccelerationRatio only
+    if (cfg.gravity && cfg.gravity_time_step != SPX_GRAVITY_DT_ACCELERATION_RATIO)
+      raise(SPX_ERR_UNSUPPORTED, "TreeGravity timeStepChoice DynamicalTime is not implemented on the GPU "
+                                 "(AccelerationRatio is)");
     if (cfg.integrator != SPX_INTEGRATOR_CHEAP_SYNCHRONOUS_RK2 && cfg.integrator != SPX_INTEGRATOR_SYNCHRONOUS_RK2)
       raise(SPX_ERR_ARG, "unknown integrator");
     if (cfg.gravity_only && (!cfg.gravity || cfg.integrator != SPX_INTEGRATOR_SYNCHRONOUS_RK2))
@@ -1256,6 +1262,7 @@ class Batch {
     if (!cfg.gravity || nActive == 0) return;
     double* xs[3] = {mid ? f.xm[0] : f.x[0], mid ? f.xm[1] : f.x[1], mid ? f.xm[2] : f.x[2]};
     grav.build(xs, f.m, (int)nActive);
+    mark("tree_build");
     grav.evaluate(f.DvDtG, f.pot, (int)nActive, cfg.G, cfg.softening, cfg.opening, dsc, cfg.ftimestep);
     if (toState)
       HIP_CHECK(hipMemcpyAsync(f.potS, f.pot, nActive*sizeof(double), hipMemcpyDeviceToDevice, stream));

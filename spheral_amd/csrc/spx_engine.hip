@@ -449,6 +449,16 @@ int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs) {
   });
 }
 
+int spx_run_gravity_interactions(spx_run* run, int64_t* out2) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    if (!out2) spx::raise(SPX_ERR_ARG, "null argument");
+    auto& B = *run->impl;
+    if (!B.cfg.gravity) spx::raise(SPX_ERR_STATE, "the run has no TreeGravity");
+    B.grav.countInteractions(B.cfg.softening, B.cfg.opening, out2);
+  });
+}
+
 int spx_run_set_step_graphs(spx_run* run, int32_t enable) {
   RUN_OR_FAIL(run);
   return guarded([&] {

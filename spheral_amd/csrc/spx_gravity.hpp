@@ -210,8 +210,9 @@ __device__ __forceinline__ int group_min(int v) {
 // compiler they are not written here.  GROUP 16: each row of 16 lanes walks together instead
 // of the whole wavefront (the walks of 16 neighbouring points differ less than those of 64, so
 // fewer cells are visited), with per-group cell reads (64/GROUP addresses per instruction);
-// 8 and 4 likewise.
-template <int GROUP>
+// 8 and 4 likewise.  COUNT (a measurement variant, never in a step): also counts each point's
+// monopole (accepted cell) and point-point interactions into counts[2p], counts[2p+1].
+template <int GROUP, bool COUNT = false>
 __global__ void __launch_bounds__(kBlock) k_tree_walk(
     const double* __restrict__ tx0, const double* __restrict__ tx1, const double* __restrict__ tx2,
     const double* __restrict__ tm, const double* __restrict__ cx0, const double* __restrict__ cx1,
@@ -219,8 +220,10 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
     const int* __restrict__ cfirst, const int* __restrict__ clast, const int* __restrict__ cskip,
     const unsigned char* __restrict__ clevel, const double* __restrict__ ccs2, int ncells, int npts,
     const uint32_t* __restrict__ perm, double soft2, double open2, double soft, double G, double* __restrict__ acc0,
-    double* __restrict__ acc1, double* __restrict__ acc2, double* __restrict__ pot, double* __restrict__ blockMin) {
+    double* __restrict__ acc1, double* __restrict__ acc2, double* __restrict__ pot, double* __restrict__ blockMin,
+    unsigned long long* __restrict__ counts = nullptr) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  unsigned long long nCell = 0, nPoint = 0;
   const bool valid = p < npts;
   double dtv = 1.7976931348623157e308;
   double xi = 0.0, yi = 0.0, zi = 0.0;
@@ -254,6 +257,7 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
       const double f = Mc*(1.0/rs);
       ax += f*nx; ay += f*ny; az += f*nz;
       phi -= Mc*(1.0/sqrt(rs));
+      if (COUNT) ++nCell;
       next = sk;
     } else if (q0 == q1 || lvl == kTreeBits) {
       for (int q = q0; q <= q1; ++q) {
@@ -267,6 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
           const double f = mq*(1.0/rs);
           ax += f*(ex*inv); ay += f*(ey*inv); az += f*(ez*inv);
           phi -= mq*(1.0/sqrt(rs));
+          if (COUNT) ++nPoint;
         }
       }
       next = sk;
@@ -282,6 +287,7 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
     acc1[s] = G*ay;
     acc2[s] = G*az;
     pot[s] = G*phi;
+    if (COUNT) { counts[2*(size_t)p] = nCell; counts[2*(size_t)p + 1] = nPoint; }
   }
   __shared__ double red[kBlock];
   red[threadIdx.x] = dtv;
@@ -453,6 +459,37 @@ class GravityTree {
                        acc[0], acc[1], acc[2], pot, part);
     hipLaunchKernelGGL(k_tree_dt, dim3(1), dim3(kBlock), 0, stream, part, nb, sc, G, ftimestep);
     HIP_CHECK(hipGetLastError());
+  }
+
+  // Interactions of one walk over the current tree (the counting variant of the walk, into
+  // throw-away outputs): out[0] monopole (accepted cell) interactions, out[1] point-point
+  // interactions, summed over the points.  A measurement aid: allocates, waits on the stream.
+  void countInteractions(double soft, double opening, int64_t* out) {
+    const int nn = (int)view.n;
+    if (!active || nn <= 0) raise(SPX_ERR_STATE, "no TreeGravity evaluation to count");
+    const int nb = (int)((nn + kBlock - 1)/kBlock);
+    double* scratch = nullptr;
+    unsigned long long* counts = nullptr;
+    HIP_CHECK(hipMalloc(&scratch, (size_t)(4*(size_t)std::max(nn, 1) + nb)*sizeof(double)));
+    HIP_CHECK(hipMalloc(&counts, 2*(size_t)std::max(nn, 1)*sizeof(unsigned long long)));
+    double* a = scratch;
+    hipLaunchKernelGGL((k_tree_walk<kTreeGroup, true>), dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1],
+                       view.tx[2], view.tm, view.cx[0], view.cx[1], view.cx[2], view.M, view.rc2, view.first, view.last,
+                       view.skip, view.level, (const double*)cs2, view.ncells, view.n, (const uint32_t*)perm, soft*soft,
+                       opening*opening, soft, 1.0, a, a + nn, a + 2*(size_t)nn, a + 3*(size_t)nn, a + 4*(size_t)nn, counts);
+    const hipError_t e = hipGetLastError();
+    std::vector<unsigned long long> h(2*(size_t)nn);
+    hipError_t e2 = hipSuccess;
+    if (e == hipSuccess) {
+      e2 = hipMemcpyAsync(h.data(), counts, h.size()*sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
+      if (e2 == hipSuccess) e2 = hipStreamSynchronize(stream);
+    }
+    (void)hipFree(scratch);
+    (void)hipFree(counts);
+    HIP_CHECK(e);
+    HIP_CHECK(e2);
+    out[0] = out[1] = 0;
+    for (int p = 0; p < nn; ++p) { out[0] += (int64_t)h[2*(size_t)p]; out[1] += (int64_t)h[2*(size_t)p + 1]; }
   }
 };
 

@@ -26,7 +26,7 @@ EXPORTS = (
     "spx_batch_create", "spx_batch_destroy", "spx_batch_add_run", "spx_batch_remove_run", "spx_batch_step",
     "spx_batch_run_status", "spx_batch_get_field", "spx_batch_conservation", "spx_batch_usage",
     "spx_batch_kernel_times", "spx_batch_neighbor_builds", "spx_bandwidth_probe",
-    "spx_run_set_step_graphs", "spx_run_set_group_sizes", "spx_run_step_stats",
+    "spx_run_set_step_graphs", "spx_run_set_group_sizes", "spx_run_step_stats", "spx_run_gravity_interactions",
     "spx_batch_set_step_graphs", "spx_batch_set_group_sizes", "spx_batch_step_stats",
 )
 
@@ -94,6 +94,7 @@ def lib():
             "spx_batch_usage": (C.c_int, [P, I64, I32]),
             "spx_batch_neighbor_builds": (C.c_int, [P, C.c_int32, I64]),
             "spx_run_set_step_graphs": (C.c_int, [P, C.c_int32]),
+            "spx_run_gravity_interactions": (C.c_int, [P, I64]),
             "spx_run_set_group_sizes": (C.c_int, [P, I32]),
             "spx_run_step_stats": (C.c_int, [P, I64]),
             "spx_batch_set_step_graphs": (C.c_int, [P, C.c_int32]),
@@ -280,6 +281,12 @@ class Run:
         n = C.c_int64()
         _check(lib().spx_run_num_pairs_noalloc(self.h, C.byref(n)))
         return n.value
+
+    def gravity_interactions(self):
+        """(accepted-cell, point-point) interactions of one TreeGravity walk over the current tree."""
+        out = (C.c_int64*2)()
+        _check(lib().spx_run_gravity_interactions(self.h, out))
+        return int(out[0]), int(out[1])
 
     def set_step_graphs(self, enable):
         """Replay steady CheapSynchronousRK2 steps from captured HIP graphs (default on)."""

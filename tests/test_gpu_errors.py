@@ -19,6 +19,9 @@ UNSUP, ARG = -4, -1
     (dict(viscosity=7), ARG),
     (dict(viscosity=abi.SPX_Q_LIMITED_MONAGHAN_GINGOLD, eta_fold_frac=0.0), ARG),
     (dict(integrator=5), ARG),
+    # TreeGravity's DynamicalTime vote is refused, not replaced by AccelerationRatio
+    (dict(gravity=1, G=1.0, softening=0.01, opening=0.5, ftimestep=0.1,
+          gravity_time_step=abi.SPX_GRAVITY_DT_DYNAMICAL_TIME), UNSUP),
 ])
 def test_bad_configurations_are_rejected(gpu, change, code):
     cfg = abi.default_config(3)
