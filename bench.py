@@ -314,7 +314,13 @@ def serve_trace(args):
     cfgs = {k: serve_problem(k, 4096, abi.SPX_PREC_MIXED)[0] for k in SERVE_KINDS}
     print(f"[bench] serve: {R} requests over {arrivals[-1]:.2f} s, {sum(len(r[1]['m']) for r in reqs)} particles",
           file=sys.stderr, flush=True)
-    srv = serve.Server({k: (cfgs[k], 6_000_000, 128) for k in SERVE_KINDS})
+    # one lane per kind, each with an HBM budget (bytes); the lane sizes its batch from its
+    # configuration's measured bytes per particle (a 65536-particle probe request)
+    probes = {k: serve_problem(k, 65536, abi.SPX_PREC_MIXED)[1] for k in SERVE_KINDS}
+    srv = serve.Server({k: (cfgs[k], args.lane_gib*2**30, 128, probes[k]) for k in SERVE_KINDS})
+    lanes = {k: {"hbm_budget_bytes": l.hbm_bytes, "bytes_per_particle": l.bytes_per_particle, "capacity": l.capacity}
+             for k, l in srv.lanes.items()}
+    print(f"[bench] lanes: {lanes}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     handles = []
     for (k, st, te), at in zip(reqs, arrivals):
@@ -327,6 +333,9 @@ def serve_trace(args):
         if h.error is not None:
             raise h.error
     makespan = time.perf_counter() - t0
+    for k, l in srv.lanes.items():
+        lanes[k]["peak_device_bytes"] = l.peak_bytes
+        lanes[k]["batch_steps"] = l.steps
     srv.close()
     tts = [h.time_to_solution for h in handles]
     print(f"[bench] served: makespan {makespan:.2f} s, p50 {serve.percentile(tts, 50):.2f} s", file=sys.stderr, flush=True)
@@ -356,7 +365,7 @@ def serve_trace(args):
         "makespan_s": makespan, "particle_steps_per_s": psteps/makespan,
         "sequential_fp64_runs_per_hr": R/end*3600.0, "sequential_fp64_tts_p50_s": serve.percentile(tts_seq, 50),
         "sequential_fp64_tts_p99_s": serve.percentile(tts_seq, 99), "speedup_runs_per_hr": end/makespan,
-        "max_l1_vs_fp64": l1,
+        "max_l1_vs_fp64": l1, "lanes": lanes,
     }
     print(json.dumps(line), flush=True)
 
@@ -429,11 +438,14 @@ def main():
     ap.add_argument("--cpu-n1d", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps per timed CPU segment (median of 3)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rebuild-window", action="store_true",
+                    help="skip the rate over whole candidate-rebuild cycles (profiling runs)")
     ap.add_argument("--integrator", choices=["cheap", "synchronous"], default="cheap",
                     help="taylor_green: CheapSynchronousRK2 (the headline) or SynchronousRK2")
     ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard", "serve", "accuracy"],
                     default="taylor_green")
-    ap.add_argument("--requests", type=int, default=100, help="serve: number of requests in the trace")
+    ap.add_argument("--requests", type=int, default=500, help="serve: number of requests in the trace")
+    ap.add_argument("--lane-gib", type=float, default=24.0, help="serve: HBM budget of each lane (GiB)")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--particles", type=int, default=250_000, help="evrard: particle count")
     ap.add_argument("--max-steps", type=int, default=100_000, help="evrard: step limit")
@@ -541,7 +553,7 @@ def main():
     line["step_stats"] = run.step_stats()
     # The timed window may hold few candidate-list rebuilds (none at 20 steps); the rate over whole
     # rebuild cycles: from the step after one rebuild through the step of the second rebuild after it.
-    amort = rebuild_amortised(run, n)
+    amort = None if args.no_rebuild_window else rebuild_amortised(run, n)
     if amort is not None:
         line["value_rebuild_amortised"] = amort["value"]
         line["rebuild_window"] = {k: amort[k] for k in ("steps", "builds", "ms_per_step")}

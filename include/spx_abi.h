@@ -286,6 +286,10 @@ int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs);
    spx_run_update_connectivity, and when the run's nodes have moved or grown enough that
    the candidate list could miss a pair. */
 int spx_run_neighbor_builds(spx_run* run, int64_t* builds);
+/* Device memory the run holds, in bytes: its state and work arrays, neighbour and candidate
+   lists (which grow with the largest neighbour count seen) and the tree (no counterpart in the
+   reference: for admission by memory budget). */
+int spx_run_device_bytes(spx_run* run, int64_t* bytes);
 /* TreeGravity interactions of one walk over the run's current tree (no counterpart in the
    reference: a measurement aid for the walk's throughput).  out2[0]: accepted-cell (monopole)
    interactions, out2[1]: point-point interactions in opened leaves, summed over the points.
@@ -360,6 +364,8 @@ int spx_batch_neighbor_builds(spx_batch* batch, int32_t run, int64_t* builds);
 /* As spx_run_set_step_graphs, spx_run_set_group_sizes and spx_run_step_stats for the batch's
    steps (one measurement and one graph pair per batch: the runs share the launches). */
 int spx_batch_set_step_graphs(spx_batch* batch, int32_t enable);
+/* As spx_run_device_bytes for the batch (all its runs share the arrays sized by its capacity). */
+int spx_batch_device_bytes(spx_batch* batch, int64_t* bytes);
 int spx_batch_set_group_sizes(spx_batch* batch, const int32_t* sizes);
 int spx_batch_step_stats(spx_batch* batch, int64_t* out7);
 /* As spx_run_kernel_times for the batch's steps. */

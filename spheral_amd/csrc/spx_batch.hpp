@@ -106,6 +106,8 @@ class Batch {
   GravityTree grav;
   std::vector<RunInfo> runs;
   std::vector<void*> allocs;
+  std::vector<size_t> allocBytes;   // size of each entry of allocs
+  int64_t deviceBytes = 0;          // device memory the batch holds (besides the tree's)
   // device side
   RunGrid* dgrids = nullptr;
   Scalars* dsc = nullptr;
@@ -164,15 +166,21 @@ class Batch {
   // passes, 2 the compatible-energy update.  Every size performs the same arithmetic in the same
   // order, so the choice never changes a result bit; it trades latency hiding against registers
   // (waves per SIMD), and which size wins depends on the dimension, the precision, the neighbour
-  // count and the particle count.  The batch measures instead of guessing: over the first four
-  // steps of a composition it alternates sizes 4 and 8 of every pass (HIP events around each
-  // pass), keeps the faster per pass, and measures again once the active particle count has
-  // changed by more than 2x.  Callers may fix the sizes (setGroupSizes).
+  // count and the particle count.  The batch measures instead of guessing: once a composition
+  // has run two warm-up steps it alternates sizes 4 and 8 of every pass over six steps (HIP
+  // events around each pass, three samples per size), and moves a pass off its default size
+  // (4, 8, 8: measured on Taylor-Green 128^3) only when the other size's fastest sample is at
+  // least 3 % faster; it measures again once the active particle count has changed by more
+  // than 2x.  Callers may fix the sizes (setGroupSizes).
   struct GroupTuner {
     static constexpr int kPasses = 3;
+    static constexpr int kSteps = 6;            // measured steps: sizes 4, 8, 4, 8, 4, 8
+    static constexpr int kWarm = 2;             // steps run before measuring
+    static constexpr int kDefault[kPasses] = {kGroup, kGroupOmega, kGroupUpdate};
     int size[kPasses] = {kGroup, kGroupOmega, kGroupUpdate};
     bool fixed = false;
-    int phase = -1;                 // -1: not measuring; else the measurement step (0..3)
+    int warm = 0;                   // > 0: warm-up steps left before measuring
+    int phase = -1;                 // -1: not measuring; else the measurement step (0..kSteps-1)
     int64_t tunedFor = 0;           // active particles when the sizes were last measured
     float best[kPasses][2];         // fastest time seen per pass and size (4, 8)
     int64_t tunings = 0;
@@ -269,8 +277,11 @@ class Batch {
   template <typename T>
   T* dalloc(size_t count) {
     void* p = nullptr;
-    HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1)*sizeof(T)));
+    const size_t bytes = std::max<size_t>(count, 1)*sizeof(T);
+    HIP_CHECK(hipMalloc(&p, bytes));
     allocs.push_back(p);
+    allocBytes.push_back(bytes);
+    deviceBytes += (int64_t)bytes;
     return static_cast<T*>(p);
   }
   template <typename T>
@@ -470,7 +481,12 @@ class Batch {
   void dfree(void* p) {
     if (!p) return;
     auto it = std::find(allocs.begin(), allocs.end(), p);
-    if (it != allocs.end()) allocs.erase(it);
+    if (it != allocs.end()) {
+      const size_t k = (size_t)(it - allocs.begin());
+      deviceBytes -= (int64_t)allocBytes[k];
+      allocBytes.erase(allocBytes.begin() + (std::ptrdiff_t)k);
+      allocs.erase(it);
+    }
     HIP_CHECK(hipFree(p));
   }
 
@@ -501,12 +517,7 @@ class Batch {
   void ensureNbrCapacity(int c) {
     if (c <= f.cap) return;
     HIP_CHECK(hipStreamSynchronize(stream));
-    for (void* old : {(void*)f.nbr, f.pdd}) {
-      if (!old) continue;
-      auto it = std::find(allocs.begin(), allocs.end(), old);
-      if (it != allocs.end()) allocs.erase(it);
-      HIP_CHECK(hipFree(old));
-    }
+    for (void* old : {(void*)f.nbr, f.pdd}) dfree(old);
     f.nbr = nullptr;
     f.pdd = nullptr;
     f.nbr = dalloc<uint32_t>((size_t)c*ld);
@@ -748,12 +759,7 @@ class Batch {
     if (base > cellsAlloc) {
       HIP_CHECK(hipStreamSynchronize(stream));
       const int64_t c = base + base/4 + 1024;
-      for (void* p : {(void*)cellStart, (void*)cellEnd}) {
-        if (!p) continue;
-        auto it = std::find(allocs.begin(), allocs.end(), p);
-        if (it != allocs.end()) allocs.erase(it);
-        HIP_CHECK(hipFree(p));
-      }
+      for (void* p : {(void*)cellStart, (void*)cellEnd}) dfree(p);
       cellStart = dalloc<int>(c);
       cellEnd = dalloc<int>(c);
       cellsAlloc = c;
@@ -787,9 +793,7 @@ class Batch {
     HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, keys, keysAlt, vals, valsAlt, (size_t)nTotal, 0, endBit, stream));
     if (tmp > sortTmpBytes) {
       HIP_CHECK(hipStreamSynchronize(stream));
-      auto it = std::find(allocs.begin(), allocs.end(), sortTmp);
-      if (it != allocs.end()) allocs.erase(it);
-      HIP_CHECK(hipFree(sortTmp));
+      dfree(sortTmp);
       sortTmp = dalloc<char>(tmp);
       sortTmpBytes = tmp;
     }
@@ -1147,10 +1151,15 @@ class Batch {
     HIP_CHECK(hipEventRecord(tuneEv[k], stream));
   }
   void maybeStartTuning() {
-    if (tuner.fixed || tuner.phase >= 0 || nActive == 0) return;
+    if (tuner.fixed || tuner.phase >= 0 || tuner.warm > 0 || nActive == 0) return;
     if (tuner.tunedFor && nActive <= 2*tuner.tunedFor && 2*nActive >= tuner.tunedFor) return;
     for (auto& b : tuner.best) b[0] = b[1] = 3.0e38f;
-    tuner.phase = 0;
+    tuner.warm = GroupTuner::kWarm;
+    tuner.tunedFor = nActive;
+  }
+  // End of any step: count down the warm-up; the step after it is measured.
+  void tunerStepDone() {
+    if (tuner.warm > 0 && --tuner.warm == 0) tuner.phase = 0;
   }
   // After a measured step (the run scalars' download has synchronised the stream).
   void finishTuningStep() {
@@ -1166,8 +1175,11 @@ class Batch {
     ms[1] = el(2, 3) + el(4, 5);
     ms[2] = el(6, 7);
     for (int p = 0; p < 3; ++p) tuner.best[p][v] = std::min(tuner.best[p][v], ms[p]);
-    if (++tuner.phase < 4) return;
-    for (int p = 0; p < 3; ++p) tuner.size[p] = tuner.best[p][1] < tuner.best[p][0] ? 8 : 4;
+    if (++tuner.phase < GroupTuner::kSteps) return;
+    for (int p = 0; p < 3; ++p) {
+      const int d = GroupTuner::kDefault[p] == 8 ? 1 : 0;
+      tuner.size[p] = tuner.best[p][1 - d] < 0.97f*tuner.best[p][d] ? (d ? 4 : 8) : GroupTuner::kDefault[p];
+    }
     tuner.phase = -1;
     tuner.tunedFor = nActive;
     ++tuner.tunings;
@@ -1183,7 +1195,8 @@ class Batch {
   }
   // Fixed sizes (each 4 or 8), or sizes <= 0: measure.
   void setGroupSizes(const int* sizes) {
-    if (tuner.phase >= 0) tuner.phase = -1;
+    tuner.phase = -1;
+    tuner.warm = 0;
     if (!sizes || sizes[0] <= 0) {
       tuner.fixed = false;
       tuner.tunedFor = 0;
@@ -1533,6 +1546,7 @@ class Batch {
     download(hsc, dsc, sizeof(Scalars)*maxRuns);
     collectTimes();
     finishTuningStep();
+    tunerStepDone();
     endOfStep();
   }
   // The full step's launches: update (compatible energy first), velocity buffer swap, end-of-step
@@ -1684,6 +1698,7 @@ class Batch {
     HIP_CHECK(hipMemcpyAsync(hOverflow, dOverflowG, sizeof(int), hipMemcpyDeviceToHost, stream));
     download(hsc, dsc, sizeof(Scalars)*maxRuns);
     if (*hOverflow != 0) raise(SPX_ERR_STATE, "neighbour list overflow in a replayed step");
+    tunerStepDone();
     endOfStep();
   }
 

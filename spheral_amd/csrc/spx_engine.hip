@@ -449,6 +449,14 @@ int spx_run_num_pairs_noalloc(spx_run* run, int64_t* npairs) {
   });
 }
 
+int spx_run_device_bytes(spx_run* run, int64_t* bytes) {
+  RUN_OR_FAIL(run);
+  return guarded([&] {
+    if (!bytes) spx::raise(SPX_ERR_ARG, "null argument");
+    *bytes = run->impl->deviceBytes + run->impl->grav.bytes;
+  });
+}
+
 int spx_run_gravity_interactions(spx_run* run, int64_t* out2) {
   RUN_OR_FAIL(run);
   return guarded([&] {
@@ -580,6 +588,14 @@ int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs) {
       if (r.state == spx::RUN_ACTIVE || r.state == spx::RUN_DONE) { ++k; np += r.n; }
     if (particles) *particles = np;
     if (runs) *runs = k;
+  });
+}
+
+int spx_batch_device_bytes(spx_batch* batch, int64_t* bytes) {
+  BATCH_OR_FAIL(batch);
+  return guarded([&] {
+    if (!bytes) spx::raise(SPX_ERR_ARG, "null argument");
+    *bytes = batch->impl->deviceBytes + batch->impl->grav.bytes;
   });
 }
 

@@ -28,6 +28,7 @@ EXPORTS = (
     "spx_batch_kernel_times", "spx_batch_neighbor_builds", "spx_bandwidth_probe",
     "spx_run_set_step_graphs", "spx_run_set_group_sizes", "spx_run_step_stats", "spx_run_gravity_interactions",
     "spx_batch_set_step_graphs", "spx_batch_set_group_sizes", "spx_batch_step_stats",
+    "spx_run_device_bytes", "spx_batch_device_bytes",
 )
 
 RUN_ACTIVE = 1
@@ -100,6 +101,8 @@ def lib():
             "spx_batch_set_step_graphs": (C.c_int, [P, C.c_int32]),
             "spx_batch_set_group_sizes": (C.c_int, [P, I32]),
             "spx_batch_step_stats": (C.c_int, [P, I64]),
+            "spx_run_device_bytes": (C.c_int, [P, I64]),
+            "spx_batch_device_bytes": (C.c_int, [P, I64]),
             "spx_batch_kernel_times": (C.c_int, [P, D, C.c_int32, I32, C.c_char_p, C.c_int64]),
             "spx_bandwidth_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, D]),
         }
@@ -282,6 +285,12 @@ class Run:
         _check(lib().spx_run_num_pairs_noalloc(self.h, C.byref(n)))
         return n.value
 
+    def device_bytes(self):
+        """Device memory the run holds (bytes)."""
+        n = C.c_int64()
+        _check(lib().spx_run_device_bytes(self.h, C.byref(n)))
+        return n.value
+
     def gravity_interactions(self):
         """(accepted-cell, point-point) interactions of one TreeGravity walk over the current tree."""
         out = (C.c_int64*2)()
@@ -381,6 +390,12 @@ class Batch:
         p, r = C.c_int64(), C.c_int32()
         _check(lib().spx_batch_usage(self.h, C.byref(p), C.byref(r)))
         return p.value, r.value
+
+    def device_bytes(self):
+        """Device memory the batch holds (bytes)."""
+        n = C.c_int64()
+        _check(lib().spx_batch_device_bytes(self.h, C.byref(n)))
+        return n.value
 
     def set_step_graphs(self, enable):
         _check(lib().spx_batch_set_step_graphs(self.h, 1 if enable else 0))

@@ -1,9 +1,12 @@
 """Continuous batching of solve requests on one GPU.
 
 Each problem type (one engine configuration: dimension, kernel, EOS, boundaries) owns one
-resident batch (spx_batch) and one host worker thread.  A worker admits queued requests
-while the batch's particle budget and run slots allow, steps every active run of its batch
-once, hands back the state of runs that reached their end time and frees their slots.
+resident batch (spx_batch) and one host worker thread.  A lane is given an HBM budget in
+bytes: it measures the device bytes per particle of its configuration on a probe run (state,
+work arrays, neighbour and candidate lists, all of which the engine sizes by the batch's
+particle capacity), and sizes its batch to the largest capacity that fits the budget.  A worker
+admits queued requests while that capacity and the run slots allow, steps every active run of
+its batch once, hands back the state of runs that reached their end time and frees their slots.
 Requests that arrive while others are on the GPU join the next step; batches of different
 types step concurrently (each on its own HIP stream; the engine's calls release the GIL).
 
@@ -41,14 +44,41 @@ class Request:
         return None if self.done is None else self.done - self.arrival
 
 
-class _Lane:
-    """One problem type: a batch, its queue and its worker."""
+PROBE_STEPS = 12
 
-    def __init__(self, kind, cfg, capacity, max_runs, device=0):
+
+def bytes_per_particle(cfg, probe_state, device=0):
+    """Device bytes per particle of a batch of this configuration: a probe batch sized to the
+    probe state, at its largest over PROBE_STEPS steps -- past the first steps (which size the
+    neighbour and candidate lists), the group-size measurement and into graph-replayed steps (whose
+    neighbour lists are sized to the candidate lists' capacity)."""
+    n = len(probe_state["m"])
+    b = engine.Batch(cfg, n, 1, device)
+    try:
+        b.add_run(probe_state, 1.0e30)
+        peak = b.device_bytes()
+        for _ in range(PROBE_STEPS):
+            b.step()
+            peak = max(peak, b.device_bytes())
+        return peak/n
+    finally:
+        b.close()
+
+
+class _Lane:
+    """One problem type: a batch sized by an HBM budget, its queue and its worker."""
+
+    def __init__(self, kind, cfg, hbm_bytes, max_runs, probe_state, device=0):
         self.kind = kind
-        self.capacity = int(capacity)
+        self.hbm_bytes = int(hbm_bytes)
+        self.bytes_per_particle = bytes_per_particle(cfg, probe_state, device)
+        # the lists may grow past the probe's (denser regions): keep 10 % of the budget in hand
+        self.capacity = int(0.9*self.hbm_bytes/self.bytes_per_particle)
+        if self.capacity < 1:
+            raise ValueError(f"{kind}: an HBM budget of {hbm_bytes} bytes holds no particle")
         self.max_runs = int(max_runs)
         self.batch = engine.Batch(cfg, self.capacity, self.max_runs, device)
+        self.peak_bytes = self.batch.device_bytes()
         self.queue = collections.deque()
         self.active = {}
         self.cv = threading.Condition()
@@ -58,7 +88,8 @@ class _Lane:
 
     def submit(self, req):
         if req.n > self.capacity:
-            raise ValueError(f"request of {req.n} particles exceeds the {self.kind} batch capacity {self.capacity}")
+            raise ValueError(f"request of {req.n} particles exceeds the {self.kind} batch capacity {self.capacity} "
+                             f"({self.hbm_bytes} bytes of HBM)")
         with self.cv:
             self.queue.append(req)
             self.cv.notify()
@@ -98,6 +129,7 @@ class _Lane:
                 if self.active:
                     self.batch.step()
                     self.steps += 1
+                    self.peak_bytes = max(self.peak_bytes, self.batch.device_bytes())
                 for rid in list(self.active):
                     st = self.batch.status(rid)
                     if st["state"] != engine.RUN_DONE:
@@ -123,8 +155,9 @@ class Server:
     """Continuous-batching front end: one lane per problem type."""
 
     def __init__(self, lanes, device=0):
-        """lanes: {kind: (cfg, particle capacity, run slots)}."""
-        self.lanes = {k: _Lane(k, cfg, cap, slots, device) for k, (cfg, cap, slots) in lanes.items()}
+        """lanes: {kind: (cfg, HBM budget in bytes, run slots, probe state)}; the probe state (a
+        typical request of the kind) measures the configuration's bytes per particle."""
+        self.lanes = {k: _Lane(k, cfg, hbm, slots, probe, device) for k, (cfg, hbm, slots, probe) in lanes.items()}
         for lane in self.lanes.values():
             lane.thread.start()
 

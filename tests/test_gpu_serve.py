@@ -1,5 +1,7 @@
 """Continuous batching: requests that arrive while others run are admitted into the resident
-batches, and every result equals the same request solved alone (bit-identical)."""
+batches, and every result equals the same request solved alone (bit-identical).  Lanes are sized
+by an HBM budget in bytes; the budgets here are small on purpose (a few thousand particles), so
+admission has to wait for slots and memory to free up."""
 import time
 
 import numpy as np
@@ -10,6 +12,11 @@ from spheral_amd import abi, problems, serve
 pytestmark = pytest.mark.gpu
 
 
+def budget_for(cfg, probe, particles):
+    """The HBM budget (bytes) of a lane that holds `particles` particles of this configuration."""
+    return int(serve.bytes_per_particle(cfg, probe)*(particles + 0.5)/0.9)
+
+
 def test_served_results_equal_alone(gpu):
     print("[case] serve mixed trace", flush=True)
     kinds = {"sod2d": lambda n: problems.sod_tube(2, n), "sod3d": lambda n: problems.sod_tube(3, n),
@@ -17,8 +24,11 @@ def test_served_results_equal_alone(gpu):
     trace = [("sod2d", 900, 0.01), ("gresho2d", 1000, 0.004), ("sod3d", 1200, 0.006),
              ("sod2d", 1500, 0.008), ("gresho2d", 700, 0.003), ("sod3d", 800, 0.004)]
     cfgs = {k: f(1000)[0] for k, f in kinds.items()}
-    # small budgets so that admission has to wait for slots and particles to free up
-    srv = serve.Server({k: (cfgs[k], 2600, 2) for k in kinds})
+    probes = {k: f(1000)[1] for k, f in kinds.items()}
+    # small budgets so that admission has to wait for slots and memory to free up
+    srv = serve.Server({k: (cfgs[k], budget_for(cfgs[k], probes[k], 2600), 2, probes[k]) for k in kinds})
+    for k, lane in srv.lanes.items():
+        assert lane.capacity == 2600, k
     handles = []
     for k, n, te in trace:
         handles.append((k, n, te, srv.submit(k, kinds[k](n)[1], te)))
@@ -26,6 +36,8 @@ def test_served_results_equal_alone(gpu):
     for *_, h in handles:
         assert h.event.wait(120)
         assert h.error is None
+    for k, lane in srv.lanes.items():
+        assert lane.peak_bytes <= lane.hbm_bytes, k
     srv.close()
     for k, n, te, h in handles:
         ref, steps = serve.solve_alone(cfgs[k], kinds[k](n)[1], te)
@@ -43,8 +55,9 @@ def test_lane_admits_after_every_run_finished(gpu, slots, trace):
     queued: the removed runs' slots and particles count as free, so the lane admits the queue
     instead of waiting on a step that never comes."""
     print(f"[case] serve slots={slots} trace={trace}", flush=True)
-    cfg = problems.sod_tube(2, 1000)[0]
-    srv = serve.Server({"sod2d": (cfg, 2000, slots)})
+    cfg, probe = problems.sod_tube(2, 1000)
+    srv = serve.Server({"sod2d": (cfg, budget_for(cfg, probe, 2000), slots, probe)})
+    assert srv.lanes["sod2d"].capacity == 2000
     handles = [srv.submit("sod2d", problems.sod_tube(2, n)[1], te) for n, te in trace]
     handles.append(srv.submit("sod2d", problems.sod_tube(2, 600)[1], 0.002))
     for h in handles:
