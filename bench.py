@@ -139,12 +139,13 @@ def ensemble(args):
     b = engine.Batch(cfg_m, capacity=total, max_runs=args.runs)
     ids = [b.add_run(s, args.t_end) for s in states]
     t_admit = time.perf_counter() - t0
-    b.enable_timing()
+    if args.kernel_breakdown:
+        b.enable_timing()            # per-class events: every step then runs eagerly
     bsteps = 0
     while b.step() > 0:
         bsteps += 1
     t_batch = time.perf_counter() - t0
-    per_class = {k: v/max(bsteps, 1) for k, v in b.kernel_times().items()}
+    per_class = {k: v/max(bsteps, 1) for k, v in b.kernel_times().items()} if args.kernel_breakdown else None
     cycles = np.array([b.status(r)["cycle"] for r in ids])
     psteps = int((cycles*sizes).sum())
     print(f"[bench] batched: {t_batch:.2f} s, {bsteps} batch steps", file=sys.stderr, flush=True)
@@ -167,6 +168,7 @@ def ensemble(args):
         "batched_seconds": t_batch,
         "admission_seconds": t_admit,
         "kernel_ms_per_batch_step": per_class,
+        "step_stats": b.step_stats(),
         "neighbor_list_builds": int(sum(b.neighbor_builds(r) for r in ids)),
         "particle_steps": psteps,
         "particle_steps_per_s": psteps/t_batch,
@@ -453,6 +455,8 @@ def main():
     ap.add_argument("--t-end", type=float, default=0.1, help="ensemble: end time of every run")
     ap.add_argument("--seed", type=int, default=7, help="ensemble: seed of the run sizes")
     ap.add_argument("--no-sequential", action="store_true", help="ensemble: skip the sequential FP64 leg")
+    ap.add_argument("--kernel-breakdown", action="store_true",
+                    help="ensemble: per-kernel-class HIP-event times (the batch's steps then run eagerly)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU: sharding a run across GPUs is not "
