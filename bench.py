@@ -317,9 +317,10 @@ def serve_trace(args):
     print(f"[bench] serve: {R} requests over {arrivals[-1]:.2f} s, {sum(len(r[1]['m']) for r in reqs)} particles",
           file=sys.stderr, flush=True)
     # one lane per kind, each with an HBM budget (bytes); the lane sizes its batch from its
-    # configuration's measured bytes per particle (a 65536-particle probe request)
+    # configuration's measured bytes per particle (a 65536-particle probe request run to the
+    # kind's end time)
     probes = {k: serve_problem(k, 65536, abi.SPX_PREC_MIXED)[1] for k in SERVE_KINDS}
-    srv = serve.Server({k: (cfgs[k], args.lane_gib*2**30, 128, probes[k]) for k in SERVE_KINDS})
+    srv = serve.Server({k: (cfgs[k], args.lane_gib*2**30, 128, probes[k], SERVE_T_END[k]) for k in SERVE_KINDS})
     lanes = {k: {"hbm_budget_bytes": l.hbm_bytes, "bytes_per_particle": l.bytes_per_particle, "capacity": l.capacity}
              for k, l in srv.lanes.items()}
     print(f"[bench] lanes: {lanes}", file=sys.stderr, flush=True)
@@ -334,9 +335,11 @@ def serve_trace(args):
         h.event.wait()
         if h.error is not None:
             raise h.error
+        print(f"[bench] served {i + 1}/{R}: {h.kind} n={h.n} tts {h.time_to_solution:.2f} s", file=sys.stderr, flush=True)
     makespan = time.perf_counter() - t0
     for k, l in srv.lanes.items():
         lanes[k]["peak_device_bytes"] = l.peak_bytes
+        lanes[k]["peak_within_budget"] = l.peak_bytes <= l.hbm_bytes
         lanes[k]["batch_steps"] = l.steps
     srv.close()
     tts = [h.time_to_solution for h in handles]

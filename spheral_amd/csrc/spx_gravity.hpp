@@ -1,4 +1,4 @@
-// spx_gravity.hpp -- TreeGravity on the device (3-D).
+// spx_gravity.hpp -- TreeGravity on the device (3-D octree; 2-D quadtree).
 //
 // Tree: the reference's octree (Gravity/TreeGravityInline.hh:37-160, TreeGravity.cc:362-477)
 // rebuilt from 60-bit Morton keys.  The level-l cell of a point is the l-level prefix of its
@@ -13,8 +13,13 @@
 // Walk: TreeGravity::applyTreeForces (TreeGravity.cc:818-950) per point from the root's
 // daughters: accept a cell's monopole when |xcm - xi|^2 > cellsize^2/opening^2 + rcm2cc2,
 // otherwise sum a leaf's points directly (self screened by r^2/soft^2 > 1e-10) or descend.
-// Force M/(r^2+soft^2) along the unit vector, potential -M/sqrt(r^2+soft^2); then
-// DvDt *= G, potential *= G, dt vote ftimestep*min sqrt(soft/|a|)/sqrt(G) (TreeGravity.cc:143-328,483-497).
+// Force M/(r^2+soft^2) along the unit vector, potential -M/sqrt(r^2+soft^2) in 3-D; in 2-D
+// M/sqrt(r^2+soft^2) and -M log(sqrt(r^2+soft^2)) (TreeGravity.cc:54-90); then DvDt *= G,
+// potential *= G, dt vote ftimestep*min sqrt(soft/|a|)/sqrt(G) (TreeGravity.cc:143-328,483-497).
+//
+// 2-D: the points get z = 0, so the interleaved keys' z bits are all zero and the octree's cells
+// are exactly the quadtree's (level-l cell = floor(u*2^l) in x and y, TreeGravityInline.hh:16-33);
+// the cell-centre offset rcm2cc2 is summed over x and y only.
 #pragma once
 
 #include <rocprim/rocprim.hpp>
@@ -131,7 +136,7 @@ __global__ void __launch_bounds__(kBlock) k_tree_cells(const uint64_t* code, int
 
 // Monopoles of the cells at one level, bottom-up (children already done).
 __global__ void __launch_bounds__(kBlock) k_tree_moments(TreeView t, int lvl, double* M, double* cx0, double* cx1, double* cx2,
-                                                         double* rc2) {
+                                                         double* rc2, int dim) {
   const int c = blockIdx.x*blockDim.x + threadIdx.x;
   if (c >= t.ncells || t.level[c] != lvl) return;
   const int f = t.first[c], l = t.last[c];
@@ -159,8 +164,7 @@ __global__ void __launch_bounds__(kBlock) k_tree_moments(TreeView t, int lvl, do
   const double cs = t.box/(double)(1u << lvl);
   const uint64_t key = t.code[f];
   double r2 = 0.0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
+  for (int a = 0; a < dim; ++a) {
     uint64_t ia = 0;
     for (int b = 0; b < lvl; ++b) ia |= ((key >> (3*(kTreeBits - lvl + b) + a)) & 1ull) << b;
     const double cc = t.xmin[a] + ((double)ia + 0.5)*cs;
@@ -212,7 +216,7 @@ __device__ __forceinline__ int group_min(int v) {
 // fewer cells are visited), with per-group cell reads (64/GROUP addresses per instruction);
 // 8 and 4 likewise.  COUNT (a measurement variant, never in a step): also counts each point's
 // monopole (accepted cell) and point-point interactions into counts[2p], counts[2p+1].
-template <int GROUP, bool COUNT = false>
+template <int GROUP, bool COUNT = false, int DIM = 3>
 __global__ void __launch_bounds__(kBlock) k_tree_walk(
     const double* __restrict__ tx0, const double* __restrict__ tx1, const double* __restrict__ tx2,
     const double* __restrict__ tm, const double* __restrict__ cx0, const double* __restrict__ cx1,
@@ -257,9 +261,9 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
       const double nx = r2 > 1.0e-100 ? dx*inv : 1.0, ny = dy*inv, nz = dz*inv;
       const double rs = r2 + soft2;
       const double ri = rsqrt(rs);
-      const double f = Mc*(ri*ri);
+      const double f = DIM == 3 ? Mc*(ri*ri) : Mc*ri;
       ax += f*nx; ay += f*ny; az += f*nz;
-      phi -= Mc*ri;
+      phi -= DIM == 3 ? Mc*ri : Mc*(0.5*log(rs));
       if (COUNT) ++nCell;
       next = sk;
     } else if (q0 == q1 || lvl == kTreeBits) {
@@ -271,9 +275,9 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
           const double rs = q2 + soft2;
           const double mq = tm[q];
           const double ri = rsqrt(rs);
-          const double f = mq*(ri*ri);
+          const double f = DIM == 3 ? mq*(ri*ri) : mq*ri;
           ax += f*(ex*inv); ay += f*(ey*inv); az += f*(ez*inv);
-          phi -= mq*ri;
+          phi -= DIM == 3 ? mq*ri : mq*(0.5*log(rs));
           if (COUNT) ++nPoint;
         }
       }
@@ -337,6 +341,8 @@ class GravityTree {
   double* hpart = nullptr;
   double* cs2 = nullptr;
   double** xsPtrs = nullptr;
+  double* zero = nullptr;           // 2-D: the z coordinate of every point
+  double* sink = nullptr;           // 2-D: the walk's z acceleration
   // cells
   int64_t cellCap = 0;
   int ncells = 0;
@@ -361,8 +367,14 @@ class GravityTree {
     stream = s;
     active = enabled;
     if (!active) return;
-    if (D != 3) raise(SPX_ERR_UNSUPPORTED, "TreeGravity on the GPU is implemented for 3-D");
+    if (D != 3 && D != 2) raise(SPX_ERR_UNSUPPORTED, "TreeGravity on the GPU is implemented for 2-D and 3-D");
     (void)ld;
+    if (D == 2) {
+      // z = 0 for every point (the quadtree as a flat octree), and a sink for the z acceleration
+      zero = alloc<double>(n);
+      HIP_CHECK(hipMemsetAsync(zero, 0, (size_t)std::max<int64_t>(n, 1)*sizeof(double), stream));
+      sink = alloc<double>(n);
+    }
     codes = alloc<uint64_t>(n);
     codesSorted = alloc<uint64_t>(n);
     idx = alloc<uint32_t>(n);
@@ -406,7 +418,7 @@ class GravityTree {
 
   void build(double* const* xs, const double* m, int nn) {
     const int nb = (int)((nn + kBlock - 1)/kBlock);
-    double* hx[3] = {xs[0], xs[1], xs[2]};
+    double* hx[3] = {xs[0], xs[1], D == 3 ? xs[2] : zero};
     HIP_CHECK(hipMemcpyAsync(xsPtrs, hx, sizeof(hx), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_tree_bbox, dim3(nb), dim3(kBlock), 0, stream, xsPtrs, nn, part);
     HIP_CHECK(hipGetLastError());
@@ -447,7 +459,7 @@ class GravityTree {
     view.box = box;
     const int cb = (int)((total + kBlock - 1)/kBlock);
     for (int l = kTreeBits; l >= 0; --l) {
-      hipLaunchKernelGGL(k_tree_moments, dim3(cb), dim3(kBlock), 0, stream, view, l, M, cx[0], cx[1], cx[2], rc2);
+      hipLaunchKernelGGL(k_tree_moments, dim3(cb), dim3(kBlock), 0, stream, view, l, M, cx[0], cx[1], cx[2], rc2, D);
     }
     HIP_CHECK(hipGetLastError());
     double h2[32];
@@ -459,10 +471,12 @@ class GravityTree {
   void evaluate(double* const* acc, double* pot, int nn, double G, double soft, double opening, Scalars* sc,
                 double ftimestep) {
     const int nb = (int)((nn + kBlock - 1)/kBlock);
-    hipLaunchKernelGGL(k_tree_walk<kTreeGroup>, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1], view.tx[2], view.tm,
+    double* az = D == 3 ? acc[2] : sink;
+    auto walk = D == 3 ? k_tree_walk<kTreeGroup, false, 3> : k_tree_walk<kTreeGroup, false, 2>;
+    hipLaunchKernelGGL(walk, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1], view.tx[2], view.tm,
                        view.cx[0], view.cx[1], view.cx[2], view.M, view.rc2, view.first, view.last, view.skip, view.level,
                        (const double*)cs2, view.ncells, view.n, (const uint32_t*)perm, soft*soft, opening*opening, soft, G,
-                       acc[0], acc[1], acc[2], pot, part);
+                       acc[0], acc[1], az, pot, part, (unsigned long long*)nullptr);
     hipLaunchKernelGGL(k_tree_dt, dim3(1), dim3(kBlock), 0, stream, part, nb, sc, G, ftimestep);
     HIP_CHECK(hipGetLastError());
   }
@@ -479,7 +493,8 @@ class GravityTree {
     HIP_CHECK(hipMalloc(&scratch, (size_t)(4*(size_t)std::max(nn, 1) + nb)*sizeof(double)));
     HIP_CHECK(hipMalloc(&counts, 2*(size_t)std::max(nn, 1)*sizeof(unsigned long long)));
     double* a = scratch;
-    hipLaunchKernelGGL((k_tree_walk<kTreeGroup, true>), dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1],
+    auto walk = D == 3 ? k_tree_walk<kTreeGroup, true, 3> : k_tree_walk<kTreeGroup, true, 2>;
+    hipLaunchKernelGGL(walk, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1],
                        view.tx[2], view.tm, view.cx[0], view.cx[1], view.cx[2], view.M, view.rc2, view.first, view.last,
                        view.skip, view.level, (const double*)cs2, view.ncells, view.n, (const uint32_t*)perm, soft*soft,
                        opening*opening, soft, 1.0, a, a + nn, a + 2*(size_t)nn, a + 3*(size_t)nn, a + 4*(size_t)nn, counts);

@@ -218,6 +218,32 @@ def evrard_3d(n=250_000, seed=4242, nperh=1.51, gamma=5.0 / 3.0, u0=0.05, M=1.0,
     return cfg, dict(x=x, v=np.zeros((n, 3)), m=m, rho=rho, eps=eps, H=H)
 
 
+def gravity_disk_2d(n=20_000, seed=4243, nperh=1.51, gamma=5.0 / 3.0, u0=0.05, M=1.0, R=1.0,
+                    kernel=abi.SPX_KERNEL_BSPLINE, precision=abi.SPX_PREC_FP64):
+    """The Evrard set-up in 2-D (TreeGravity's quadtree, Gravity_PYB11.py QuadTreeGravity): a disk
+    with surface density M/(2 pi R r) sampled by r = R*u (uniform in u), isotropic angles,
+    u = 0.05, v = 0; TreeGravity G = 1, opening 0.5, softening 0.01 (2-D force law M/sqrt(r^2 + s^2),
+    potential -M log(sqrt(r^2 + s^2)), TreeGravity.cc:54-64)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = R * rng.uniform(0.0, 1.0, n)
+    ph = rng.uniform(0.0, 2.0 * math.pi, n)
+    x = np.stack([r * np.cos(ph), r * np.sin(ph)], 1)
+    m = np.full(n, M / n)
+    rho = M / (2.0 * math.pi * R * np.maximum(r, 1e-3 * R))
+    eps = np.full(n, u0)
+    hloc = np.sqrt(m / rho)
+    H = _iso_H(n, 2, 1.0 / (nperh * hloc))
+    cfg = abi.default_config(2, kernel=kernel, nperh=nperh, gamma=gamma, precision=precision)
+    cfg.kernel = kernel
+    cfg.neighbor_extent = abi.KERNEL_EXTENT[kernel]
+    cfg.gravity = 1
+    cfg.G = 1.0
+    cfg.opening = 0.5
+    cfg.softening = 0.01
+    cfg.ftimestep = 0.1
+    return cfg, dict(x=x, v=np.zeros((n, 2)), m=m, rho=rho, eps=eps, H=H)
+
+
 def uniform_lattice(dim, n1d, nperh=2.0, kernel=abi.SPX_KERNEL_WENDLANDC4, rho0=1.0, P0=1.0, gamma=5.0 / 3.0,
                     periodic=True):
     """Perfect periodic lattice at rest (uniform pressure): a closed-form test state."""

@@ -45,21 +45,30 @@ class Request:
 
 
 PROBE_STEPS = 12
+PROBE_MAX_STEPS = 200000
 
 
-def bytes_per_particle(cfg, probe_state, device=0):
-    """Device bytes per particle of a batch of this configuration: a probe batch sized to the
-    probe state, at its largest over PROBE_STEPS steps -- past the first steps (which size the
-    neighbour and candidate lists), the group-size measurement and into graph-replayed steps (whose
-    neighbour lists are sized to the candidate lists' capacity)."""
+def bytes_per_particle(cfg, probe_state, t_end=None, device=0):
+    """Device bytes per particle of a batch of this configuration, measured on a probe batch sized
+    to the probe state: the largest device footprint over the probe run divided by its particle
+    count.  The neighbour and candidate lists are sized by the most crowded node of the batch, which
+    changes as the flow evolves (a shock compresses, a vortex shears), so with `t_end` the probe
+    runs to that time -- a typical request of the kind, end to end.  Without it the probe takes
+    PROBE_STEPS steps: past the first steps (which size the lists), the group-size measurement and
+    into graph-replayed steps (whose neighbour lists are sized to the candidate lists' capacity)."""
     n = len(probe_state["m"])
     b = engine.Batch(cfg, n, 1, device)
     try:
-        b.add_run(probe_state, 1.0e30)
+        rid = b.add_run(probe_state, 1.0e30 if t_end is None else float(t_end))
         peak = b.device_bytes()
-        for _ in range(PROBE_STEPS):
+        for k in range(PROBE_STEPS if t_end is None else PROBE_MAX_STEPS):
             b.step()
             peak = max(peak, b.device_bytes())
+            if t_end is not None and b.status(rid)["state"] == engine.RUN_DONE:
+                break
+        else:
+            if t_end is not None:
+                raise RuntimeError(f"probe run did not reach t = {t_end} in {PROBE_MAX_STEPS} steps")
         return peak/n
     finally:
         b.close()
@@ -68,10 +77,10 @@ def bytes_per_particle(cfg, probe_state, device=0):
 class _Lane:
     """One problem type: a batch sized by an HBM budget, its queue and its worker."""
 
-    def __init__(self, kind, cfg, hbm_bytes, max_runs, probe_state, device=0):
+    def __init__(self, kind, cfg, hbm_bytes, max_runs, probe_state, probe_t_end=None, device=0):
         self.kind = kind
         self.hbm_bytes = int(hbm_bytes)
-        self.bytes_per_particle = bytes_per_particle(cfg, probe_state, device)
+        self.bytes_per_particle = bytes_per_particle(cfg, probe_state, probe_t_end, device)
         # the lists may grow past the probe's (denser regions): keep 10 % of the budget in hand
         self.capacity = int(0.9*self.hbm_bytes/self.bytes_per_particle)
         if self.capacity < 1:
@@ -155,9 +164,11 @@ class Server:
     """Continuous-batching front end: one lane per problem type."""
 
     def __init__(self, lanes, device=0):
-        """lanes: {kind: (cfg, HBM budget in bytes, run slots, probe state)}; the probe state (a
-        typical request of the kind) measures the configuration's bytes per particle."""
-        self.lanes = {k: _Lane(k, cfg, hbm, slots, probe, device) for k, (cfg, hbm, slots, probe) in lanes.items()}
+        """lanes: {kind: (cfg, HBM budget in bytes, run slots, probe state[, probe end time])}; the
+        probe (a typical request of the kind, run to its end time when one is given) measures the
+        configuration's bytes per particle."""
+        self.lanes = {k: _Lane(k, spec[0], spec[1], spec[2], spec[3], spec[4] if len(spec) > 4 else None, device)
+                      for k, spec in lanes.items()}
         for lane in self.lanes.values():
             lane.thread.start()
 

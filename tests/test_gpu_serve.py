@@ -70,3 +70,24 @@ def test_lane_admits_after_every_run_finished(gpu, slots, trace):
         ref, steps = serve.solve_alone(cfg, problems.sod_tube(2, n)[1], te)
         assert h.steps == steps
         assert np.array_equal(h.result["mass density"], ref["mass density"])
+
+
+def test_probe_to_end_time_bounds_the_lane(gpu):
+    """The lists grow as the flow evolves (the shock compresses the gas), so a lane sized from a
+    probe run to the request's end time stays within its HBM budget over whole requests, and that
+    probe measures at least what a 12-step probe does."""
+    print("[case] serve probe to t_end", flush=True)
+    cfg, probe = problems.sod_tube(2, 2000)
+    short = serve.bytes_per_particle(cfg, probe)
+    full = serve.bytes_per_particle(cfg, probe, 0.15)
+    assert full >= short
+    budget = int(full*(4000 + 0.5)/0.9)
+    srv = serve.Server({"sod2d": (cfg, budget, 4, probe, 0.15)})
+    lane = srv.lanes["sod2d"]
+    assert lane.capacity == 4000
+    handles = [srv.submit("sod2d", problems.sod_tube(2, 2000)[1], 0.15) for _ in range(2)]
+    for h in handles:
+        assert h.event.wait(300)
+        assert h.error is None
+    assert lane.peak_bytes <= lane.hbm_bytes
+    srv.close()

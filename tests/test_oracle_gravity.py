@@ -1,6 +1,8 @@
 """Pins of the oracle's TreeGravity (Gravity/TreeGravity.cc): with the opening angle -> 0 no
 cell monopole is ever accepted (TreeGravity.cc:882), so the walk reduces to the direct sum
-of the Plummer-like force law over all other points (TreeGravity.cc:60-90)."""
+of the Plummer-like force law over all other points (TreeGravity.cc:60-90): in 3-D
+M/(r^2+s^2) along the unit vector with potential -M/sqrt(r^2+s^2), in 2-D (the quadtree)
+M/sqrt(r^2+s^2) with potential -M log(sqrt(r^2+s^2))."""
 import numpy as np
 import pytest
 
@@ -17,6 +19,51 @@ def direct(x, m, G, soft):
     a = G*np.sum(f[:, :, None]*d, 1)
     phi = -G*np.sum(np.where(mask, m[None, :]/np.sqrt(rs), 0.0), 1)
     return a, phi
+
+
+def direct_2d(x, m, G, soft):
+    d = x[None, :, :] - x[:, None, :]
+    r2 = np.sum(d*d, 2)
+    mask = r2/(soft*soft) > 1e-10
+    r = np.sqrt(np.where(mask, r2, 1.0))
+    rs = r2 + soft*soft
+    f = np.where(mask, m[None, :]/np.sqrt(rs)/r, 0.0)
+    a = G*np.sum(f[:, :, None]*d, 1)
+    phi = -G*np.sum(np.where(mask, m[None, :]*np.log(np.sqrt(rs)), 0.0), 1)
+    return a, phi
+
+
+@pytest.mark.parametrize("n", [400, 1600])
+def test_zero_opening_is_direct_summation_2d(oracle_mod, n):
+    cfg, st = problems.gravity_disk_2d(n)
+    cfg.opening = 1.0e-4
+    cfg.G = 2.0
+    r = oracle_mod.OracleRun(cfg, n)
+    r.set_state(**st)
+    r.initialize()
+    r.evaluate_derivatives()
+    a = r.field("delta velocity")
+    phi = r.field("gravitational potential")
+    ad, pd = direct_2d(st["x"], st["m"], cfg.G, cfg.softening)
+    assert np.allclose(a, ad, rtol=1e-10, atol=1e-10*np.abs(ad).max())
+    assert np.allclose(phi, pd, rtol=1e-10, atol=1e-12*np.abs(pd).max())
+
+
+def test_monopole_error_shrinks_with_opening_2d(oracle_mod):
+    n = 3000
+    cfg, st = problems.gravity_disk_2d(n)
+    ad, _ = direct_2d(st["x"], st["m"], cfg.G, cfg.softening)
+    errs = []
+    for th in (1.0, 0.5, 0.25):
+        cfg.opening = th
+        r = oracle_mod.OracleRun(cfg, n)
+        r.set_state(**st)
+        r.initialize()
+        r.evaluate_derivatives()
+        a = r.field("delta velocity")
+        errs.append(np.median(np.linalg.norm(a - ad, axis=1)/np.linalg.norm(ad, axis=1)))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[1] < 2e-2
 
 
 @pytest.mark.parametrize("n", [300, 1500])
