@@ -1001,7 +1001,7 @@ class Batch {
                              (const RunGrid*)dgrids, (const int*)dbuild, (int)nActive, (const int*)cellStart,
                              (const int*)cellEnd, tlist, tcnt, tcap, dOverflow);
         });
-        hipLaunchKernelGGL(k_candidates_merge, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, (const int*)dbuild,
+        hipLaunchKernelGGL(k_candidates_merge, dim3((unsigned)nActive), dim3(64), 0, stream, f, (const int*)dbuild,
                            (int)nActive, (const uint32_t*)tlist, (const int*)tcnt, tcap, dOverflow);
         HIP_CHECK(hipGetLastError());
         download(hOverflow, dOverflow, 2*sizeof(int));
@@ -1240,7 +1240,9 @@ class Batch {
   // what Integrator::postStateUpdate left: tensor images with grad-h on (SPH asks for boundaries
   // after the Q's copy), zero with it off.  Package calls see boundaries applied.
   void sphPass(bool full, bool withV12 = false, bool stepMid = false) {
-    const int nb = (int)xcd_grid(nActive, kBlock);
+    // lanes per node: the kernel's own choice, from dimension and precision
+    const int64_t lanes = sph_lanes(D, mixed);
+    const int nb = (int)xcd_grid(nActive*lanes, kBlock);
     Phys ph = this->ph;
     ph.qImages = stepMid ? (cfg.gradh_correction ? 1 : 0) : 1;
     const size_t tabBytes = tableBytes();

@@ -374,26 +374,43 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
 }
 
 // Appends each rebuilding node's handed-over entries (tlist) to its candidate list, in
-// ascending entry order (the order they arrived in is not deterministic).  overflow[0]: the
-// candidate capacity needed, overflow[1]: the handed-over capacity needed.
-__global__ void __launch_bounds__(kBlock) k_candidates_merge(Fields f, const int* build, int n, const uint32_t* tlist,
-                                                             const int* tcnt, int tcap, int* overflow) {
-  const int i = blockIdx.x*blockDim.x + threadIdx.x;
+// ascending entry order (the order they arrived in is not deterministic).  One wavefront per
+// node (blockDim 64): lane l ranks entries l, l+64, ... against the node's entries staged 64 at a
+// time in LDS; entry k goes to position c0 + (number of smaller entries), entries being
+// distinct.  overflow[0]: the candidate capacity needed, overflow[1]: the handed-over capacity
+// needed.
+__global__ void __launch_bounds__(64) k_candidates_merge(Fields f, const int* build, int n, const uint32_t* tlist,
+                                                         const int* tcnt, int tcap, int* overflow) {
+  __shared__ uint32_t stage[64];
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
   if (i >= n || !build[f.run[i]]) return;
   const int t = tcnt[i];
   if (t == 0) return;
-  if (t > tcap) { atomicMax(&overflow[1], t); return; }
+  if (t > tcap) {
+    if (lane == 0) atomicMax(&overflow[1], t);
+    return;
+  }
   const size_t ld = (size_t)f.ld;
   const int c0 = f.ccnt[i];
-  if (c0 + t > f.ccap) { atomicMax(&overflow[0], c0 + t); return; }
-  // selection by rank: entry k goes to position c0 + (number of smaller entries); entries are distinct
-  for (int k = 0; k < t; ++k) {
-    const uint32_t e = tlist[ell_at(k, i, ld)];
-    int rank = 0;
-    for (int q = 0; q < t; ++q) rank += tlist[ell_at(q, i, ld)] < e ? 1 : 0;
-    f.cand[ell_at(c0 + rank, i, ld)] = e;
+  if (c0 + t > f.ccap) {
+    if (lane == 0) atomicMax(&overflow[0], c0 + t);
+    return;
   }
-  f.ccnt[i] = c0 + t;
+  for (int kb = 0; kb < t; kb += 64) {
+    const int k = kb + lane;
+    const uint32_t e = k < t ? tlist[ell_at(k, i, ld)] : 0u;
+    int rank = 0;
+    for (int cb = 0; cb < t; cb += 64) {
+      __syncthreads();
+      if (cb + lane < t) stage[lane] = tlist[ell_at(cb + lane, i, ld)];
+      __syncthreads();
+      const int m = min(64, t - cb);
+      for (int q = 0; q < m; ++q) rank += stage[q] < e ? 1 : 0;
+    }
+    if (k < t) f.cand[ell_at(c0 + rank, i, ld)] = e;
+  }
+  if (lane == 0) f.ccnt[i] = c0 + t;
 }
 
 // The exact criterion over each active node's candidates; writes the ELL neighbour list in

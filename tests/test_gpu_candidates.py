@@ -4,7 +4,8 @@ candidate lists it rebuilds only when nodes have moved or grown enough to need i
 Each step's sets must equal, bit for bit, what the oracle's from-scratch ConnectivityMap gives
 on the same positions and H (the state read back just before the step).  The cases cover
 periodic re-imaging of nodes that cross the box, reflecting walls, an open expanding blob
-(growing h) and a fast shear flow, so both list reuse and rebuilds are exercised."""
+(growing h), a fast shear flow and a collapsing sphere whose h varies many-fold (partner-side
+hand-over of candidates), so both list reuse and rebuilds are exercised."""
 import numpy as np
 import pytest
 
@@ -39,6 +40,10 @@ def cases():
     cfg.bc_lo[0] = cfg.bc_hi[0] = abi.SPX_BC_REFLECTING
     st["v"] = np.random.default_rng(9).normal(0.0, 0.5, st["x"].shape)
     out.append(("lattice_2d_reflecting_x_periodic_y", cfg, st, 30))
+    # h varying by far more than 2x (density ~ 1/r): shrunken cells, per-node stencils and entries
+    # handed over by the partner's side, merged in ascending order
+    cfg, st = problems.evrard_3d(3000, precision=abi.SPX_PREC_FP64)
+    out.append(("evrard_3d_collapse", cfg, st, 20))
     return out
 
 
