@@ -558,6 +558,8 @@ def main():
     line["step_roofline"]["frac"] = value/line["step_roofline"]["ceiling_particle_steps_per_s"]
     # graph replays (none here: timed steps run eagerly) and the measured neighbour-group sizes
     line["step_stats"] = run.step_stats()
+    line["device_bytes"] = run.device_bytes()
+    line["device_bytes_per_particle"] = run.device_bytes()/n
     # The timed window may hold few candidate-list rebuilds (none at 20 steps); the rate over whole
     # rebuild cycles: from the step after one rebuild through the step of the second rebuild after it.
     amort = None if args.no_rebuild_window else rebuild_amortised(run, n)

@@ -263,7 +263,7 @@ int spx_run_compatible_energy(spx_run* run, double multiplier);
 /* SpheralConservation totals (SimulationControl/SpheralConservation.py:37-150):
    out[0]=mass, out[1..3]=linear momentum, out[4]=kinetic, out[5]=thermal,
    out[6]=gravitational extra energy (0.5 sum m phi). */
-int spx_run_conservation(spx_run* run, double* out7);
+int spx_run_conservation(spx_run* run, double* out9);
 /* Device time per kernel class (ms), measured with HIP events on the run's stream and
    summed over the steps since timing was enabled; max_entries < 0 enables timing and
    clears the sums.  names: comma-separated labels written to buf. */
@@ -306,13 +306,15 @@ int spx_run_set_step_graphs(spx_run* run, int32_t enable);
 /* Neighbour-group sizes of the filtering sum-density pass, the grad-h passes and the
    compatible-energy update (sizes[0..3), each 4 or 8: how many neighbours' loads a lane
    issues before using them).  Every size gives the same result bits.  By default (sizes null
-   or sizes[0] <= 0) the engine measures: over the first four steps it alternates 4 and 8
-   per pass, timing each, keeps the faster, and measures again once the active particle count
-   has changed by more than 2x. */
+   or sizes[0] <= 0) the engine measures: after two warm-up steps it alternates 4 and 8 per
+   pass over six steps, timing each, moves a pass off its default (4, 8, 8) only when the
+   other size is at least 3 % faster, and measures again once the active particle count has
+   changed by more than 2x. */
 int spx_run_set_group_sizes(spx_run* run, const int32_t* sizes);
-/* out7: graph captures, graph replays, eager steps, group-size measurements, current sizes of
-   the three passes. */
-int spx_run_step_stats(spx_run* run, int64_t* out7);
+/* out9: graph captures, graph replays, eager steps, group-size measurements, current sizes of
+   the three passes, whole-batch reorders, steps whose finished runs retired without a reorder
+   (the last two count batch composition changes; a single run reorders once). */
+int spx_run_step_stats(spx_run* run, int64_t* out9);
 /* HBM copy probe (no counterpart in the reference: a measurement aid).  Copies `bytes`
    between two device buffers `reps` times with a plain streaming kernel and returns the
    achieved rate, read + write bytes per second, in GB/s. */
@@ -355,7 +357,7 @@ int spx_batch_run_status(spx_batch* batch, int32_t run, int32_t* state, double* 
 int spx_batch_get_field(spx_batch* batch, int32_t run, const char* name, double* out,
                         int64_t capacity, int64_t* count);
 /* As spx_run_conservation for one run. */
-int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7);
+int spx_batch_conservation(spx_batch* batch, int32_t run, double* out9);
 /* Occupancy for admission control: particles and run slots held by active and finished runs
    (removed runs' slots and particles count as free: the next admission reclaims them). */
 int spx_batch_usage(spx_batch* batch, int64_t* particles, int32_t* runs);
@@ -367,7 +369,7 @@ int spx_batch_set_step_graphs(spx_batch* batch, int32_t enable);
 /* As spx_run_device_bytes for the batch (all its runs share the arrays sized by its capacity). */
 int spx_batch_device_bytes(spx_batch* batch, int64_t* bytes);
 int spx_batch_set_group_sizes(spx_batch* batch, const int32_t* sizes);
-int spx_batch_step_stats(spx_batch* batch, int64_t* out7);
+int spx_batch_step_stats(spx_batch* batch, int64_t* out9);
 /* As spx_run_kernel_times for the batch's steps. */
 int spx_batch_kernel_times(spx_batch* batch, double* ms, int32_t max_entries,
                            int32_t* n_entries, char* names, int64_t names_cap);

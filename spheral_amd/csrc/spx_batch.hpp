@@ -113,6 +113,10 @@ class Batch {
   Scalars* dsc = nullptr;
   unsigned long long* dext = nullptr;
   int* dstate = nullptr;
+  int* dorder = nullptr;         // place of each active run in the batch (segment order)
+  int* horder = nullptr;
+  int2* dseg = nullptr;          // rebuilding runs' segments: (first position, compacted start)
+  int2* hseg = nullptr;
   int* dinit = nullptr;
   int* dOverflow = nullptr;
   int* dbuild = nullptr;      // per run: 1 if it builds its candidate list this step
@@ -156,6 +160,8 @@ class Batch {
   int64_t nTotal = 0;     // particles resident (active, done, removed-but-not-yet-dropped)
   bool initialized = false, connValid = false, wrapPending = false;
   bool orderDirty = false;  // a run was added, finished or removed since the last reorder
+  long long inPlaceRetirements = 0;   // steps whose finished runs retired without a reorder
+  long long fullReorders = 0;         // reorders of the whole batch (composition changes)
   int64_t candBuilds = 0;   // candidate-list builds, summed over runs
   int64_t npairsCache = -1;
   unsigned long long* pairScratch = nullptr;   // numPairs' partial counts (library-allocated)
@@ -194,6 +200,8 @@ class Batch {
   };
   StepGraph graphs[2];
   int graphNext = 0;
+  std::vector<std::vector<unsigned char>> seenKeys = std::vector<std::vector<unsigned char>>(2);
+  int seenNext = 0;
   bool useGraphs = true;
   int64_t graphCaptures = 0, graphReplays = 0, eagerSteps = 0;
   int* dOverflowG = nullptr;  // the filter's overflow flag in graph-replayed steps
@@ -440,6 +448,12 @@ class Batch {
     dsc = dalloc<Scalars>(maxRuns);
     dext = dalloc<unsigned long long>((size_t)maxRuns*kExt);
     dstate = dalloc<int>(maxRuns);
+    dseg = dalloc<int2>(maxRuns);
+    hseg = halloc<int2>(maxRuns);
+    dorder = dalloc<int>(maxRuns);
+    horder = halloc<int>(maxRuns);
+    for (int r = 0; r < maxRuns; ++r) horder[r] = r;
+    upload(dorder, horder, sizeof(int)*maxRuns);
     dinit = dalloc<int>(maxRuns);
     dOverflow = dalloc<int>(2);
     dOverflowG = dalloc<int>(1);
@@ -618,12 +632,40 @@ class Batch {
     return slot;
   }
 
-  void markDone(int slot) {
+  void markDone(int slot, bool reorder = true) {
     runs[slot].state = RUN_DONE;
     runs[slot].candValid = false;
-    orderDirty = true;
+    if (reorder) orderDirty = true;
     hsc[slot].active = 0;
     upload(&dsc[slot].active, &hsc[slot].active, sizeof(int));
+  }
+
+  // Active runs in the order of their segments.
+  std::vector<int> runsByOffset() const {
+    std::vector<int> a;
+    for (int r = 0; r < maxRuns; ++r)
+      if (runs[r].state == RUN_ACTIVE) a.push_back(r);
+    std::sort(a.begin(), a.end(), [&](int x, int y) { return runs[x].offset < runs[y].offset; });
+    return a;
+  }
+
+  // Runs finished by a step: when their segments are the tail of the active range (the
+  // estimated-remaining-steps order puts the soonest to finish last), the active range just
+  // shrinks and nothing moves; otherwise the next connectivity pass reorders the batch.
+  void retireFinished() {
+    int64_t na = 0;
+    for (int r = 0; r < maxRuns; ++r)
+      if (runs[r].state == RUN_ACTIVE) na += runs[r].n;
+    bool tail = !orderDirty;
+    for (int r = 0; r < maxRuns && tail; ++r)
+      if (runs[r].state == RUN_ACTIVE && runs[r].offset + runs[r].n > na) tail = false;
+    if (tail) {
+      nActive = na;
+      npairsCache = -1;
+      ++inPlaceRetirements;
+    } else {
+      orderDirty = true;
+    }
   }
 
   void removeRun(int slot) {
@@ -781,11 +823,25 @@ class Batch {
     if (endBit > 64) raise(SPX_ERR_CAPACITY, "sort key exceeds 64 bits");
     for (int r = 0; r < maxRuns; ++r) hstate[r] = runs[r].state;
     HIP_CHECK(hipMemcpyAsync(dstate, hstate, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
+    // active runs in decreasing order of their estimated remaining steps: the run expected to
+    // finish first sits last, so it can retire without moving anything (retireFinished)
+    std::vector<int> act;
+    std::vector<double> est(maxRuns, 0.0);
+    for (int r = 0; r < maxRuns; ++r) {
+      if (runs[r].state != RUN_ACTIVE) continue;
+      act.push_back(r);
+      const double dt = hsc[r].dt > 0.0 ? hsc[r].dt : hsc[r].lastDt;
+      est[r] = (runs[r].needInit || !(dt > 0.0)) ? HUGE_VAL : (hsc[r].tEnd - hsc[r].t)/dt;
+    }
+    std::stable_sort(act.begin(), act.end(), [&](int a, int b) { return est[a] > est[b]; });
+    ++fullReorders;
+    for (size_t k = 0; k < act.size(); ++k) horder[act[k]] = (int)k;
+    HIP_CHECK(hipMemcpyAsync(dorder, horder, sizeof(int)*maxRuns, hipMemcpyHostToDevice, stream));
     const int nb = nblocks(nTotal);
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       hipLaunchKernelGGL(k_keys<DD>, dim3(nb), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)dgrids, (const int*)dstate,
-                         (const int*)dbuild, (int)nTotal, wrapPending ? 1 : 0, 1, mortonBits, runBits, keys, vals);
+                         (const int*)dorder, (const int*)dbuild, (int)nTotal, wrapPending ? 1 : 0, 1, mortonBits, runBits, keys, vals);
     });
     HIP_CHECK(hipGetLastError());
     wrapPending = false;
@@ -815,14 +871,16 @@ class Batch {
     std::swap(f.DvDxQ, alt.DvDxQ);
     std::swap(f.oid, oidAlt);
     std::swap(f.run, runAlt);
-    // segments in key order: active runs by slot, then finished runs by slot
+    // segments in key order: active runs by place, then finished runs by slot
     std::vector<int64_t> oldOffset(maxRuns);
     for (int r = 0; r < maxRuns; ++r) oldOffset[r] = runs[r].offset;
     nActive = 0;
     int64_t live = 0;
-    for (int r = 0; r < maxRuns; ++r) {
+    for (int r : act) {
       RunInfo& ri = runs[r];
-      if (ri.state == RUN_ACTIVE) { ri.offset = live; live += ri.n; nActive += ri.n; }
+      ri.offset = live;
+      live += ri.n;
+      nActive += ri.n;
     }
     for (int r = 0; r < maxRuns; ++r) {
       RunInfo& ri = runs[r];
@@ -866,8 +924,8 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       hipLaunchKernelGGL(k_keys<DD>, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)dgrids,
-                         (const int*)dstate, (const int*)dbuild, (int)nTotal, wrapPending ? 1 : 0, 1, mortonBits, runBits,
-                         keys, vals);
+                         (const int*)dstate, (const int*)dorder, (const int*)dbuild, (int)nTotal, wrapPending ? 1 : 0, 1,
+                         mortonBits, runBits, keys, vals);
     });
     HIP_CHECK(hipGetLastError());
     wrapPending = false;
@@ -991,18 +1049,27 @@ class Batch {
                            cellStart, cellEnd);
       });
       HIP_CHECK(hipGetLastError());
+      // the rebuilding runs' segments, in position order, with their compacted starts
+      int nseg = 0;
+      int64_t nBuild = 0;
+      for (int r : runsByOffset()) {
+        if (!hbuild[r]) continue;
+        hseg[nseg++] = make_int2((int)runs[r].offset, (int)nBuild);
+        nBuild += runs[r].n;
+      }
+      HIP_CHECK(hipMemcpyAsync(dseg, hseg, sizeof(int2)*nseg, hipMemcpyHostToDevice, stream));
       bool ok = false;
       for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
         HIP_CHECK(hipMemsetAsync(dOverflow, 0, 2*sizeof(int), stream));
         HIP_CHECK(hipMemsetAsync(tcnt, 0, nActive*sizeof(int), stream));
         launchD([&](auto d) {
           constexpr int DD = decltype(d)::value;
-          hipLaunchKernelGGL(k_candidates<DD>, dim3(xcd_grid(nActive, kBlock)), dim3(kBlock), 0, stream, f, geo,
-                             (const RunGrid*)dgrids, (const int*)dbuild, (int)nActive, (const int*)cellStart,
+          hipLaunchKernelGGL(k_candidates<DD>, dim3(xcd_grid(nBuild, kBlock)), dim3(kBlock), 0, stream, f, geo,
+                             (const RunGrid*)dgrids, (const int2*)dseg, nseg, (int)nBuild, (const int*)cellStart,
                              (const int*)cellEnd, tlist, tcnt, tcap, dOverflow);
         });
-        hipLaunchKernelGGL(k_candidates_merge, dim3((unsigned)nActive), dim3(64), 0, stream, f, (const int*)dbuild,
-                           (int)nActive, (const uint32_t*)tlist, (const int*)tcnt, tcap, dOverflow);
+        hipLaunchKernelGGL(k_candidates_merge, dim3(nblocks(nBuild)), dim3(kBlock), 0, stream, f, (const int2*)dseg,
+                           nseg, (int)nBuild, (const uint32_t*)tlist, (const int*)tcnt, tcap, dOverflow);
         HIP_CHECK(hipGetLastError());
         download(hOverflow, dOverflow, 2*sizeof(int));
         if (hOverflow[0] == 0 && hOverflow[1] == 0) { ok = true; break; }
@@ -1068,7 +1135,8 @@ class Batch {
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
       hipLaunchKernelGGL(k_keys<DD>, dim3(nblocks(nTotal)), dim3(kBlock), 0, stream, f, geo, (const RunGrid*)nullptr,
-                         (const int*)nullptr, (const int*)nullptr, (int)nTotal, 1, 0, 0, 0, keys, vals);
+                         (const int*)nullptr, (const int*)nullptr, (const int*)nullptr, (int)nTotal, 1, 0, 0, 0, keys,
+                         vals);
     });
     HIP_CHECK(hipGetLastError());
     wrapPending = false;
@@ -1192,6 +1260,8 @@ class Batch {
     out[2] = eagerSteps;
     out[3] = tuner.tunings;
     for (int p = 0; p < 3; ++p) out[4 + p] = tuner.size[p];
+    out[7] = fullReorders;
+    out[8] = inPlaceRetirements;
   }
   // Fixed sizes (each 4 or 8), or sizes <= 0: measure.
   void setGroupSizes(const int* sizes) {
@@ -1583,11 +1653,16 @@ class Batch {
   }
   // Host bookkeeping once the step's run scalars are downloaded.
   void endOfStep() {
+    bool finished = false;
     for (int r = 0; r < maxRuns; ++r) {
       if (runs[r].state != RUN_ACTIVE) continue;
       runs[r].stepOut = true;
-      if (retire && reached_end(hsc[r].t, hsc[r].tEnd)) markDone(r);
+      if (retire && reached_end(hsc[r].t, hsc[r].tEnd)) {
+        markDone(r, false);
+        finished = true;
+      }
     }
+    if (finished) retireFinished();
   }
 
   // ---------------------------------------------------------------- graph-replayed steps
@@ -1635,6 +1710,7 @@ class Batch {
       g.graph = nullptr;
       g.key.clear();
     }
+    for (auto& k : seenKeys) k.clear();
   }
   // The eager step's launches after the connectivity build decisions (connectivity records,
   // filter + sum density + dt vote, dt selection, half step, midpoint grad-h, derivatives, full
@@ -1666,6 +1742,26 @@ class Batch {
     StepGraph* g = nullptr;
     for (auto& c : graphs)
       if (c.exec && c.key == key) g = &c;
+    if (!g && std::find(seenKeys.begin(), seenKeys.end(), key) == seenKeys.end()) {
+      // first sighting of this launch configuration: launch it directly and capture it only if
+      // it comes back (a batch whose composition changes every few steps would otherwise pay a
+      // capture and an instantiation per step and replay none of them)
+      seenKeys[seenNext] = std::move(key);
+      seenNext ^= 1;
+      launchSteadyStep();
+      f = f0;
+      for (int a = 0; a < D; ++a) std::swap(f.v[a], f.v1[a]);
+      ++eagerSteps;
+      connValid = true;
+      npairsCache = -1;
+      wrapPending = true;
+      HIP_CHECK(hipMemcpyAsync(hOverflow, dOverflowG, sizeof(int), hipMemcpyDeviceToHost, stream));
+      download(hsc, dsc, sizeof(Scalars)*maxRuns);
+      if (*hOverflow != 0) raise(SPX_ERR_STATE, "neighbour list overflow in a steady step");
+      tunerStepDone();
+      endOfStep();
+      return;
+    }
     if (!g) {
       StepGraph& c = graphs[graphNext];
       graphNext ^= 1;

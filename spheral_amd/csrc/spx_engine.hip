@@ -411,12 +411,12 @@ int spx_run_compatible_energy(spx_run* run, double multiplier) {
   return guarded([&] { RUN_STATE(run); run->impl->compatibleEnergy(multiplier); });
 }
 
-int spx_run_conservation(spx_run* run, double* out7) {
+int spx_run_conservation(spx_run* run, double* out9) {
   RUN_OR_FAIL(run);
   return guarded([&] {
-    if (!out7) spx::raise(SPX_ERR_ARG, "null argument");
+    if (!out9) spx::raise(SPX_ERR_ARG, "null argument");
     RUN_STATE(run);
-    run->impl->conservation(0, out7);
+    run->impl->conservation(0, out9);
   });
 }
 
@@ -484,11 +484,11 @@ int spx_run_set_group_sizes(spx_run* run, const int32_t* sizes) {
   });
 }
 
-int spx_run_step_stats(spx_run* run, int64_t* out7) {
+int spx_run_step_stats(spx_run* run, int64_t* out9) {
   RUN_OR_FAIL(run);
   return guarded([&] {
-    if (!out7) spx::raise(SPX_ERR_ARG, "null argument");
-    run->impl->stepStats(out7);
+    if (!out9) spx::raise(SPX_ERR_ARG, "null argument");
+    run->impl->stepStats(out9);
   });
 }
 
@@ -568,11 +568,11 @@ int spx_batch_get_field(spx_batch* batch, int32_t run, const char* name, double*
   });
 }
 
-int spx_batch_conservation(spx_batch* batch, int32_t run, double* out7) {
+int spx_batch_conservation(spx_batch* batch, int32_t run, double* out9) {
   BATCH_OR_FAIL(batch);
   return guarded([&] {
-    if (!out7) spx::raise(SPX_ERR_ARG, "null argument");
-    batch->impl->conservation(run, out7);
+    if (!out9) spx::raise(SPX_ERR_ARG, "null argument");
+    batch->impl->conservation(run, out9);
   });
 }
 
@@ -616,11 +616,11 @@ int spx_batch_set_group_sizes(spx_batch* batch, const int32_t* sizes) {
   });
 }
 
-int spx_batch_step_stats(spx_batch* batch, int64_t* out7) {
+int spx_batch_step_stats(spx_batch* batch, int64_t* out9) {
   BATCH_OR_FAIL(batch);
   return guarded([&] {
-    if (!out7) spx::raise(SPX_ERR_ARG, "null argument");
-    batch->impl->stepStats(out7);
+    if (!out9) spx::raise(SPX_ERR_ARG, "null argument");
+    batch->impl->stepStats(out9);
   });
 }
 

@@ -54,8 +54,8 @@ __device__ __forceinline__ int cell_coord(const RunGrid& gr, int a, double x) {
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, const RunGrid* grids, const int* rstate,
-                                                 const int* build, int n, int enforce, int withKeys, int mortonBits,
-                                                 int runBits, uint64_t* keys, uint32_t* vals) {
+                                                 const int* rorder, const int* build, int n, int enforce, int withKeys,
+                                                 int mortonBits, int runBits, uint64_t* keys, uint32_t* vals) {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int r = f.run[i];
@@ -84,10 +84,11 @@ __global__ void __launch_bounds__(kBlock) k_keys(Fields f, Geo g, const RunGrid*
     if (withKeys && build[r]) key |= spread_bits((uint64_t)cell_coord<D>(grids[r], a, xa), D) << a;
   }
   if (withKeys) {
-    // layout: [rank | run | Morton], rank 0 active, 1 done; removed runs take the maximum key
+    // layout: [rank | run | Morton], rank 0 active (runs by their place rorder), 1 done (by
+    // slot); removed runs take the maximum key
     const int st = rstate[r];
     const int shift = mortonBits + runBits;
-    keys[i] = st == RUN_ACTIVE ? (((uint64_t)r << mortonBits) | key)
+    keys[i] = st == RUN_ACTIVE ? (((uint64_t)rorder[r] << mortonBits) | key)
             : (st == RUN_DONE ? ((1ull << shift) | ((uint64_t)r << mortonBits)) : ~0ull);
     vals[i] = (uint32_t)i;
   }
@@ -276,14 +277,28 @@ __device__ __forceinline__ int inverse_image(const Geo& g, int code) {
 // A j whose own support does not reach i cannot find the pair itself, so i hands it the
 // mirrored entry (tlist, merged by k_candidates_merge).  Together each list holds exactly the
 // j with |Hi rij| or |Hj rij| within the widened extent.
+// The rebuilding runs' segments, listed as (first position, first index in the compacted
+// range [0, nBuild)) in increasing order of both: compacted index k -> particle position.
+__device__ __forceinline__ int build_pos(const int2* seg, int nseg, int k) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg[mid].y <= k) lo = mid; else hi = mid - 1;
+  }
+  return seg[lo].x + (k - seg[lo].y);
+}
+
+// Launched over the rebuilding runs' particles only (seg, nBuild): a step that rebuilds a few
+// small runs of a large batch neither walks the others nor leaves the work to the one XCD whose
+// index range holds those runs.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const RunGrid* grids, const int* build, int n,
-                                                       const int* cstart, const int* cend, uint32_t* tlist, int* tcnt,
-                                                       int tcap, int* overflow) {
-  const int i = xcd_block()*blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const RunGrid* grids, const int2* seg, int nseg,
+                                                       int nBuild, const int* cstart, const int* cend, uint32_t* tlist,
+                                                       int* tcnt, int tcap, int* overflow) {
+  const int k = xcd_block()*blockDim.x + threadIdx.x;
+  if (k >= nBuild) return;
+  const int i = build_pos(seg, nseg, k);
   const int r = f.run[i];
-  if (!build[r]) return;
   const RunGrid gr = grids[r];
   double xi[D];
   int ci[3] = {0, 0, 0};
@@ -374,43 +389,50 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
 }
 
 // Appends each rebuilding node's handed-over entries (tlist) to its candidate list, in
-// ascending entry order (the order they arrived in is not deterministic).  One wavefront per
-// node (blockDim 64): lane l ranks entries l, l+64, ... against the node's entries staged 64 at a
-// time in LDS; entry k goes to position c0 + (number of smaller entries), entries being
-// distinct.  overflow[0]: the candidate capacity needed, overflow[1]: the handed-over capacity
-// needed.
-__global__ void __launch_bounds__(64) k_candidates_merge(Fields f, const int* build, int n, const uint32_t* tlist,
-                                                         const int* tcnt, int tcap, int* overflow) {
-  __shared__ uint32_t stage[64];
-  const int i = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (i >= n || !build[f.run[i]]) return;
-  const int t = tcnt[i];
-  if (t == 0) return;
-  if (t > tcap) {
-    if (lane == 0) atomicMax(&overflow[1], t);
-    return;
+// ascending entry order (the order they arrived in is not deterministic).  Over the rebuilding
+// runs' particles (seg, nBuild); each wavefront takes its 64 nodes in turn, only those that
+// received entries: lane l ranks entries l, l+64, ...
+// against all the node's entries, read 64 at a time and broadcast lane by lane
+// (v_readlane); entry k goes to c0 + (number of smaller entries), entries being distinct.
+// Batches whose h is uniform hand nothing over, so most waves only read their counts.
+// overflow[0]: the candidate capacity needed, overflow[1]: the handed-over capacity needed.
+__global__ void __launch_bounds__(kBlock) k_candidates_merge(Fields f, const int2* seg, int nseg, int nBuild,
+                                                             const uint32_t* tlist, const int* tcnt, int tcap,
+                                                             int* overflow) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x*blockDim.x + threadIdx.x;
+  const int me = k < nBuild ? build_pos(seg, nseg, k) : 0;
+  int t = 0, c0 = 0;
+  bool ok = false;
+  if (k < nBuild) {
+    t = tcnt[me];
+    if (t > tcap) {
+      atomicMax(&overflow[1], t);
+    } else if (t > 0) {
+      c0 = f.ccnt[me];
+      if (c0 + t > f.ccap) atomicMax(&overflow[0], c0 + t);
+      else ok = true;
+    }
   }
   const size_t ld = (size_t)f.ld;
-  const int c0 = f.ccnt[i];
-  if (c0 + t > f.ccap) {
-    if (lane == 0) atomicMax(&overflow[0], c0 + t);
-    return;
-  }
-  for (int kb = 0; kb < t; kb += 64) {
-    const int k = kb + lane;
-    const uint32_t e = k < t ? tlist[ell_at(k, i, ld)] : 0u;
-    int rank = 0;
-    for (int cb = 0; cb < t; cb += 64) {
-      __syncthreads();
-      if (cb + lane < t) stage[lane] = tlist[ell_at(cb + lane, i, ld)];
-      __syncthreads();
-      const int m = min(64, t - cb);
-      for (int q = 0; q < m; ++q) rank += stage[q] < e ? 1 : 0;
+  unsigned long long todo = __ballot(ok);
+  while (todo) {
+    const int l = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    const int i = __shfl(me, l), ti = __shfl(t, l), ci = __shfl(c0, l);
+    for (int kb = 0; kb < ti; kb += 64) {
+      const int k = kb + lane;
+      const uint32_t e = k < ti ? tlist[ell_at(k, i, ld)] : 0u;
+      int rank = 0;
+      for (int cb = 0; cb < ti; cb += 64) {
+        const int v = cb + lane < ti ? (int)tlist[ell_at(cb + lane, i, ld)] : -1;
+        const int m = min(64, ti - cb);
+        for (int q = 0; q < m; ++q) rank += (uint32_t)__builtin_amdgcn_readlane(v, q) < e ? 1 : 0;
+      }
+      if (k < ti) f.cand[ell_at(ci + rank, i, ld)] = e;
     }
-    if (k < t) f.cand[ell_at(c0 + rank, i, ld)] = e;
   }
-  if (lane == 0) f.ccnt[i] = c0 + t;
+  if (ok) f.ccnt[me] = c0 + t;
 }
 
 // The exact criterion over each active node's candidates; writes the ELL neighbour list in

@@ -149,7 +149,7 @@ FIELD_KIND = {
 
 
 STEP_STATS = ("graph_captures", "graph_replays", "eager_steps", "group_tunings", "group_filter", "group_gradh",
-              "group_update")
+              "group_update", "full_reorders", "in_place_retirements")
 
 
 def _sizes(sizes):
@@ -160,7 +160,7 @@ def _sizes(sizes):
 
 
 def _stats(fn, h):
-    out = (C.c_int64*7)()
+    out = (C.c_int64*len(STEP_STATS))()
     _check(fn(h, out))
     return dict(zip(STEP_STATS, list(out)))
 

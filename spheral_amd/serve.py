@@ -4,7 +4,8 @@ Each problem type (one engine configuration: dimension, kernel, EOS, boundaries)
 resident batch (spx_batch) and one host worker thread.  A lane is given an HBM budget in
 bytes: it measures the device bytes per particle of its configuration on a probe run (state,
 work arrays, neighbour and candidate lists, all of which the engine sizes by the batch's
-particle capacity), and sizes its batch to the largest capacity that fits the budget.  A worker
+particle capacity), and sizes its batch to the capacity that fits the budget with HEADROOM for
+the lists' growth beyond the probe's.  A worker
 admits queued requests while that capacity and the run slots allow, steps every active run of
 its batch once, hands back the state of runs that reached their end time and frees their slots.
 Requests that arrive while others are on the GPU join the next step; batches of different
@@ -46,6 +47,12 @@ class Request:
 
 PROBE_STEPS = 12
 PROBE_MAX_STEPS = 200000
+# A lane's batch holds budget / (bytes per particle x HEADROOM) particles.  The probe measures one
+# typical request, but a batch's neighbour and candidate lists are sized by the most crowded node
+# over all of its runs and grow in steps of 1.25x, so a batch serving many runs reaches a growth
+# step or so beyond the probe: on the 500-request Sod 2-D / Sod 3-D / Gresho trace the lanes'
+# peaks were 1.21x, 1.19x and 1.36x the probes' bytes per particle.
+HEADROOM = 1.5
 
 
 def bytes_per_particle(cfg, probe_state, t_end=None, device=0):
@@ -81,8 +88,7 @@ class _Lane:
         self.kind = kind
         self.hbm_bytes = int(hbm_bytes)
         self.bytes_per_particle = bytes_per_particle(cfg, probe_state, probe_t_end, device)
-        # the lists may grow past the probe's (denser regions): keep 10 % of the budget in hand
-        self.capacity = int(0.9*self.hbm_bytes/self.bytes_per_particle)
+        self.capacity = int(self.hbm_bytes/(self.bytes_per_particle*HEADROOM))
         if self.capacity < 1:
             raise ValueError(f"{kind}: an HBM budget of {hbm_bytes} bytes holds no particle")
         self.max_runs = int(max_runs)

@@ -54,6 +54,24 @@ def test_batch_2d_runs_equal_alone(gpu, precision):
         assert_same_as_alone(gpu, b, rid, cfg, st, te)
 
 
+def test_finished_runs_retire_in_place(gpu):
+    """Runs finishing at different steps: after the first reorder the batch orders its runs by
+    estimated remaining steps, so later finishers are the tail of the active range and retire
+    without moving the others.  Every run still equals itself alone."""
+    sizes, t_ends = [1200, 2500, 1800, 3000, 900, 2100], [0.01, 0.03, 0.02, 0.04, 0.015, 0.025]
+    print(f"[case] batch retire in place sizes={sizes}", flush=True)
+    cfg, _ = problems.kelvin_helmholtz_2d(sizes[0], precision=abi.SPX_PREC_MIXED)
+    states = [problems.kelvin_helmholtz_2d(n, precision=abi.SPX_PREC_MIXED)[1] for n in sizes]
+    b = gpu.Batch(cfg, capacity=sum(len(s["m"]) for s in states), max_runs=6)
+    ids = [b.add_run(s, te) for s, te in zip(states, t_ends)]
+    run_to_completion(b)
+    st = b.step_stats()
+    print(f"[case] reorders {st['full_reorders']}, in-place retirements {st['in_place_retirements']}", flush=True)
+    assert st["in_place_retirements"] >= 1
+    for rid, s, te in zip(ids, states, t_ends):
+        assert_same_as_alone(gpu, b, rid, cfg, s, te)
+
+
 def test_batch_3d_runs_equal_alone(gpu):
     print("[case] batch tg3d 16^3 + 12^3", flush=True)
     cfg, s16 = problems.taylor_green_3d(16)

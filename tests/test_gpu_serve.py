@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 def budget_for(cfg, probe, particles):
     """The HBM budget (bytes) of a lane that holds `particles` particles of this configuration."""
-    return int(serve.bytes_per_particle(cfg, probe)*(particles + 0.5)/0.9)
+    return int(serve.bytes_per_particle(cfg, probe)*serve.HEADROOM*(particles + 0.5))
 
 
 def test_served_results_equal_alone(gpu):
@@ -81,7 +81,7 @@ def test_probe_to_end_time_bounds_the_lane(gpu):
     short = serve.bytes_per_particle(cfg, probe)
     full = serve.bytes_per_particle(cfg, probe, 0.15)
     assert full >= short
-    budget = int(full*(4000 + 0.5)/0.9)
+    budget = int(full*serve.HEADROOM*(4000 + 0.5))
     srv = serve.Server({"sod2d": (cfg, budget, 4, probe, 0.15)})
     lane = srv.lanes["sod2d"]
     assert lane.capacity == 4000
