@@ -175,12 +175,44 @@ __global__ void __launch_bounds__(kBlock) k_tree_moments(TreeView t, int lvl, do
 }
 
 __global__ void __launch_bounds__(kBlock) k_tree_gather(double* const* xs, const double* m, const uint32_t* perm, int n,
-                                                        double* tx0, double* tx1, double* tx2, double* tm) {
+                                                        double* tx0, double* tx1, double* tx2, double* tm, double2* tp) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint32_t q = perm[p];
-  tx0[p] = xs[0][q]; tx1[p] = xs[1][q]; tx2[p] = xs[2][q];
-  tm[p] = m[q];
+  const double x = xs[0][q], y = xs[1][q], z = xs[2][q], mq = m[q];
+  tx0[p] = x; tx1[p] = y; tx2[p] = z;
+  tm[p] = mq;
+  // the walk's leaf record: (x, y), (z, m) in one 32-byte block
+  tp[2*(size_t)p] = make_double2(x, y);
+  tp[2*(size_t)p + 1] = make_double2(z, mq);
+}
+
+// A cell as the walk reads it: one 64-byte record, (x, y), (z, M) of the centre of mass, the
+// opening radius^2 l^2/theta^2 + |xcm - centre|^2 for the walk's opening angle, and (first,
+// last, skip, level) -- one cache line per visited cell instead of one per array.
+struct alignas(64) TreeCell {
+  double x, y, z, M;
+  double open, pad;
+  int first, last, skip, level;
+};
+static_assert(sizeof(TreeCell) == 64, "TreeCell is one 64-byte record");
+
+__global__ void __launch_bounds__(kBlock) k_tree_pack(TreeView v, const double* cs2, double open2, TreeCell* cells) {
+  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  if (c >= v.ncells) return;
+  const int lvl = v.level[c];
+  TreeCell t;
+  t.x = v.cx[0][c];
+  t.y = v.cx[1][c];
+  t.z = v.cx[2][c];
+  t.M = v.M[c];
+  t.open = cs2[lvl]/open2 + v.rc2[c];
+  t.pad = 0.0;
+  t.first = v.first[c];
+  t.last = v.last[c];
+  t.skip = v.skip[c];
+  t.level = lvl;
+  cells[c] = t;
 }
 
 // Lanes that walk the tree together (64: the wavefront; 16, 8, 4: aligned lane groups).
@@ -219,11 +251,8 @@ __device__ __forceinline__ int group_min(int v) {
 template <int GROUP, bool COUNT = false, int DIM = 3>
 __global__ void __launch_bounds__(kBlock) k_tree_walk(
     const double* __restrict__ tx0, const double* __restrict__ tx1, const double* __restrict__ tx2,
-    const double* __restrict__ tm, const double* __restrict__ cx0, const double* __restrict__ cx1,
-    const double* __restrict__ cx2, const double* __restrict__ cM, const double* __restrict__ crc2,
-    const int* __restrict__ cfirst, const int* __restrict__ clast, const int* __restrict__ cskip,
-    const unsigned char* __restrict__ clevel, const double* __restrict__ ccs2, int ncells, int npts,
-    const uint32_t* __restrict__ perm, double soft2, double open2, double soft, double G, double* __restrict__ acc0,
+    const double2* __restrict__ tp, const TreeCell* __restrict__ cells, int ncells, int npts,
+    const uint32_t* __restrict__ perm, double soft2, double soft, double G, double* __restrict__ acc0,
     double* __restrict__ acc1, double* __restrict__ acc2, double* __restrict__ pot, double* __restrict__ blockMin,
     unsigned long long* __restrict__ counts = nullptr) {
   const int p = blockIdx.x*blockDim.x + threadIdx.x;
@@ -246,10 +275,12 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
       if (__all(c >= ncells)) break;              // every group is done
       if (c >= ncells) continue;                  // this group is done
     }
-    // the cell's data, read at one address per group
-    const int lvl = clevel[c];
-    const double ccx = cx0[c], ccy = cx1[c], ccz = cx2[c], Mc = cM[c], open = ccs2[lvl]/open2 + crc2[c];
-    const int q0 = cfirst[c], q1 = clast[c], sk = cskip[c];
+    // the cell's record, read at one address per group
+    const double2* cr = reinterpret_cast<const double2*>(cells + c);
+    const double2 c01 = cr[0], c23 = cr[1], co = cr[2];
+    const int4 ci = *reinterpret_cast<const int4*>(cr + 3);
+    const double ccx = c01.x, ccy = c01.y, ccz = c23.x, Mc = c23.y, open = co.x;
+    const int q0 = ci.x, q1 = ci.y, sk = ci.z, lvl = ci.w;
     if (next != c) continue;
     const double dx = ccx - xi, dy = ccy - yi, dz = ccz - zi;
     const double r2 = dx*dx + dy*dy + dz*dz;
@@ -268,12 +299,13 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
       next = sk;
     } else if (q0 == q1 || lvl == kTreeBits) {
       for (int q = q0; q <= q1; ++q) {
-        const double ex = tx0[q] - xi, ey = tx1[q] - yi, ez = tx2[q] - zi;
+        const double2 pa = tp[2*(size_t)q], pb = tp[2*(size_t)q + 1];
+        const double ex = pa.x - xi, ey = pa.y - yi, ez = pb.x - zi;
         const double q2 = ex*ex + ey*ey + ez*ez;
         if (q2/soft2 > 1.0e-10) {
           const double inv = rsqrt(q2);
           const double rs = q2 + soft2;
-          const double mq = tm[q];
+          const double mq = pb.y;
           const double ri = rsqrt(rs);
           const double f = DIM == 3 ? mq*(ri*ri) : mq*ri;
           ax += f*(ex*inv); ay += f*(ey*inv); az += f*(ez*inv);
@@ -333,6 +365,9 @@ class GravityTree {
   uint64_t *codes = nullptr, *codesSorted = nullptr;
   uint32_t *idx = nullptr, *perm = nullptr;
   double *tx[3] = {nullptr, nullptr, nullptr}, *tm = nullptr;
+  double2* tp = nullptr;            // tree-order (x, y), (z, m) records of the walk's leaves
+  TreeCell* cells = nullptr;        // the walk's cell records
+  double packedOpen2 = -1.0;        // opening^2 the cell records were packed for (< 0: stale)
   int *nheads = nullptr, *cstart = nullptr;
   unsigned char* lstart = nullptr;
   void* tmp = nullptr;
@@ -381,6 +416,7 @@ class GravityTree {
     perm = alloc<uint32_t>(n);
     for (int a = 0; a < 3; ++a) tx[a] = alloc<double>(n);
     tm = alloc<double>(n);
+    tp = alloc<double2>(2*(size_t)n);
     nheads = alloc<int>(n + 1);
     cstart = alloc<int>(n + 1);
     lstart = alloc<unsigned char>(n);
@@ -413,6 +449,7 @@ class GravityTree {
     M = alloc<double>(cap);
     for (int a = 0; a < 3; ++a) cx[a] = alloc<double>(cap);
     rc2 = alloc<double>(cap);
+    cells = alloc<TreeCell>(cap);
     cellCap = cap;
   }
 
@@ -434,7 +471,9 @@ class GravityTree {
     HIP_CHECK(hipGetLastError());
     size_t t = tmpBytes;
     HIP_CHECK(rocprim::radix_sort_pairs(tmp, t, codes, codesSorted, idx, perm, (size_t)nn, 0, 60, stream));
-    hipLaunchKernelGGL(k_tree_gather, dim3(nb), dim3(kBlock), 0, stream, xsPtrs, m, perm, nn, tx[0], tx[1], tx[2], tm);
+    hipLaunchKernelGGL(k_tree_gather, dim3(nb), dim3(kBlock), 0, stream, xsPtrs, m, perm, nn, tx[0], tx[1], tx[2], tm,
+                       tp);
+    packedOpen2 = -1.0;
     hipLaunchKernelGGL(k_tree_heads, dim3(nb), dim3(kBlock), 0, stream, codesSorted, nn, nheads, lstart);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemsetAsync(nheads + nn, 0, sizeof(int), stream));
@@ -468,14 +507,24 @@ class GravityTree {
     HIP_CHECK(hipStreamSynchronize(stream));
   }
 
+  // The walk's cell records for this opening angle (once per tree and angle).
+  void pack(double opening) {
+    const double open2 = opening*opening;
+    if (packedOpen2 == open2) return;
+    const int cb = (int)((view.ncells + kBlock - 1)/kBlock);
+    hipLaunchKernelGGL(k_tree_pack, dim3(std::max(cb, 1)), dim3(kBlock), 0, stream, view, (const double*)cs2, open2, cells);
+    HIP_CHECK(hipGetLastError());
+    packedOpen2 = open2;
+  }
+
   void evaluate(double* const* acc, double* pot, int nn, double G, double soft, double opening, Scalars* sc,
                 double ftimestep) {
     const int nb = (int)((nn + kBlock - 1)/kBlock);
     double* az = D == 3 ? acc[2] : sink;
+    pack(opening);
     auto walk = D == 3 ? k_tree_walk<kTreeGroup, false, 3> : k_tree_walk<kTreeGroup, false, 2>;
-    hipLaunchKernelGGL(walk, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1], view.tx[2], view.tm,
-                       view.cx[0], view.cx[1], view.cx[2], view.M, view.rc2, view.first, view.last, view.skip, view.level,
-                       (const double*)cs2, view.ncells, view.n, (const uint32_t*)perm, soft*soft, opening*opening, soft, G,
+    hipLaunchKernelGGL(walk, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1], view.tx[2], (const double2*)tp,
+                       (const TreeCell*)cells, view.ncells, view.n, (const uint32_t*)perm, soft*soft, soft, G,
                        acc[0], acc[1], az, pot, part, (unsigned long long*)nullptr);
     hipLaunchKernelGGL(k_tree_dt, dim3(1), dim3(kBlock), 0, stream, part, nb, sc, G, ftimestep);
     HIP_CHECK(hipGetLastError());
@@ -493,11 +542,11 @@ class GravityTree {
     HIP_CHECK(hipMalloc(&scratch, (size_t)(4*(size_t)std::max(nn, 1) + nb)*sizeof(double)));
     HIP_CHECK(hipMalloc(&counts, 2*(size_t)std::max(nn, 1)*sizeof(unsigned long long)));
     double* a = scratch;
+    pack(opening);
     auto walk = D == 3 ? k_tree_walk<kTreeGroup, true, 3> : k_tree_walk<kTreeGroup, true, 2>;
-    hipLaunchKernelGGL(walk, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1],
-                       view.tx[2], view.tm, view.cx[0], view.cx[1], view.cx[2], view.M, view.rc2, view.first, view.last,
-                       view.skip, view.level, (const double*)cs2, view.ncells, view.n, (const uint32_t*)perm, soft*soft,
-                       opening*opening, soft, 1.0, a, a + nn, a + 2*(size_t)nn, a + 3*(size_t)nn, a + 4*(size_t)nn, counts);
+    hipLaunchKernelGGL(walk, dim3(nb), dim3(kBlock), 0, stream, view.tx[0], view.tx[1], view.tx[2], (const double2*)tp,
+                       (const TreeCell*)cells, view.ncells, view.n, (const uint32_t*)perm, soft*soft, soft, 1.0, a, a + nn,
+                       a + 2*(size_t)nn, a + 3*(size_t)nn, a + 4*(size_t)nn, counts);
     const hipError_t e = hipGetLastError();
     std::vector<unsigned long long> h(2*(size_t)nn);
     hipError_t e2 = hipSuccess;
