@@ -730,6 +730,8 @@ class Batch {
       gr.cellBase = (int)base;
       gr.hinvMin = 0.0;
       gr.ext2c = 0.0;
+      gr.support = 0.0;
+      gr.levels = 1;
       if (runs[r].state != RUN_ACTIVE || !hbuild[r]) continue;
       const double hinvMin = ord_value(hext[r*kExt]), hinvMax = ord_value(hext[r*kExt + 7]);
       if (!(hinvMin > 0.0) || !std::isfinite(hinvMin)) raise(SPX_ERR_NUMERIC, "non-positive or non-finite H on the device");
@@ -737,10 +739,19 @@ class Batch {
       const double extc = cfg.neighbor_extent*(1.0 + runs[r].skin);
       gr.ext2c = extc*extc;
       const double support = extc/hinvMin*(1.0 + 1.0e-9);
-      // where h varies, cells shrink toward the smaller supports (by up to 4x, while the grid
-      // stays within ~8 cells per node) and each node scans the cells its own support reaches
-      int F = 1;
-      if (hinvMax >= 2.0*hinvMin) F = (int)std::min(4.0, std::floor(hinvMax/hinvMin));
+      gr.support = support;
+      // where h varies by 2x or more: with open boundaries, cell levels down to the smallest
+      // support (up to kCellLevels, while the finest grid stays within ~8 cells per node); with
+      // periodic or reflecting axes, cells shrink toward the smaller supports (by up to 4x, same
+      // bound) and each node scans the cells its own support reaches
+      bool open = true;
+      for (int a = 0; a < D; ++a) open &= geo.bc_lo[a] != BC_PERIODIC && geo.bc_lo[a] != BC_REFLECTING &&
+                                          geo.bc_hi[a] != BC_PERIODIC && geo.bc_hi[a] != BC_REFLECTING;
+      int F = 1, lev = 1;
+      if (hinvMax >= 2.0*hinvMin) {
+        if (open) lev = std::min(kCellLevels, 1 + (int)std::floor(std::log2(hinvMax/hinvMin)));
+        else F = (int)std::min(4.0, std::floor(hinvMax/hinvMin));
+      }
       int64_t total = 1;
       for (;;) {
         total = 1;
@@ -750,7 +761,7 @@ class Batch {
           if (!std::isfinite(lo) || !std::isfinite(hi)) raise(SPX_ERR_NUMERIC, "non-finite position on the device");
           const int blo = geo.bc_lo[a], bhi = geo.bc_hi[a];
           const double L = cfg.xmax[a] - cfg.xmin[a];
-          const double target = support/F;
+          const double target = lev > 1 ? support/(double)(2 << (lev - 1)) : support/F;
           int64_t nc;
           double cell, origin;
           if (blo == BC_PERIODIC) {
@@ -791,10 +802,24 @@ class Batch {
           while (((int64_t)1 << b) < nc) ++b;
           runBitsAxis = std::max(runBitsAxis, b);
         }
+        if (lev > 1) {
+          // every level's table: level-l cells are 2^(lev-1-l) finest cells per axis
+          int64_t all = 0;
+          for (int l = 0; l < lev; ++l) {
+            int64_t t = 1;
+            for (int a = 0; a < D; ++a) t *= ((int64_t)(gr.nc[a] - 1) >> (lev - 1 - l)) + 1;
+            gr.levelBase[l] = (int)(base + all);
+            all += t;
+          }
+          total = all;
+        }
         if (F > 1 && total > 8*runs[r].n + 4096) { --F; continue; }
+        if (lev > 1 && total > 8*runs[r].n + 4096) { --lev; continue; }
         bits = std::max(bits, runBitsAxis);
         break;
       }
+      gr.levels = lev;
+      if (lev <= 1) gr.levelBase[0] = (int)base;
       base += total;
       if (base > ((int64_t)1 << 28)) raise(SPX_ERR_CAPACITY, "cell grids too large (support radius tiny relative to the domain)");
     }

@@ -107,13 +107,19 @@ struct Geo {
 
 // Cell grid of one run for the current step.  Cells of run r occupy the global
 // cell-index range [cellBase, cellBase + nc0*nc1*nc2).
+// Cell levels of a run whose h varies (open boundaries): level-l cells are 2^(levels-1-l)
+// finest cells per axis, l < kCellLevels.
+constexpr int kCellLevels = 8;
 struct RunGrid {
   double origin[3];
-  double inv_cell[3];
+  double inv_cell[3];  // of the finest level when levels > 1
   double hinvMin;      // smallest h^-1 of the run's nodes: bounds every node's support radius
   double ext2c;        // candidate-list criterion: ext^2 (1 + skin)^2
-  int nc[3];
-  int cellBase;
+  double support;      // the largest widened support radius
+  int nc[3];           // cells per axis (of the finest level when levels > 1)
+  int cellBase;        // first cell of the run's table (of level 0 when levels > 1)
+  int levels;          // > 1: cell levels, level l at least support/2^(l+1) wide
+  int levelBase[kCellLevels];   // first cell of each level's table
 };
 
 // Physics constants of the run.

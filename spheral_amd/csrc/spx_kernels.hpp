@@ -137,6 +137,21 @@ __device__ __forceinline__ void unspread(uint64_t key, int mortonBits, int* c) {
 // Cell ranges of the sorted keys of the runs that build candidate lists: start[cell],
 // end[cell] (both 0 when empty).  The low mortonBits of a key are the cell's Morton code;
 // a run's particles are one contiguous segment (run[] tells where it ends).
+// With cell levels, each level's table: a level-l cell is the key prefix above the low
+// D*(levels-1-l) bits, one contiguous range of the sorted particles.
+template <int D>
+__device__ __forceinline__ int level_cells(const RunGrid& g, int l, int* ncl) {
+  const int sh = g.levels - 1 - l;
+  int t = 1;
+  for (int a = 0; a < D; ++a) { ncl[a] = ((g.nc[a] - 1) >> sh) + 1; t *= ncl[a]; }
+  return t;
+}
+template <int D>
+__device__ __forceinline__ int lin_level_cell(const RunGrid& g, int l, const int* ncl, const int* c) {
+  int q = c[D - 1];
+  for (int a = D - 2; a >= 0; --a) q = q*ncl[a] + c[a];
+  return g.levelBase[l] + q;
+}
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, const int32_t* run, int n,
                                                         const RunGrid* grids, const int* build, int mortonBits,
@@ -147,9 +162,23 @@ __global__ void __launch_bounds__(kBlock) k_cell_bounds(const uint64_t* keys, co
   if (!build[r]) return;
   const uint64_t mask = (1ull << mortonBits) - 1ull;
   const uint64_t k = keys[p] & mask;
+  const RunGrid& gr = grids[r];
+  const bool first = p == 0 || run[p - 1] != r, last = p == n - 1 || run[p + 1] != r;
   int c[3];
-  if (p == 0 || run[p - 1] != r || (keys[p - 1] & mask) != k) { unspread<D>(k, mortonBits, c); start[lin_cell<D>(grids[r], c)] = p; }
-  if (p == n - 1 || run[p + 1] != r || (keys[p + 1] & mask) != k) { unspread<D>(k, mortonBits, c); end[lin_cell<D>(grids[r], c)] = p + 1; }
+  if (gr.levels <= 1) {
+    if (first || (keys[p - 1] & mask) != k) { unspread<D>(k, mortonBits, c); start[lin_cell<D>(gr, c)] = p; }
+    if (last || (keys[p + 1] & mask) != k) { unspread<D>(k, mortonBits, c); end[lin_cell<D>(gr, c)] = p + 1; }
+    return;
+  }
+  const uint64_t kp = first ? 0 : (keys[p - 1] & mask), kn = last ? 0 : (keys[p + 1] & mask);
+  for (int l = 0; l < gr.levels; ++l) {
+    const int sh = D*(gr.levels - 1 - l);
+    const uint64_t q = k >> sh;
+    int ncl[3] = {1, 1, 1};
+    level_cells<D>(gr, l, ncl);
+    if (first || (kp >> sh) != q) { unspread<D>(q, mortonBits, c); start[lin_level_cell<D>(gr, l, ncl, c)] = p; }
+    if (last || (kn >> sh) != q) { unspread<D>(q, mortonBits, c); end[lin_level_cell<D>(gr, l, ncl, c)] = p + 1; }
+  }
 }
 
 // Sort keys of the particles of the runs that rebuild (build[run] != 0), compacted: run r's
@@ -271,33 +300,52 @@ __device__ __forceinline__ int inverse_image(const Geo& g, int code) {
   return out;
 }
 
-// Candidate lists of the runs that rebuild.  Each node gathers the j within its own widened
-// support (|Hi rij|^2 <= ext^2 (1 + skin)^2), scanning the cells that support reaches (cells
-// may be smaller than the largest support when h varies; the stencil then grows per node).
-// A j whose own support does not reach i cannot find the pair itself, so i hands it the
-// mirrored entry (tlist, merged by k_candidates_merge).  Together each list holds exactly the
-// j with |Hi rij| or |Hj rij| within the widened extent.
+// Candidate lists of the runs that rebuild: each node gathers the j within its own widened
+// support (|Hi rij|^2 <= ext^2 (1 + skin)^2); see k_candidates for the cells it scans.
+//
 // The rebuilding runs' segments, listed as (first position, first index in the compacted
-// range [0, nBuild)) in increasing order of both: compacted index k -> particle position.
-__device__ __forceinline__ int build_pos(const int2* seg, int nseg, int k) {
+// range [0, nBuild)) in increasing order of both: compacted index k -> particle position, and
+// (with seg_of) the segment that holds it.
+__device__ __forceinline__ int seg_of(const int2* seg, int nseg, int k) {
   int lo = 0, hi = nseg - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (seg[mid].y <= k) lo = mid; else hi = mid - 1;
   }
-  return seg[lo].x + (k - seg[lo].y);
+  return lo;
+}
+__device__ __forceinline__ int build_pos(const int2* seg, int nseg, int k) {
+  const int s = seg_of(seg, nseg, k);
+  return seg[s].x + (k - seg[s].y);
 }
 
 // Launched over the rebuilding runs' particles only (seg, nBuild): a step that rebuilds a few
 // small runs of a large batch neither walks the others nor leaves the work to the one XCD whose
 // index range holds those runs.
+//
+// Cells.  A run whose h is near uniform has one grid of cells at least as wide as the widest
+// widened support, and each node scans the 3^D cells around its own (image cells through
+// periodic and reflecting planes).  Where h varies by 2x or more:
+//  * open boundaries: levels.  Level-l cells are at least support/2^(l+1) wide (l = 0..levels-1;
+//    the grid is that of the finest level, level-l cells are 2^(levels-1-l) finest cells per
+//    axis, so each is one contiguous range of the Morton-sorted particles, with a table per
+//    level).  A node works at the finest level whose cells are at least half its widened
+//    support and scans the 3 or 5 cells per axis that support reaches: dense regions scan cells
+//    of their own size, sparse ones cells of theirs.
+//  * periodic or reflecting axes: one grid, cells shrunk toward the smaller supports (by up to
+//    4x while the grid stays within ~8 cells per node), and each node scans the cells its own
+//    support reaches.
+// Every node collects each j within its own widened support; where j's support does not reach
+// i, i hands j the mirrored entry (tlist, merged by k_candidates_merge).  Together each list
+// holds exactly the j with |Hi rij| or |Hj rij| within the widened extent.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const RunGrid* grids, const int2* seg, int nseg,
                                                        int nBuild, const int* cstart, const int* cend, uint32_t* tlist,
                                                        int* tcnt, int tcap, int* overflow) {
   const int k = xcd_block()*blockDim.x + threadIdx.x;
   if (k >= nBuild) return;
-  const int i = build_pos(seg, nseg, k);
+  const int sg = seg_of(seg, nseg, k);
+  const int i = seg[sg].x + (k - seg[sg].y);
   const int r = f.run[i];
   const RunGrid gr = grids[r];
   double xi[D];
@@ -312,10 +360,6 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
   f.hb[i] = hi;
   const double ext2c = gr.ext2c;
   const double reach = sqrt(ext2c)/hi;
-  // stencil half-width per axis: the cells the node's own widened support reaches
-  int kw[3] = {0, 0, 0};
-#pragma unroll
-  for (int a = 0; a < D; ++a) kw[a] = max(1, (int)ceil(reach*gr.inv_cell[a]*(1.0 + 1.0e-12)));
   // FP32 distance pre-filter with a 1e-4 relative margin (far above its own rounding): only the
   // FP64 criterion decides membership
   const float reach2 = (float)(ext2c/(hi*hi)*(1.0 + 1.0e-4));
@@ -323,66 +367,104 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
   const int cap = f.ccap;
   int count = 0;
   uint32_t chunk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int kz = D >= 3 ? kw[2] : 0, ky = D >= 2 ? kw[1] : 0;
-  for (int dz = -kz; dz <= kz; ++dz)
-    for (int dy = -ky; dy <= ky; ++dy)
-      for (int dx = -kw[0]; dx <= kw[0]; ++dx) {
-        const int off[3] = {dx, dy, dz};
-        int cc[3] = {0, 0, 0};
-        int code = 0;
-        bool skip = false;
+  auto visit = [&](int j, int code) {
+    if (j == i && code == 0) return;
+    double xj[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-          int c = ci[a] + off[a];
-          const bool per = (g.permask >> a) & 1;
-          if (c < 0) {
-            if (per) { c += gr.nc[a]; code |= 1 << (2*a); }
-            else if ((g.reflmask >> (2*a)) & 1) { c = -c - 1; code |= 1 << (2*a); skip |= c >= gr.nc[a]; }
-            else skip = true;
-          } else if (c >= gr.nc[a]) {
-            if (per) { c -= gr.nc[a]; code |= 2 << (2*a); }
-            else if ((g.reflmask >> (2*a + 1)) & 1) { c = 2*gr.nc[a] - 1 - c; code |= 2 << (2*a); skip |= c < 0; }
-            else skip = true;
-          }
-          cc[a] = c;
-        }
-        if (skip) continue;
-        const int cell = lin_cell<D>(gr, cc);
-        const int q1 = cend[cell];
-        for (int j = cstart[cell]; j < q1; ++j) {
-          if (j == i && code == 0) continue;
-          double xj[D];
+    for (int a = 0; a < D; ++a) xj[a] = f.x[a][j];
+    if (code) image_pos<D>(g, code, xj);
+    float r2 = 0.0f;
 #pragma unroll
-          for (int a = 0; a < D; ++a) xj[a] = f.x[a][j];
-          if (code) image_pos<D>(g, code, xj);
-          float r2 = 0.0f;
+    for (int a = 0; a < D; ++a) { const float d = (float)(xi[a] - xj[a]); r2 += d*d; }
+    if (r2 > reach2) return;
+    const double hj = f.hinv[j];
+    double e2i = 0.0, e2j = 0.0;
 #pragma unroll
-          for (int a = 0; a < D; ++a) { const float d = (float)(xi[a] - xj[a]); r2 += d*d; }
-          if (r2 > reach2) continue;
-          const double hj = f.hinv[j];
-          double e2i = 0.0, e2j = 0.0;
+    for (int a = 0; a < D; ++a) {
+      const double rr = dsub(xi[a], xj[a]);
+      const double ei = dmul(hi, rr), ej = dmul(hj, rr);
+      e2i = a == 0 ? dmul(ei, ei) : dadd(e2i, dmul(ei, ei));
+      e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
+    }
+    if (!(e2i <= ext2c)) return;
+    // write-combine 8 entries in registers, flush them as one 32-byte chunk
+    const uint32_t e = (uint32_t)j | ((uint32_t)code << kImgShift);
+    const int s = count & 7;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) chunk[u] = s == u ? e : chunk[u];
+    ++count;
+    if (s == 7 && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
+    if (!(e2j <= ext2c)) {
+      // j's own support does not reach i: j receives the pair from here
+      const int slot = atomicAdd(&tcnt[j], 1);
+      if (slot < tcap) tlist[ell_at(slot, j, ld)] = (uint32_t)i | ((uint32_t)inverse_image(g, code) << kImgShift);
+    }
+  };
+  if (gr.levels > 1) {
+    // the node's level: the finest whose cells (at least support/2^(l+1) wide) are at least half
+    // its reach, so it scans 3 or 5 cells per axis
+    int lv = 0;
+    double w = 0.5*gr.support;
+    while (lv + 1 < gr.levels && w >= reach) { ++lv; w *= 0.5; }
+    const int sh = gr.levels - 1 - lv;
+    int cl[3] = {0, 0, 0}, ncl[3] = {1, 1, 1}, kl[3] = {0, 0, 0};
+    level_cells<D>(gr, lv, ncl);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      cl[a] = ci[a] >> sh;
+      kl[a] = max(1, (int)ceil(reach*gr.inv_cell[a]/(double)(1 << sh)*(1.0 + 1.0e-12)));
+    }
+    const int kz = D >= 3 ? kl[2] : 0, ky = D >= 2 ? kl[1] : 0;
+    for (int dz = -kz; dz <= kz; ++dz)
+      for (int dy = -ky; dy <= ky; ++dy)
+        for (int dx = -kl[0]; dx <= kl[0]; ++dx) {
+          const int off[3] = {dx, dy, dz};
+          int cc[3] = {0, 0, 0};
+          bool skip = false;
 #pragma unroll
           for (int a = 0; a < D; ++a) {
-            const double rr = dsub(xi[a], xj[a]);
-            const double ei = dmul(hi, rr), ej = dmul(hj, rr);
-            e2i = a == 0 ? dmul(ei, ei) : dadd(e2i, dmul(ei, ei));
-            e2j = a == 0 ? dmul(ej, ej) : dadd(e2j, dmul(ej, ej));
+            cc[a] = cl[a] + off[a];
+            skip |= cc[a] < 0 || cc[a] >= ncl[a];
           }
-          if (!(e2i <= ext2c)) continue;
-          // write-combine 8 entries in registers, flush them as one 32-byte chunk
-          const uint32_t e = (uint32_t)j | ((uint32_t)code << kImgShift);
-          const int s = count & 7;
-#pragma unroll
-          for (int u = 0; u < 8; ++u) chunk[u] = s == u ? e : chunk[u];
-          ++count;
-          if (s == 7 && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
-          if (!(e2j <= ext2c)) {
-            // j's own support does not reach i: j receives the pair from here
-            const int slot = atomicAdd(&tcnt[j], 1);
-            if (slot < tcap) tlist[ell_at(slot, j, ld)] = (uint32_t)i | ((uint32_t)inverse_image(g, code) << kImgShift);
-          }
+          if (skip) continue;
+          const int cell = lin_level_cell<D>(gr, lv, ncl, cc);
+          const int q1 = cend[cell];
+          for (int j = cstart[cell]; j < q1; ++j) visit(j, 0);
         }
-      }
+  } else {
+    // stencil half-width per axis: the cells the node's own widened support reaches
+    int kw[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) kw[a] = max(1, (int)ceil(reach*gr.inv_cell[a]*(1.0 + 1.0e-12)));
+    const int kz = D >= 3 ? kw[2] : 0, ky = D >= 2 ? kw[1] : 0;
+    for (int dz = -kz; dz <= kz; ++dz)
+      for (int dy = -ky; dy <= ky; ++dy)
+        for (int dx = -kw[0]; dx <= kw[0]; ++dx) {
+          const int off[3] = {dx, dy, dz};
+          int cc[3] = {0, 0, 0};
+          int code = 0;
+          bool skip = false;
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            int c = ci[a] + off[a];
+            const bool per = (g.permask >> a) & 1;
+            if (c < 0) {
+              if (per) { c += gr.nc[a]; code |= 1 << (2*a); }
+              else if ((g.reflmask >> (2*a)) & 1) { c = -c - 1; code |= 1 << (2*a); skip |= c >= gr.nc[a]; }
+              else skip = true;
+            } else if (c >= gr.nc[a]) {
+              if (per) { c -= gr.nc[a]; code |= 2 << (2*a); }
+              else if ((g.reflmask >> (2*a + 1)) & 1) { c = 2*gr.nc[a] - 1 - c; code |= 2 << (2*a); skip |= c < 0; }
+              else skip = true;
+            }
+            cc[a] = c;
+          }
+          if (skip) continue;
+          const int cell = lin_cell<D>(gr, cc);
+          const int q1 = cend[cell];
+          for (int j = cstart[cell]; j < q1; ++j) visit(j, code);
+        }
+  }
   if ((count & 7) && count <= cap) flush_chunk(f.cand, count - 1, i, ld, chunk);
   f.ccnt[i] = count;
   if (count > cap) atomicMax(overflow, count);
