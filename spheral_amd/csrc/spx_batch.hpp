@@ -761,7 +761,7 @@ class Batch {
           if (!std::isfinite(lo) || !std::isfinite(hi)) raise(SPX_ERR_NUMERIC, "non-finite position on the device");
           const int blo = geo.bc_lo[a], bhi = geo.bc_hi[a];
           const double L = cfg.xmax[a] - cfg.xmin[a];
-          const double target = lev > 1 ? support/(double)(2 << (lev - 1)) : support/F;
+          const double target = lev > 1 ? support/(double)(4 << (lev - 1)) : support/F;
           int64_t nc;
           double cell, origin;
           if (blo == BC_PERIODIC) {

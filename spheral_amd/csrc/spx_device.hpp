@@ -118,7 +118,7 @@ struct RunGrid {
   double support;      // the largest widened support radius
   int nc[3];           // cells per axis (of the finest level when levels > 1)
   int cellBase;        // first cell of the run's table (of level 0 when levels > 1)
-  int levels;          // > 1: cell levels, level l at least support/2^(l+1) wide
+  int levels;          // > 1: cell levels, level l at least support/2^(l+2) wide
   int levelBase[kCellLevels];   // first cell of each level's table
 };
 

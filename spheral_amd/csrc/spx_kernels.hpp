@@ -326,12 +326,12 @@ __device__ __forceinline__ int build_pos(const int2* seg, int nseg, int k) {
 // Cells.  A run whose h is near uniform has one grid of cells at least as wide as the widest
 // widened support, and each node scans the 3^D cells around its own (image cells through
 // periodic and reflecting planes).  Where h varies by 2x or more:
-//  * open boundaries: levels.  Level-l cells are at least support/2^(l+1) wide (l = 0..levels-1;
+//  * open boundaries: levels.  Level-l cells are at least support/2^(l+2) wide (l = 0..levels-1;
 //    the grid is that of the finest level, level-l cells are 2^(levels-1-l) finest cells per
 //    axis, so each is one contiguous range of the Morton-sorted particles, with a table per
-//    level).  A node works at the finest level whose cells are at least half its widened
-//    support and scans the 3 or 5 cells per axis that support reaches: dense regions scan cells
-//    of their own size, sparse ones cells of theirs.
+//    level).  A node works at the finest level l with support/2^l >= its widened support, whose
+//    cells are at least a quarter of that support, and scans the 3 to 9 cells per axis the
+//    support reaches: dense regions scan cells of their own size, sparse ones cells of theirs.
 //  * periodic or reflecting axes: one grid, cells shrunk toward the smaller supports (by up to
 //    4x while the grid stays within ~8 cells per node), and each node scans the cells its own
 //    support reaches.
@@ -401,8 +401,8 @@ __global__ void __launch_bounds__(kBlock) k_candidates(Fields f, Geo g, const Ru
     }
   };
   if (gr.levels > 1) {
-    // the node's level: the finest whose cells (at least support/2^(l+1) wide) are at least half
-    // its reach, so it scans 3 or 5 cells per axis
+    // the node's level: the finest l with support/2^l >= its reach; those cells are at least
+    // support/2^(l+2) wide, a quarter of the reach or more, so it scans 3 to 9 cells per axis
     int lv = 0;
     double w = 0.5*gr.support;
     while (lv + 1 < gr.levels && w >= reach) { ++lv; w *= 0.5; }
