@@ -218,7 +218,8 @@ __global__ void __launch_bounds__(kBlock) k_tree_pack(TreeView v, const double* 
 // Lanes that walk the tree together (64: the wavefront; 16, 8, 4: aligned lane groups).
 // Measured on the Evrard collapse (250k points) with the cells read as separate arrays: 64 ->
 // 5.0 ms per walk, 16 -> 4.15, 8 -> 3.99, 4 -> 4.69; with the packed cell records (gravity per
-// step over 60 steps from the start): 16 -> 2.16 ms, 8 -> 1.99, 4 -> 1.86.  Same bits for all.
+// step over 60 steps from the start): 16 -> 2.16 ms, 8 -> 1.99, 4 -> 1.86, 2 -> 1.92, 1 -> 2.14.
+// Same bits for all.
 constexpr int kTreeGroup = 4;
 
 // Device-library wavefront minimum (DPP lane exchanges, no LDS round trip).
