@@ -560,6 +560,16 @@ def main():
     line["step_stats"] = run.step_stats()
     line["device_bytes"] = run.device_bytes()
     line["device_bytes_per_particle"] = run.device_bytes()/n
+    # The end-of-step grad-h Omega (SPHBase::postStateUpdate) is computed when first read: the
+    # next step never reads it, and the state gives the same bits either way.  The rate with it
+    # computed at every step, as the reference's integrator does (timing on, as above):
+    k2 = max(10, args.steps//4)
+    run.set_end_of_step_omega(True)
+    t0 = time.perf_counter()
+    for _ in range(k2):
+        run.step()
+    line["value_eager_end_of_step_omega"] = n*k2/(time.perf_counter() - t0)
+    run.set_end_of_step_omega(False)
     # The timed window may hold few candidate-list rebuilds (none at 20 steps); the rate over whole
     # rebuild cycles: from the step after one rebuild through the step of the second rebuild after it.
     amort = None if args.no_rebuild_window else rebuild_amortised(run, n)

@@ -303,6 +303,14 @@ int spx_run_gravity_interactions(spx_run* run, int64_t* out2);
    step eagerly (and drops the captured graphs).  TreeGravity, the Limited Q and
    SynchronousRK2 steps always run eagerly. */
 int spx_run_set_step_graphs(spx_run* run, int32_t enable);
+/* No counterpart in the reference (an evaluation-schedule switch).  The grad-h Omega that
+   SPHBase::postStateUpdate leaves at the end of a CheapSynchronousRK2 step (SPHBase.cc:526-551)
+   is read by nothing in the next step (which recomputes it at its midpoint), so by default the
+   engine computes it when it is first needed -- a state read of "grad h corrections", a package
+   hook, SynchronousRK2, a run finishing in a batch -- from the same end-of-step positions and
+   neighbour lists: the same bits whenever it is read.  eager != 0 computes it at the end of
+   every step, as the reference's integrator does. */
+int spx_run_set_end_of_step_omega(spx_run* run, int32_t eager);
 /* Neighbour-group sizes of the filtering sum-density pass, the grad-h passes and the
    compatible-energy update (sizes[0..3), each 4 or 8: how many neighbours' loads a lane
    issues before using them).  Every size gives the same result bits.  By default (sizes null

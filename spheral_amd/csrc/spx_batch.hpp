@@ -160,6 +160,12 @@ class Batch {
   int64_t nTotal = 0;     // particles resident (active, done, removed-but-not-yet-dropped)
   bool initialized = false, connValid = false, wrapPending = false;
   bool orderDirty = false;  // a run was added, finished or removed since the last reorder
+  // The end-of-step grad-h Omega (SPHBase::postStateUpdate) of the last CheapSynchronousRK2 step
+  // is not yet computed: nothing in the next such step reads it, so it is computed only when
+  // something else needs it first (ensureOmega), from the end-of-step positions and the step's
+  // neighbour lists, exactly as at the end of the step.
+  bool omegaStale = false;
+  bool eagerOmega = false;           // compute it at the end of every step anyway
   long long inPlaceRetirements = 0;   // steps whose finished runs retired without a reorder
   long long fullReorders = 0;         // reorders of the whole batch (composition changes)
   int64_t candBuilds = 0;   // candidate-list builds, summed over runs
@@ -1034,6 +1040,7 @@ class Batch {
   // resize buffers).  Leaves connValid false.
   void connectivityPrep(bool force) {
     if (nTotal == 0) raise(SPX_ERR_STATE, "no state uploaded");
+    ensureOmega();                   // a CheapSynchronousRK2 step drops it first (it never reads it)
     npairsCache = -1;                // every pass re-derives the neighbour lists
     bool any = false;
     for (int r = 0; r < maxRuns; ++r) {
@@ -1192,6 +1199,11 @@ class Batch {
     hipLaunchKernelGGL(k_eos, dim3(nblocks(nActive)), dim3(kBlock), 0, stream, f, ph, (int)nActive);
     HIP_CHECK(hipGetLastError());
   }
+  void ensureOmega() {
+    if (!omegaStale) return;
+    omegaStale = false;
+    omegaPass(false);
+  }
   void omegaPass(bool mid, const int* runMask = nullptr) {
     if (nActive == 0) return;
     if (!cfg.gradh_correction) {
@@ -1201,10 +1213,10 @@ class Batch {
     const int nb = (int)xcd_grid(nActive, kBlock);
     // midpoint: the record k_midstep wrote; state: the connectivity record (k_pack), or the
     // end-of-step position k_full_update left in it
-    tuneMark(mid ? 2 : 4);
+    if (mid) tuneMark(2);
     if (groupSize(1) == 8) launchOmega<8>(mid, nb, runMask);
     else launchOmega<4>(mid, nb, runMask);
-    tuneMark(mid ? 3 : 5);
+    if (mid) tuneMark(3);
   }
   template <int G>
   void launchOmega(bool mid, int nb, const int* runMask) {
@@ -1265,7 +1277,7 @@ class Batch {
       return t;
     };
     ms[0] = el(0, 1);
-    ms[1] = el(2, 3) + el(4, 5);
+    ms[1] = el(2, 3);
     ms[2] = el(6, 7);
     for (int p = 0; p < 3; ++p) tuner.best[p][v] = std::min(tuner.best[p][v], ms[p]);
     if (++tuner.phase < GroupTuner::kSteps) return;
@@ -1304,6 +1316,7 @@ class Batch {
   }
   // Packed neighbour records and nodeScale of the current state (active runs).
   void pack() {
+    ensureOmega();
     if (nActive == 0) return;
     launchD([&](auto d) {
       constexpr int DD = decltype(d)::value;
@@ -1378,6 +1391,7 @@ class Batch {
       HIP_CHECK(hipMemcpyAsync(f.potS, f.pot, nActive*sizeof(double), hipMemcpyDeviceToDevice, stream));
   }
   void copyStateToMid() {
+    ensureOmega();
     auto cp = [&](double* d, const double* s) {
       HIP_CHECK(hipMemcpyAsync(d, s, nActive*sizeof(double), hipMemcpyDeviceToDevice, stream));
     };
@@ -1546,6 +1560,7 @@ class Batch {
     if (!retire && !initialized) raise(SPX_ERR_STATE, "spx_run_initialize must be called before stepping");
     if (cfg.gravity_only) { stepSynchronousGravity(); return; }
     if (cfg.integrator == SPX_INTEGRATOR_SYNCHRONOUS_RK2) { stepSynchronousHydro(); return; }
+    omegaStale = false;              // this step never reads the last step's end-of-step Omega
     maybeStartTuning();
     if (graphsAllowed()) { graphStep(); return; }
     ++eagerSteps;
@@ -1640,6 +1655,8 @@ class Batch {
   // first), the end-of-step grad-h Omega and the time advance; shared by both integrators.
   void fullStep(int nb) {
     fullStepLaunches(nb);
+    omegaStale = cfg.gradh_correction != 0;
+    if (eagerOmega) ensureOmega();
     download(hsc, dsc, sizeof(Scalars)*maxRuns);
     collectTimes();
     finishTuningStep();
@@ -1670,14 +1687,17 @@ class Batch {
       }
     }
     mark("energy_update");
-    omegaPass(false);
-    mark("omega");
     wrapPending = true;
     hipLaunchKernelGGL(k_advance_time, dim3(nblocks(maxRuns)), dim3(kBlock), 0, stream, dsc, maxRuns);
     HIP_CHECK(hipGetLastError());
   }
   // Host bookkeeping once the step's run scalars are downloaded.
   void endOfStep() {
+    for (int r = 0; r < maxRuns; ++r)
+      if (runs[r].state == RUN_ACTIVE && retire && reached_end(hsc[r].t, hsc[r].tEnd)) {
+        ensureOmega();               // a finishing run's final state, before it leaves the active range
+        break;
+      }
     bool finished = false;
     for (int r = 0; r < maxRuns; ++r) {
       if (runs[r].state != RUN_ACTIVE) continue;
@@ -1776,6 +1796,8 @@ class Batch {
       launchSteadyStep();
       f = f0;
       for (int a = 0; a < D; ++a) std::swap(f.v[a], f.v1[a]);
+      omegaStale = cfg.gradh_correction != 0;
+      if (eagerOmega) ensureOmega();
       ++eagerSteps;
       connValid = true;
       npairsCache = -1;
@@ -1814,6 +1836,8 @@ class Batch {
     f = f0;
     HIP_CHECK(hipGraphLaunch(g->exec, stream));
     for (int a = 0; a < D; ++a) std::swap(f.v[a], f.v1[a]);
+    omegaStale = cfg.gradh_correction != 0;
+    if (eagerOmega) ensureOmega();
     ++graphReplays;
     connValid = true;
     npairsCache = -1;
@@ -2071,7 +2095,10 @@ class Batch {
     if (name == "H") return iso(f.hinv);
     if (name == "pressure") return scalar(f.P);
     if (name == "sound speed") return scalar(f.cs);
-    if (name == "grad h corrections") return scalar(f.omega);
+    if (name == "grad h corrections") {
+      ensureOmega();
+      return scalar(f.omega);
+    }
     if (name == "delta velocity hydro") return vec(f.DvDt);
     if (name == "delta velocity") return vec(f.DvDtG);
     if (name == "delta position") return vec(f.DxDt);

@@ -475,6 +475,11 @@ int spx_run_set_step_graphs(spx_run* run, int32_t enable) {
   });
 }
 
+int spx_run_set_end_of_step_omega(spx_run* run, int32_t eager) {
+  RUN_OR_FAIL(run);
+  return guarded([&] { run->impl->eagerOmega = eager != 0; });
+}
+
 int spx_run_set_group_sizes(spx_run* run, const int32_t* sizes) {
   RUN_OR_FAIL(run);
   return guarded([&] {

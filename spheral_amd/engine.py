@@ -26,7 +26,8 @@ EXPORTS = (
     "spx_batch_create", "spx_batch_destroy", "spx_batch_add_run", "spx_batch_remove_run", "spx_batch_step",
     "spx_batch_run_status", "spx_batch_get_field", "spx_batch_conservation", "spx_batch_usage",
     "spx_batch_kernel_times", "spx_batch_neighbor_builds", "spx_bandwidth_probe",
-    "spx_run_set_step_graphs", "spx_run_set_group_sizes", "spx_run_step_stats", "spx_run_gravity_interactions",
+    "spx_run_set_step_graphs", "spx_run_set_end_of_step_omega", "spx_run_set_group_sizes", "spx_run_step_stats",
+    "spx_run_gravity_interactions",
     "spx_batch_set_step_graphs", "spx_batch_set_group_sizes", "spx_batch_step_stats",
     "spx_run_device_bytes", "spx_batch_device_bytes",
 )
@@ -95,6 +96,7 @@ def lib():
             "spx_batch_usage": (C.c_int, [P, I64, I32]),
             "spx_batch_neighbor_builds": (C.c_int, [P, C.c_int32, I64]),
             "spx_run_set_step_graphs": (C.c_int, [P, C.c_int32]),
+            "spx_run_set_end_of_step_omega": (C.c_int, [P, C.c_int32]),
             "spx_run_gravity_interactions": (C.c_int, [P, I64]),
             "spx_run_set_group_sizes": (C.c_int, [P, I32]),
             "spx_run_step_stats": (C.c_int, [P, I64]),
@@ -300,6 +302,11 @@ class Run:
     def set_step_graphs(self, enable):
         """Replay steady CheapSynchronousRK2 steps from captured HIP graphs (default on)."""
         _check(lib().spx_run_set_step_graphs(self.h, 1 if enable else 0))
+
+    def set_end_of_step_omega(self, eager):
+        """Compute the end-of-step grad-h Omega at every step (eager) or when it is first read
+        (default); the results are the same bits either way."""
+        _check(lib().spx_run_set_end_of_step_omega(self.h, 1 if eager else 0))
 
     def set_group_sizes(self, sizes=None):
         """Fix the neighbour-group sizes (filter, grad-h, update; each 4 or 8), or None: measure."""
