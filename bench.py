@@ -445,6 +445,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rebuild-window", action="store_true",
                     help="skip the rate over whole candidate-rebuild cycles (profiling runs)")
+    ap.add_argument("--no-eager-omega", action="store_true",
+                    help="skip the rate with the end-of-step grad-h Omega computed every step (profiling runs)")
     ap.add_argument("--integrator", choices=["cheap", "synchronous"], default="cheap",
                     help="taylor_green: CheapSynchronousRK2 (the headline) or SynchronousRK2")
     ap.add_argument("--workload", choices=["taylor_green", "ensemble", "evrard", "serve", "accuracy"],
@@ -563,13 +565,14 @@ def main():
     # The end-of-step grad-h Omega (SPHBase::postStateUpdate) is computed when first read: the
     # next step never reads it, and the state gives the same bits either way.  The rate with it
     # computed at every step, as the reference's integrator does (timing on, as above):
-    k2 = max(10, args.steps//4)
-    run.set_end_of_step_omega(True)
-    t0 = time.perf_counter()
-    for _ in range(k2):
-        run.step()
-    line["value_eager_end_of_step_omega"] = n*k2/(time.perf_counter() - t0)
-    run.set_end_of_step_omega(False)
+    if not args.no_eager_omega:
+        k2 = max(10, args.steps//4)
+        run.set_end_of_step_omega(True)
+        t0 = time.perf_counter()
+        for _ in range(k2):
+            run.step()
+        line["value_eager_end_of_step_omega"] = n*k2/(time.perf_counter() - t0)
+        run.set_end_of_step_omega(False)
     # The timed window may hold few candidate-list rebuilds (none at 20 steps); the rate over whole
     # rebuild cycles: from the step after one rebuild through the step of the second rebuild after it.
     amort = None if args.no_rebuild_window else rebuild_amortised(run, n)
