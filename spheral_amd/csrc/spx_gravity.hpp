@@ -218,8 +218,9 @@ __global__ void __launch_bounds__(kBlock) k_tree_pack(TreeView v, const double* 
 // Lanes that walk the tree together (64: the wavefront; 16, 8, 4: aligned lane groups).
 // Measured on the Evrard collapse (250k points) with the cells read as separate arrays: 64 ->
 // 5.0 ms per walk, 16 -> 4.15, 8 -> 3.99, 4 -> 4.69; with the packed cell records (gravity per
-// step over 60 steps from the start): 16 -> 2.16 ms, 8 -> 1.99, 4 -> 1.86, 2 -> 1.92, 1 -> 2.14.
-// Same bits for all.
+// step over 60 steps from the start): 16 -> 2.16 ms, 8 -> 1.99, 4 -> 1.86, 2 -> 1.92, 1 -> 2.14;
+// with single-point leaves taken through the monopole path (one branch, no point-record read):
+// 8 -> 1.72 ms, 4 -> 1.59, 2 -> 1.60.  Same bits for every group size.
 constexpr int kTreeGroup = 4;
 
 // Device-library wavefront minimum (DPP lane exchanges, no LDS round trip).
@@ -286,20 +287,33 @@ __global__ void __launch_bounds__(kBlock) k_tree_walk(
     if (next != c) continue;
     const double dx = ccx - xi, dy = ccy - yi, dz = ccz - zi;
     const double r2 = dx*dx + dy*dy + dz*dz;
-    if (r2 > open) {
+    // An accepted cell acts through its monopole.  A single-point leaf acts as its point, and
+    // its record already holds that point: the monopole pass copies a one-point cell's x and m
+    // unchanged (no m*x/m round trip), so (ccx, ccy, ccz, Mc) are the point's own values and
+    // the point-point term below is the monopole formula with the point's softening guard.
+    // One code path for both: no second, dependent read of the point record, and lanes of a
+    // wavefront at accepted cells and at single-point leaves no longer run two branches (gravity
+    // 1.86 -> 1.59 ms per step on the 250k Evrard collapse; results move at roundoff, 7e-15 in
+    // density after 65 steps, as the compiler contracts the merged arithmetic differently).
+    const bool mono = r2 > open;
+    const bool single = !mono && q0 == q1;
+    if (mono || (single && r2/soft2 > 1.0e-10)) {
       // 1/|r| and 1/sqrt(r^2 + s^2) from reciprocal square roots (the force is M/(r^2 + s^2) along
       // r/|r|, the potential -M/sqrt(r^2 + s^2)): 11 % faster than two square roots and three
       // divisions on the 250k Evrard walk, results within 2e-16 (L1 of density after 65 steps)
-      const double inv = r2 > 1.0e-100 ? rsqrt(r2) : 0.0;
-      const double nx = r2 > 1.0e-100 ? dx*inv : 1.0, ny = dy*inv, nz = dz*inv;
+      const bool far = single || r2 > 1.0e-100;
+      const double inv = far ? rsqrt(r2) : 0.0;
+      const double nx = far ? dx*inv : 1.0, ny = dy*inv, nz = dz*inv;
       const double rs = r2 + soft2;
       const double ri = rsqrt(rs);
       const double f = DIM == 3 ? Mc*(ri*ri) : Mc*ri;
       ax += f*nx; ay += f*ny; az += f*nz;
       phi -= DIM == 3 ? Mc*ri : Mc*(0.5*log(rs));
-      if (COUNT) ++nCell;
+      if (COUNT) { if (mono) ++nCell; else ++nPoint; }
+    }
+    if (mono || single) {
       next = sk;
-    } else if (q0 == q1 || lvl == kTreeBits) {
+    } else if (lvl == kTreeBits) {
       for (int q = q0; q <= q1; ++q) {
         const double2 pa = tp[2*(size_t)q], pb = tp[2*(size_t)q + 1];
         const double ex = pa.x - xi, ey = pa.y - yi, ez = pb.x - zi;
