@@ -46,3 +46,29 @@ def test_two_body_orbit_matches_oracle(gpu, oracle_mod):
         n += 1
     assert g.time()[1] == n
     assert np.abs(g.field("position") - o.field("position")).max() < 1e-9
+
+
+def test_coincident_points_share_a_deepest_leaf(gpu, oracle_mod):
+    """Pairs of points at one position: the octree lists both in one cell at its deepest level (the
+    walk's multi-point leaf loop, and a monopole pass over every level down to it), and each point
+    skips the other's term by the softening guard, as the oracle's walk does."""
+    print("[case] collisionless sphere with coincident pairs, 10 steps", flush=True)
+    cfg, st, _ = problems.collisionless_sphere_3d()
+    n = len(st["m"])
+    for k in range(0, n - 1, 97):
+        for key in ("x", "v"):
+            st[key][k + 1] = st[key][k]
+    g = gpu.make_run(cfg, st)
+    g.initialize()
+    o = oracle_mod.OracleRun(cfg, n)
+    o.set_state(**st)
+    o.initialize()
+    for k in range(10):
+        dg = g.step(1.0e40)
+        do = o.step(1.0e40)
+        assert abs(dg - do) <= 1e-10 * do, k
+    for f in ("position", "velocity", "gravitational potential"):
+        a, b = g.field(f), o.field(f)
+        assert np.all(np.abs(a - b) <= 1e-9 * np.abs(b).max()), f
+    x = g.field("position")
+    assert np.array_equal(x[1:n:97][:len(range(0, n - 1, 97))], x[0:n - 1:97])
