@@ -213,6 +213,10 @@ class Batch {
   int* dOverflowG = nullptr;  // the filter's overflow flag in graph-replayed steps
   // per-class kernel time (ms) summed over the steps since timing was enabled
   bool timing = false;
+  // TreeGravity's own stream and its fork / join events (gravity runs only)
+  hipStream_t gstream = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
+  bool gravPending = false;
   std::vector<std::string> tnames;
   std::vector<double> tms;
   std::vector<hipEvent_t> evPool;       // reused across steps (recording costs no allocation)
@@ -234,6 +238,13 @@ class Batch {
     while ((1 << runBits) < maxRuns) ++runBits;
     allocate();
     grav.init(D, cap, ld, stream, cfg.gravity != 0);
+    if (cfg.gravity) {
+      // the midpoint tree build and walk run on a stream of their own (default priority) beside
+      // the midpoint grad-h and SPH passes, which neither read nor write what they use
+      HIP_CHECK(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
+      HIP_CHECK(hipEventCreateWithFlags(&evFork, hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&evJoin, hipEventDisableTiming));
+    }
   }
   ~Batch() {
     (void)hipSetDevice(device);
@@ -245,7 +256,11 @@ class Batch {
     for (void* p : {(void*)hgrids, (void*)hsc, (void*)hext, (void*)hstate, (void*)hinit, (void*)hOverflow,
                     (void*)htiles, (void*)hbuild, (void*)hdelta, (void*)hsub})
       if (p) (void)hipHostFree(p);
+    if (gstream) (void)hipStreamSynchronize(gstream);
     grav.release();
+    if (evFork) (void)hipEventDestroy(evFork);
+    if (evJoin) (void)hipEventDestroy(evJoin);
+    if (gstream) (void)hipStreamDestroy(gstream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1381,14 +1396,33 @@ class Batch {
   // toState: the evaluation's potential also becomes the State's (potS), as it does for the
   // start-up dry run, a standalone evaluation and SynchronousRK2's start-of-step evaluation; a
   // midpoint evaluation's potential is replaced by the state0 copy's in the reference.
-  void gravityPass(bool mid, bool toState = false) {
+  // forked: the gravity stream has been forked from the main stream by forkGravity(); the pass
+  // runs there, and joinGravity() makes the main stream wait for it.  Otherwise it runs in order
+  // on the main stream.
+  void gravityPass(bool mid, bool toState = false, bool forked = false) {
     if (!cfg.gravity || nActive == 0) return;
+    grav.stream = forked ? gstream : stream;
     double* xs[3] = {mid ? f.xm[0] : f.x[0], mid ? f.xm[1] : f.x[1], mid ? f.xm[2] : f.x[2]};
     grav.build(xs, f.m, (int)nActive);
-    mark("tree_build");
+    if (!forked) mark("tree_build");
     grav.evaluate(f.DvDtG, f.pot, (int)nActive, cfg.G, cfg.softening, cfg.opening, dsc, cfg.ftimestep);
     if (toState)
-      HIP_CHECK(hipMemcpyAsync(f.potS, f.pot, nActive*sizeof(double), hipMemcpyDeviceToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(f.potS, f.pot, nActive*sizeof(double), hipMemcpyDeviceToDevice, grav.stream));
+    if (forked) {
+      HIP_CHECK(hipEventRecord(evJoin, gstream));
+      gravPending = true;
+      grav.stream = stream;
+    }
+  }
+  // The gravity stream starts from everything queued on the main stream so far.
+  void forkGravity() {
+    HIP_CHECK(hipEventRecord(evFork, stream));
+    HIP_CHECK(hipStreamWaitEvent(gstream, evFork, 0));
+  }
+  void joinGravity() {
+    if (!gravPending) return;
+    HIP_CHECK(hipStreamWaitEvent(stream, evJoin, 0));
+    gravPending = false;
   }
   void copyStateToMid() {
     ensureOmega();
@@ -1588,16 +1622,25 @@ class Batch {
     });
     HIP_CHECK(hipGetLastError());
     mark("update");
+    // TreeGravity at the midpoint reads the midpoint positions and the masses and writes the
+    // gravity acceleration, potential and dt vote, which only the full update reads: outside
+    // per-class timing it runs on its own stream beside the grad-h and SPH passes (forked here,
+    // its launches queued after the SPH pass's so the host's waits inside the tree build do not
+    // hold them back, joined before the update)
+    const bool overlap = cfg.gravity && !timing;
+    if (overlap) forkGravity();
     omegaPass(true);
     mark("omega");
     // derivatives at the midpoint
-    if (cfg.gravity) { gravityPass(true); mark("gravity"); }
+    if (cfg.gravity && !overlap) { gravityPass(true); mark("gravity"); }
     // the Limited Q reads its gradient (the previous evaluation's DvDx, ArtificialViscosityHandle::
     // postStateUpdate) while this evaluation writes DvDx; the end-of-step postStateUpdate then
     // copies the new DvDx into it, as the reference's state holds it after the step
     sphPass(false, true, true);
     copyQVelocityGradient();
+    if (overlap) gravityPass(true, false, true);
     mark("sph");
+    joinGravity();
     fullStep(nb);
   }
 
